@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Matrix-free Hamiltonian matvec throughput on one MI355X.
+
+A step is one y = H x over a whole momentum sector, x resident in HBM. Default workload: the
+6x6 mesh, N=12 sector (0,0) (34,770,492 states, 556 MB per complex vector). Prints one JSON
+line (metric, value in matvecs/s, roofline of the matvec kernel from HIP events on the
+library's stream, the CPU baseline on a bounded row sample, setup times). Progress goes to
+stderr.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|medium|small]
+                    [--solve] [--no-cpu-baseline] [--cpu-seconds S]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "momentumed.jl_amd"))
+
+METRIC = "Hamiltonian matvecs/s & HBM GB/s; batched sector-solves/s + p99 latency vs plain"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
+    """The oracle's row-gather matvec (restatement of linear_map.jl:100-118: every term tried
+    on every row, binary search for partners) on a random row sample, scaled to matvecs/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    D = len(basis)
+    rng = np.random.default_rng(2024)
+    n = min(D, 512)
+    rows = rng.choice(D, n, replace=False)
+    t0 = time.perf_counter()
+    yo = oracle.matvec(basis, full_terms, x_host, rows=rows, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    per_row = dt / n
+    n2 = int(min(D, max(n, seconds / max(per_row, 1e-9))))
+    if n2 > n:
+        rows = rng.choice(D, n2, replace=False) if n2 < D else np.arange(D)
+        t0 = time.perf_counter()
+        yo = oracle.matvec(basis, full_terms, x_host, rows=rows, nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        n = n2
+    rows_per_s = n / dt
+    err = float(np.abs(y_dev_host[rows] - yo).max() / max(np.abs(yo).max(), 1e-300))
+    return {
+        "value": rows_per_s / D, "unit": "matvec/s", "cores": nthreads, "kind": "port",
+        "sample": f"{n} random rows of the {D}-state sector ({dt:.1f} s, OpenMP row-parallel "
+                  f"restatement of linear_map.jl:100-118, all full-list terms per row, "
+                  f"binary search), scaled to whole matvecs",
+        "rows_per_s": rows_per_s,
+    }, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="large", choices=["small", "medium", "large"])
+    ap.add_argument("--solve", action="store_true", help="also time the lowest-nev eigensolve")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one momentum "
+              "sector per GPU; see DESIGN.md for how sectors shard). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import momentumed as me
+    from momentumed import device as dev
+    from momentumed import workloads as W
+    from momentumed._lib import check
+
+    w = W.get(args.workload)
+    t0 = time.perf_counter()
+    op = w.operator()
+    t_terms = time.perf_counter() - t0
+    log(f"[bench] {w.name}: {len(op.scats)} upper terms ({t_terms:.2f} s)")
+    if not dev.activate_gpu_device("rocm"):
+        raise RuntimeError("no GPU visible")
+    ctx = dev.get_context(0)
+    sysd = dev.system_struct(w.para.k_list, w.para.Gk, w.para.Nc_mix, w.para.Nc_conserve,
+                             w.N_each)
+    sector = me.preparation._momentum_list(w.para, w.N_each).index(w.sector)
+    t0 = time.perf_counter()
+    basis = dev.DeviceBasis.enumerate(ctx, sysd, sector)
+    t_enum = time.perf_counter() - t0
+    D = basis.D
+    log(f"[bench] sector {w.sector}: D={D} enumerated in {t_enum:.3f} s, rank_mode={basis.rank_mode}")
+    t0 = time.perf_counter()
+    dop = dev.DeviceOperator.from_mboperator(ctx, op, w.para.bits)
+    t_op = time.perf_counter() - t0
+    nnz, nnz_off = ctypes.c_int64(), ctypes.c_int64()
+    check(ctx.L.edm_connectivity_totals(ctx.h, dop.h, basis.h, ctypes.byref(nnz),
+                                        ctypes.byref(nnz_off)), ctx.h)
+    log(f"[bench] S_full={dop.S_full} nnz={nnz.value} nnz_off={nnz_off.value} "
+        f"({nnz_off.value / D:.1f} off-diagonal per row)")
+
+    x = torch.empty(D, dtype=torch.complex128, device="cuda:0")
+    y = torch.empty_like(x)
+    check(ctx.L.edm_fill_random(ctx.h, x.data_ptr(), D, 12345, 1), ctx.h)
+    ctx.synchronize()
+    x /= torch.linalg.vector_norm(x)
+    torch.cuda.synchronize()
+    lm = dev.ROCmLinearMap(dop, basis)
+
+    stream = torch.cuda.ExternalStream(ctx.stream, device="cuda:0")
+    for _ in range(args.warmup):
+        lm.matvec_device(y.data_ptr(), x.data_ptr())
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        lm.matvec_device(y.data_ptr(), x.data_ptr())
+    ev1.record(stream)
+    ev1.synchronize()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms_events = ev0.elapsed_time(ev1)
+    ms_per = ms_events / args.steps
+    log(f"[bench] {args.steps} matvecs: {ms_per:.3f} ms each (events), wall {wall:.3f} s")
+
+    S = dop.S_full
+    B_mv = D * 40 + nnz_off.value * 16 + S * 40
+    achieved = B_mv / (ms_per * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_matvec_fast<RankLin>" if basis.rank_mode == 1 else "k_matvec_fast<RankSearch>",
+                "bytes_model": "D*40 + nnz_off*16 + S_full*40", "algorithmic_bytes": B_mv}
+    tr_path = os.path.join(ROOT, "profiles", f"matvec_{args.workload}_traffic.json")
+    if os.path.exists(tr_path):
+        tr = json.load(open(tr_path))
+        roofline["traffic"] = tr.get("hbm_bytes_per_launch")
+        roofline["traffic_source"] = os.path.relpath(tr_path, ROOT)
+
+    out = {
+        "metric": METRIC, "value": round(1e3 / ms_per, 3), "unit": "matvec/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": w.name, "mesh": [int(g) for g in w.para.Gk], "N": list(w.N_each),
+                   "sector": list(w.sector), "D": D, "S_full": S, "nnz_offdiag": nnz_off.value,
+                   "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "hbm_gbps_algorithmic": round(achieved, 1),
+        "setup": {"terms_s": round(t_terms, 3), "enumerate_s": round(t_enum, 3),
+                  "op_upload_s": round(t_op, 3), "rank_mode": basis.rank_mode},
+        "wall_s_timed": round(wall, 4),
+    }
+
+    if args.solve:
+        log(f"[bench] eigensolve nev={w.nev} krylovdim={w.krylovdim} tol={w.tol}")
+        ts = time.perf_counter()
+        vals, _, info = dev.krylov_map_solve(lm, w.nev, krylovdim=w.krylovdim, tol=w.tol,
+                                             want_vectors=False)
+        out["solve"] = {"seconds": round(time.perf_counter() - ts, 3), "eigenvalues": vals.tolist(),
+                        **info}
+        log(f"[bench] solve: {out['solve']}")
+
+    if not args.no_cpu_baseline:
+        import numpy as np  # noqa: F811
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        amp, o, i = op.arrays()
+        full = oracle.full_list(oracle.TermList(amp, o, i), True)
+        basis_h = basis.download()
+        x_h = x.cpu().numpy()
+        y_h = y.cpu().numpy()
+        nthreads = min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline on {nthreads} threads (~{args.cpu_seconds:.0f} s)")
+        cb, err = cpu_baseline(w, basis_h, full, x_h, y_h, args.cpu_seconds, nthreads)
+        out["cpu_baseline"] = cb
+        out["parity_sampled_rows_max_rel_err"] = err
+        log(f"[bench] cpu baseline {cb['value']:.3e} matvec/s; GPU/CPU rows max rel err {err:.2e}")
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * edm.h — C-ABI of libedm.so, the MI355X (gfx950) exact-diagonalisation engine for
+ * momentum-conserved fermion systems.
+ *
+ * The entry points replace the device plugin contract of MomentumED.jl
+ * (MomentumED/src/method/gpu_interface.jl:1-22 and its CUDA implementation
+ * MomentumED/ext/MomentumEDCUDAExt.jl). Each declaration names the reference interface
+ * it stands in for. Conventions:
+ *   - every function returns an int status: EDM_OK (0) or a negative EDM_E* code; the
+ *     message is available from edm_last_error(ctx) (or edm_last_error(NULL) for errors
+ *     raised before a context exists);
+ *   - host buffers are owned by the caller and copied; device handles (edm_basis,
+ *     edm_op) are owned by the library until destroyed;
+ *   - complex numbers are interleaved (re, im) doubles (Julia ComplexF64 / numpy
+ *     complex128 layout);
+ *   - basis states are uint64 occupation bitstrings, bit i-1 = orbital i
+ *     (EDCore/src/types/manybodystate_basis.jl:24-32,161-168);
+ *   - one HIP stream per context; calls on one context are not thread-safe, separate
+ *     contexts are independent.
+ */
+#ifndef EDM_H
+#define EDM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EDM_OK 0
+#define EDM_EINVAL -1    /* bad argument (ArgumentError / AssertionError analogue) */
+#define EDM_ENODEV -2    /* no usable GPU (gpu_interface.jl:63-80 _throw_gpu_unavailable) */
+#define EDM_ENOMEM -3    /* device allocation failed or HBM budget exceeded */
+#define EDM_EHIP -4      /* HIP runtime error */
+#define EDM_ENOCONV -5   /* "Krylov method fails. Cannot find N eigenvectors." (MomentumED.jl:466-468) */
+#define EDM_EDIM -6      /* DimensionMismatch (linear_map.jl:93-98) */
+
+typedef struct edm_ctx edm_ctx;
+typedef struct edm_basis edm_basis;
+typedef struct edm_op edm_op;
+
+/* Description of a momentum-conserved system and particle numbers
+ * (EDPara, MomentumED/src/preparation/init_parameter.jl:62-146, and the N_each_component
+ * argument of ED_momentum_subspaces, momentum_decomposition.jl:339-343). */
+typedef struct {
+    int32_t dim;             /* spatial dimension of momenta, 1..3 */
+    int32_t nk;              /* Nk = number of momenta (columns of k_list) */
+    int32_t nc_mix;          /* Nc_mix (components mixed by H) */
+    int32_t nc_conserve;     /* Nc_conserve (components with conserved particle number) */
+    const int64_t *k_list;   /* dim x Nk, column-major: k_list[d + dim*k] */
+    const int64_t *gk;       /* Gk[dim]; 0 = no periodicity in that direction */
+    const int64_t *n_each;   /* particle number per conserved component (>= 0) */
+} edm_system;
+
+typedef struct {
+    int32_t iterations;      /* restart cycles used */
+    int32_t matvecs;         /* operator applications */
+    int32_t converged;       /* number of wanted eigenpairs with residual <= tol */
+    int32_t krylovdim;       /* Krylov dimension actually used */
+    double max_residual;     /* largest residual norm among the returned pairs */
+    double seconds;          /* wall time of the solve */
+} edm_solve_info;
+
+/* ----------------------------------------------------------------- device / context */
+/* GPU_AVAILABLE[:rocm] (gpu_interface.jl:25-30; MomentumEDCUDAExt.jl:13-24):
+ * 1 if a gfx950-class device is usable, 0 otherwise. */
+int edm_device_available(void);
+
+/* activate_gpu_device + the extension's per-device state (gpu_interface.jl:41-60).
+ * hbm_budget_bytes = 0 means no budget beyond the device's memory. */
+int edm_ctx_create(int device, uint64_t hbm_budget_bytes, edm_ctx **out);
+/* release_gpu_memory(::Val{dev}, level) (gpu_interface.jl:87; MomentumEDCUDAExt.jl:43-47)
+ * for everything the context owns. */
+int edm_ctx_destroy(edm_ctx *ctx);
+const char *edm_last_error(const edm_ctx *ctx);
+/* Block until all work queued on the context's stream is complete. */
+int edm_synchronize(edm_ctx *ctx);
+/* The HIP stream (hipStream_t) the context launches on, for callers sharing device memory. */
+int edm_ctx_stream(edm_ctx *ctx, void **stream);
+/* Device bytes currently held by the context's handles. */
+int edm_ctx_memory(edm_ctx *ctx, uint64_t *used_bytes, uint64_t *budget_bytes);
+
+/* ----------------------------------------------------------------- basis (a1) */
+/* ED_momentum_subspaces (momentum_decomposition.jl:339-413): number of momentum sectors in
+ * momentum_list order (first coordinate fastest, :360-369), including empty ones. */
+int edm_num_sectors(const edm_system *sys, int64_t *nsec);
+/* Sector sizes of every sector, enumerated on the device (counts[nsec]). */
+int edm_sector_sizes(edm_ctx *ctx, const edm_system *sys, int64_t *counts);
+/* One sector's basis (ascending uint64 states) enumerated on the device by colex
+ * unranking + momentum filter + order-preserving compaction; `sector` is the index into
+ * momentum_list. The handle carries rank tables for partner lookup. */
+int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm_basis **out);
+/* HilbertSubspace(list) (EDCore/src/types/hilbert_subspace.jl:19-30) uploaded as-is
+ * (must be sorted ascending, unique). `sys` may be NULL; when given and the list is the
+ * complete sector `sector`, rank tables replace the binary search
+ * (hilbert_subspace.jl:67-72 semantics are kept either way). */
+int edm_basis_upload(edm_ctx *ctx, const uint64_t *states, int64_t D, int32_t nbits,
+                     const edm_system *sys, int64_t sector, edm_basis **out);
+int edm_basis_size(const edm_basis *b, int64_t *D);
+int edm_basis_download(const edm_basis *b, uint64_t *states_host);
+/* Device pointer to the D sorted states (valid until edm_basis_destroy). */
+int edm_basis_device_ptr(const edm_basis *b, const uint64_t **states_dev);
+/* 1 if partner indices come from rank tables, 0 if from binary search. */
+int edm_basis_rank_mode(const edm_basis *b, int32_t *mode);
+int edm_basis_destroy(edm_basis *b);
+
+/* ----------------------------------------------------------------- operator (a2-a5) */
+/* create_gpu_linearmap(A::LinearMap, ::Val{dev}) (gpu_interface.jl:85;
+ * MomentumEDCUDAExt.jl:94-113) from an MBOperator's term list
+ * (EDCore/src/types/operator.jl:24-32): S terms with amplitude amp[2S] (interleaved),
+ * creation mask out[S] and annihilation mask in[S]. When upper_hermitian != 0 the list is
+ * the upper triangle and the adjoint of every non-diagonal term is added
+ * (MomentumED/src/method/linear_map.jl:22-41). */
+int edm_op_upload(edm_ctx *ctx, int32_t nbits, const double *amp, const uint64_t *out,
+                  const uint64_t *in, int64_t S, int32_t upper_hermitian, edm_op **op);
+/* Number of terms in the expanded (full) list. */
+int edm_op_size(const edm_op *op, int64_t *S_full);
+/* The expanded list in (in, out) order, as linear_map.jl:22-41 sorts it. Arrays of
+ * length S_full; parity = parity_mask(out) xor parity_mask(in) (scattering.jl:7-18,60). */
+int edm_op_full_list(const edm_op *op, double *amp, uint64_t *out, uint64_t *in,
+                     uint64_t *parity);
+int edm_op_destroy(edm_op *op);
+
+/* ----------------------------------------------------------------- matvec (a6-a8) */
+/* (A::CuLinearMap)(y, x) (MomentumEDCUDAExt.jl:119-122 -> gpu_interface.jl:214-220),
+ * computing y = H x with the semantics of the CPU map (linear_map.jl:100-118); targets
+ * outside the basis are dropped. x_dev, y_dev are device pointers to D complex values
+ * (16-byte aligned); stream-ordered on the context's stream (no host sync). */
+int edm_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const void *x_dev,
+               void *y_dev);
+/* The adjoint map (AdjointLinearMap, linear_map.jl:120-139): y = H^dagger x. */
+int edm_matvec_adjoint(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const void *x_dev,
+                       void *y_dev);
+/* Host-buffer convenience: copies x in, runs edm_matvec, copies y out, synchronises. */
+int edm_matvec_host(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double *x,
+                    double *y);
+
+/* Connectivity of rows [row0, row0+nrows): which (row, column, term, sign) pairs contribute
+ * (linear_map.jl:109-113). counts[nrows] first; then, with rowptr[nrows+1] = exclusive prefix
+ * of counts, cols[], terms[] (index into edm_op_full_list order) and negs[] (1 = sign -1).
+ * Entries of a row are sorted by (column, term). Host buffers. */
+int edm_connectivity_counts(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
+                            int64_t nrows, int64_t *counts);
+int edm_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
+                     int64_t nrows, const int64_t *rowptr, int64_t *cols, int32_t *terms,
+                     int8_t *negs);
+/* Totals of the same connectivity over the whole basis, counted on the device: all
+ * contributing (row, term) pairs and the off-diagonal ones (target != row state). The
+ * off-diagonal count is the number of x gathers one matvec performs (the nnz_off of the
+ * algorithmic-bytes model). No reference counterpart: it sizes the roofline of a6. */
+int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t *nnz,
+                            int64_t *nnz_offdiag);
+
+/* ----------------------------------------------------------------- eigensolve (a9-a10) */
+/* krylov_map_solve(H::AbstractGPULinearMap, N; ishermitian=true, vec0, krylovdim, tol,
+ * maxiter) (gpu_interface.jl:92-94; MomentumEDCUDAExt.jl:159-179 -> KrylovKit.eigsolve(...,
+ * :SR)): the nev lowest eigenpairs by thick-restart Lanczos in complex FP64 with full
+ * re-orthogonalisation on the device. vec0 (host, D complex) may be NULL: then a seeded
+ * uniform [0,1)+i[0,1) start vector is used. vals[nev] ascending; vecs (host, nev*D
+ * complex, vector-major) may be NULL. Returns EDM_ENOCONV when fewer than nev pairs reach
+ * tol within maxiter restarts (vals/vecs still hold the best estimates). */
+int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
+                 int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
+                 const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+
+/* ----------------------------------------------------------------- device vector helpers */
+/* Fill D complex values at a device pointer: mode 0 = uniform [0,1) in re and im
+ * (rand(ComplexF64), linear_map.jl:187), mode 1 = standard complex normal. Deterministic in
+ * (seed, index). */
+int edm_fill_random(edm_ctx *ctx, void *v_dev, int64_t D, uint64_t seed, int32_t mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDM_H */

@@ -1,0 +1,458 @@
+// edm_api.cpp — extern "C" entry points of libedm.so (declared in include/edm.h).
+#include "edm_internal.h"
+
+#include <algorithm>
+#include <cstring>
+
+
+namespace {
+
+template <class F>
+int guarded(edm_ctx *ctx, F &&f)
+{
+    try {
+        int dev = ctx ? ctx->device : -1;
+        if (ctx) {
+            DeviceGuard g(dev);
+            return f();
+        }
+        return f();
+    } catch (const EdmError &e) {
+        if (ctx) ctx->last_error = e.msg;
+        edm_set_global_error(e.msg);
+        return e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->last_error = e.what();
+        edm_set_global_error(e.what());
+        return EDM_EINVAL;
+    }
+}
+
+void free_basis(edm_basis *b)
+{
+    if (!b) return;
+    edm_ctx *ctx = b->ctx;
+    ctx->free(b->d_states, (size_t)std::max<int64_t>(b->D, 1) * sizeof(uint64_t));
+    if (b->d_jlo) ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
+    if (b->d_jhi) ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
+    delete b;
+}
+
+void attach_rank(edm_basis *b)
+{
+    b->rank_mode = 0;
+    if (!build_rank_tables(b)) return;
+    if (verify_rank_tables(b)) {
+        b->rank_mode = 1;
+    } else {
+        b->ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
+        b->ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
+        b->d_jlo = b->d_jhi = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int edm_device_available(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return 1;
+}
+
+int edm_ctx_create(int device, uint64_t hbm_budget_bytes, edm_ctx **out)
+{
+    return guarded(nullptr, [&]() -> int {
+        EDM_REQUIRE(out != nullptr, EDM_EINVAL, "out is NULL");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            throw EdmError{EDM_ENODEV,
+                           "device=:rocm requires a functional AMD GPU; none is visible"};
+        }
+        EDM_REQUIRE(device >= 0 && device < n, EDM_ENODEV, "no such GPU index");
+        DeviceGuard g(device);
+        edm_ctx *c = new edm_ctx();
+        c->device = device;
+        c->budget = hbm_budget_bytes;
+        hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            throw EdmError{EDM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+        }
+        *out = c;
+        return EDM_OK;
+    });
+}
+
+int edm_ctx_destroy(edm_ctx *ctx)
+{
+    if (!ctx) return EDM_OK;
+    {
+        DeviceGuard g(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        if (ctx->d_binom) ctx->free(ctx->d_binom, 65 * 65 * sizeof(uint64_t));
+        if (ctx->d_partials) ctx->free(ctx->d_partials, ctx->partials_bytes);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return EDM_OK;
+}
+
+const char *edm_last_error(const edm_ctx *ctx)
+{
+    return ctx ? ctx->last_error.c_str() : edm_global_error();
+}
+
+int edm_synchronize(edm_ctx *ctx)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx, EDM_EINVAL, "ctx is NULL");
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return EDM_OK;
+    });
+}
+
+int edm_ctx_stream(edm_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return EDM_EINVAL;
+    *stream = (void *)ctx->stream;
+    return EDM_OK;
+}
+
+int edm_ctx_memory(edm_ctx *ctx, uint64_t *used, uint64_t *budget)
+{
+    if (!ctx) return EDM_EINVAL;
+    if (used) *used = ctx->used;
+    if (budget) *budget = ctx->budget;
+    return EDM_OK;
+}
+
+int edm_num_sectors(const edm_system *sys, int64_t *nsec)
+{
+    return guarded(nullptr, [&]() -> int {
+        SystemSpec sp = SystemSpec::from(sys);
+        EDM_REQUIRE(nsec, EDM_EINVAL, "nsec is NULL");
+        *nsec = sp.nsec;
+        return EDM_OK;
+    });
+}
+
+int edm_sector_sizes(edm_ctx *ctx, const edm_system *sys, int64_t *counts)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && counts, EDM_EINVAL, "NULL argument");
+        SystemSpec sp = SystemSpec::from(sys);
+        std::vector<int64_t> c;
+        enum_sector_counts(ctx, sp, c);
+        std::copy(c.begin(), c.end(), counts);
+        return EDM_OK;
+    });
+}
+
+int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm_basis **out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && out, EDM_EINVAL, "NULL argument");
+        *out = nullptr;
+        SystemSpec sp = SystemSpec::from(sys);
+        EDM_REQUIRE(sector >= 0 && sector < sp.nsec, EDM_EINVAL, "sector index out of range");
+        edm_basis *b = new edm_basis();
+        b->ctx = ctx;
+        b->nbits = sp.nbits;
+        b->has_spec = true;
+        b->spec = sp;
+        b->sector = sector;
+        try {
+            b->d_states = enum_sector_states(ctx, sp, sector, &b->D);
+            attach_rank(b);
+        } catch (...) {
+            free_basis(b);
+            throw;
+        }
+        *out = b;
+        return EDM_OK;
+    });
+}
+
+int edm_basis_upload(edm_ctx *ctx, const uint64_t *states, int64_t D, int32_t nbits,
+                     const edm_system *sys, int64_t sector, edm_basis **out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && out, EDM_EINVAL, "NULL argument");
+        *out = nullptr;
+        EDM_REQUIRE(D >= 0 && (D == 0 || states), EDM_EINVAL, "bad basis array");
+        EDM_REQUIRE(nbits >= 0 && nbits <= 64, EDM_EINVAL, "nbits must be 0..64");
+        uint64_t lim = nbits >= 64 ? ~0ULL : ((1ULL << nbits) - 1ULL);
+        for (int64_t i = 0; i < D; ++i) {
+            EDM_REQUIRE((states[i] & ~lim) == 0, EDM_EINVAL,
+                        "State representation out of range for given bits");
+            if (i) EDM_REQUIRE(states[i - 1] < states[i], EDM_EINVAL,
+                               "basis must be sorted ascending without repeats");
+        }
+        edm_basis *b = new edm_basis();
+        b->ctx = ctx;
+        b->D = D;
+        b->nbits = nbits;
+        try {
+            if (sys) {
+                b->spec = SystemSpec::from(sys);
+                EDM_REQUIRE(b->spec.nbits == nbits, EDM_EINVAL, "nbits differs from the system's");
+                b->has_spec = true;
+                b->sector = sector;
+            }
+            b->d_states = (uint64_t *)ctx->alloc((size_t)std::max<int64_t>(D, 1) * sizeof(uint64_t));
+            if (D)
+                EDM_HIP_CHECK(hipMemcpyAsync(b->d_states, states, D * sizeof(uint64_t),
+                                             hipMemcpyHostToDevice, ctx->stream));
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            if (sys && sector >= 0) attach_rank(b);
+        } catch (...) {
+            free_basis(b);
+            throw;
+        }
+        *out = b;
+        return EDM_OK;
+    });
+}
+
+int edm_basis_size(const edm_basis *b, int64_t *D)
+{
+    if (!b || !D) return EDM_EINVAL;
+    *D = b->D;
+    return EDM_OK;
+}
+
+int edm_basis_download(const edm_basis *b, uint64_t *states_host)
+{
+    if (!b) return EDM_EINVAL;
+    return guarded(b->ctx, [&]() -> int {
+        EDM_REQUIRE(states_host || b->D == 0, EDM_EINVAL, "NULL output");
+        if (b->D)
+            EDM_HIP_CHECK(hipMemcpyAsync(states_host, b->d_states, b->D * sizeof(uint64_t),
+                                         hipMemcpyDeviceToHost, b->ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+        return EDM_OK;
+    });
+}
+
+int edm_basis_device_ptr(const edm_basis *b, const uint64_t **states_dev)
+{
+    if (!b || !states_dev) return EDM_EINVAL;
+    *states_dev = b->d_states;
+    return EDM_OK;
+}
+
+int edm_basis_rank_mode(const edm_basis *b, int32_t *mode)
+{
+    if (!b || !mode) return EDM_EINVAL;
+    *mode = b->rank_mode;
+    return EDM_OK;
+}
+
+int edm_basis_destroy(edm_basis *b)
+{
+    if (!b) return EDM_OK;
+    return guarded(b->ctx, [&]() -> int {
+        (void)hipStreamSynchronize(b->ctx->stream);
+        free_basis(b);
+        return EDM_OK;
+    });
+}
+
+int edm_op_upload(edm_ctx *ctx, int32_t nbits, const double *amp, const uint64_t *out,
+                  const uint64_t *in, int64_t S, int32_t upper_hermitian, edm_op **op)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op, EDM_EINVAL, "NULL argument");
+        *op = build_op(ctx, nbits, amp, out, in, S, upper_hermitian != 0);
+        return EDM_OK;
+    });
+}
+
+int edm_op_size(const edm_op *op, int64_t *S_full)
+{
+    if (!op || !S_full) return EDM_EINVAL;
+    *S_full = op->S_full;
+    return EDM_OK;
+}
+
+int edm_op_full_list(const edm_op *op, double *amp, uint64_t *out, uint64_t *in, uint64_t *parity)
+{
+    if (!op) return EDM_EINVAL;
+    size_t n = (size_t)op->S_full;
+    if (amp) std::memcpy(amp, op->amp.data(), 2 * n * sizeof(double));
+    if (out) std::memcpy(out, op->out.data(), n * sizeof(uint64_t));
+    if (in) std::memcpy(in, op->in.data(), n * sizeof(uint64_t));
+    if (parity) std::memcpy(parity, op->par.data(), n * sizeof(uint64_t));
+    return EDM_OK;
+}
+
+int edm_op_destroy(edm_op *op)
+{
+    if (!op) return EDM_OK;
+    return guarded(op->ctx, [&]() -> int {
+        (void)hipStreamSynchronize(op->ctx->stream);
+        destroy_op(op);
+        return EDM_OK;
+    });
+}
+
+int edm_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const void *x_dev, void *y_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && x_dev && y_dev, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(x_dev != y_dev, EDM_EINVAL, "x and y must not alias");
+        launch_matvec(ctx, op, b, (const double2 *)x_dev, (double2 *)y_dev, ApplyMode::Forward);
+        return EDM_OK;
+    });
+}
+
+int edm_matvec_adjoint(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const void *x_dev,
+                       void *y_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && x_dev && y_dev, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(x_dev != y_dev, EDM_EINVAL, "x and y must not alias");
+        launch_matvec(ctx, op, b, (const double2 *)x_dev, (double2 *)y_dev, ApplyMode::Adjoint);
+        return EDM_OK;
+    });
+}
+
+int edm_matvec_host(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double *x, double *y)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && x && y, EDM_EINVAL, "NULL argument");
+        size_t bytes = (size_t)std::max<int64_t>(b->D, 1) * sizeof(double2);
+        double2 *dx = (double2 *)ctx->alloc(bytes);
+        double2 *dy = nullptr;
+        try {
+            dy = (double2 *)ctx->alloc(bytes);
+            EDM_HIP_CHECK(hipMemcpyAsync(dx, x, b->D * sizeof(double2), hipMemcpyHostToDevice, ctx->stream));
+            launch_matvec(ctx, op, b, dx, dy, ApplyMode::Forward);
+            EDM_HIP_CHECK(hipMemcpyAsync(y, dy, b->D * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream));
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->free(dx, bytes);
+            ctx->free(dy, bytes);
+            throw;
+        }
+        ctx->free(dx, bytes);
+        ctx->free(dy, bytes);
+        return EDM_OK;
+    });
+}
+
+int edm_connectivity_counts(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
+                            int64_t nrows, int64_t *counts)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && counts, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= b->D, EDM_EINVAL, "row range");
+        size_t bytes = (size_t)std::max<int64_t>(nrows, 1) * sizeof(int64_t);
+        int64_t *d = (int64_t *)ctx->alloc(bytes);
+        launch_connectivity(ctx, op, b, row0, nrows, d, nullptr, nullptr, nullptr, nullptr);
+        EDM_HIP_CHECK(hipMemcpyAsync(counts, d, nrows * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->free(d, bytes);
+        return EDM_OK;
+    });
+}
+
+int edm_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
+                     int64_t nrows, const int64_t *rowptr, int64_t *cols, int32_t *terms,
+                     int8_t *negs)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && rowptr && cols && terms && negs, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= b->D, EDM_EINVAL, "row range");
+        int64_t nnz = rowptr[nrows];
+        size_t rb = (size_t)(nrows + 1) * sizeof(int64_t);
+        size_t cb = (size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t);
+        size_t tb = (size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t);
+        size_t nb = (size_t)std::max<int64_t>(nnz, 1) * sizeof(int8_t);
+        int64_t *d_rp = (int64_t *)ctx->alloc(rb);
+        int64_t *d_c = (int64_t *)ctx->alloc(cb);
+        int32_t *d_t = (int32_t *)ctx->alloc(tb);
+        int8_t *d_n = (int8_t *)ctx->alloc(nb);
+        EDM_HIP_CHECK(hipMemcpyAsync(d_rp, rowptr, rb, hipMemcpyHostToDevice, ctx->stream));
+        launch_connectivity(ctx, op, b, row0, nrows, nullptr, d_rp, d_c, d_t, d_n);
+        EDM_HIP_CHECK(hipMemcpyAsync(cols, d_c, nnz * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+        EDM_HIP_CHECK(hipMemcpyAsync(terms, d_t, nnz * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        EDM_HIP_CHECK(hipMemcpyAsync(negs, d_n, nnz * sizeof(int8_t), hipMemcpyDeviceToHost, ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->free(d_rp, rb);
+        ctx->free(d_c, cb);
+        ctx->free(d_t, tb);
+        ctx->free(d_n, nb);
+        // rows sorted by (column, term)
+        std::vector<int64_t> idx;
+        std::vector<int64_t> c2;
+        std::vector<int32_t> t2;
+        std::vector<int8_t> n2;
+        for (int64_t r = 0; r < nrows; ++r) {
+            int64_t a = rowptr[r], e = rowptr[r + 1];
+            idx.resize(e - a);
+            for (int64_t k = 0; k < e - a; ++k) idx[k] = a + k;
+            std::sort(idx.begin(), idx.end(), [&](int64_t p, int64_t q) {
+                return cols[p] < cols[q] || (cols[p] == cols[q] && terms[p] < terms[q]);
+            });
+            c2.resize(e - a); t2.resize(e - a); n2.resize(e - a);
+            for (int64_t k = 0; k < e - a; ++k) {
+                c2[k] = cols[idx[k]]; t2[k] = terms[idx[k]]; n2[k] = negs[idx[k]];
+            }
+            for (int64_t k = 0; k < e - a; ++k) {
+                cols[a + k] = c2[k]; terms[a + k] = t2[k]; negs[a + k] = n2[k];
+            }
+        }
+        return EDM_OK;
+    });
+}
+
+int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t *nnz,
+                            int64_t *nnz_offdiag)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && nnz && nnz_offdiag, EDM_EINVAL, "NULL argument");
+        launch_conn_totals(ctx, op, b, nnz, nnz_offdiag);
+        return EDM_OK;
+    });
+}
+
+int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
+                 int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
+                 const double *vec0, double *vals, double *vecs, edm_solve_info *info)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && vals, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(op->nbits == b->nbits, EDM_EDIM,
+                    "Dimension of Hamiltonian linear map mismatches vector length.");
+        int rc = run_eigsolve(ctx, op, b, nev, krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+        if (rc == EDM_ENOCONV) {
+            ctx->last_error = "Krylov method fails. Cannot find " + std::to_string(nev) +
+                              " eigenvectors within maxiter.";
+        }
+        return rc;
+    });
+}
+
+int edm_fill_random(edm_ctx *ctx, void *v_dev, int64_t D, uint64_t seed, int32_t mode)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && (v_dev || D == 0), EDM_EINVAL, "NULL argument");
+        blas_fill_random(ctx, (double2 *)v_dev, D, seed, mode);
+        return EDM_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+// edm_blas.hip — complex-FP64 vector kernels for the Krylov solver.
+//
+// These stand in for KrylovKit's BLAS-1 on device arrays (dot, norm, axpy!, scale!,
+// orthogonalize) and the chunked restart GEMM of MomentumED/ext/MomentumEDCUDAExt.jl:210-244.
+// All are HBM-bound streaming kernels over vectors of D complex doubles laid out
+// contiguously (16 B per element). Reductions are two-stage and fixed-order (per-block
+// partials, then one block per output summing the partials in block order), so results are
+// bitwise reproducible run to run, independent of what else runs on the device.
+#include "edm_internal.h"
+
+#include <algorithm>
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kRpt = 4;                    // rows per thread in reductions
+constexpr int kTile = kBlk * kRpt;         // rows per reduction block
+constexpr int kMaxVec = 256;               // vectors per multi-dot / multi-axpy call
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide sum of (a, b); result valid in thread 0
+__device__ __forceinline__ double2 block_sum2(double a, double b, double2 *sh)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = make_double2(a, b);
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { r.x += sh[w].x; r.y += sh[w].y; }
+    return r;
+}
+
+// partial[i * nb + blk] = sum over the block's rows of conj(V_i[r]) * w[r]
+__global__ void __launch_bounds__(kBlk)
+k_multidot_partial(const double2 *const *__restrict__ vptrs, int nvec,
+                   const double2 *__restrict__ w, int64_t D, double2 *__restrict__ partial)
+{
+    __shared__ double2 sh[kBlk / 64];
+    int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    double2 wv[kRpt];
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
+        int64_t r = base + (int64_t)k * kBlk;
+        wv[k] = r < D ? w[r] : make_double2(0.0, 0.0);
+    }
+    for (int i = 0; i < nvec; ++i) {
+        const double2 *v = vptrs[i];
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int k = 0; k < kRpt; ++k) {
+            int64_t r = base + (int64_t)k * kBlk;
+            if (r < D) {
+                double2 a = v[r];
+                ar += a.x * wv[k].x + a.y * wv[k].y;
+                ai += a.x * wv[k].y - a.y * wv[k].x;
+            }
+        }
+        double2 s = block_sum2(ar, ai, sh);
+        if (threadIdx.x == 0) partial[(int64_t)i * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// out[i] = sum_b partial[i * nb + b], one block per output, fixed order
+__global__ void __launch_bounds__(kBlk)
+k_reduce_partials(const double2 *__restrict__ partial, int64_t nb, double2 *__restrict__ out)
+{
+    __shared__ double2 sh[kBlk / 64];
+    const double2 *p = partial + (int64_t)blockIdx.x * nb;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += kBlk) { ar += p[b].x; ai += p[b].y; }
+    double2 s = block_sum2(ar, ai, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// w[r] -= sum_i V_i[r] h_i
+__global__ void __launch_bounds__(kBlk)
+k_multiaxpy(const double2 *const *__restrict__ vptrs, int nvec, const double2 *__restrict__ h,
+            double2 *__restrict__ w, int64_t D)
+{
+    __shared__ double2 hs[kMaxVec];
+    for (int i = threadIdx.x; i < nvec; i += kBlk) hs[i] = h[i];
+    __syncthreads();
+    int64_t r = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (r >= D) return;
+    double2 acc = w[r];
+    for (int i = 0; i < nvec; ++i) {
+        double2 a = vptrs[i][r];
+        double2 c = hs[i];
+        acc.x -= a.x * c.x - a.y * c.y;
+        acc.y -= a.x * c.y + a.y * c.x;
+    }
+    w[r] = acc;
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_norm2_partial(const double2 *__restrict__ w, int64_t D, double2 *__restrict__ partial)
+{
+    __shared__ double2 sh[kBlk / 64];
+    int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
+        int64_t r = base + (int64_t)k * kBlk;
+        if (r < D) { double2 a = w[r]; s += a.x * a.x + a.y * a.y; }
+    }
+    double2 t = block_sum2(s, 0.0, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_sqrt_first(const double2 *__restrict__ in, double *__restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = sqrt(in[0].x);
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_scale_inv(double2 *__restrict__ v, const double2 *__restrict__ w, const double *__restrict__ beta,
+            int64_t D)
+{
+    int64_t r = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (r >= D) return;
+    double inv = 1.0 / beta[0];
+    double2 a = w[r];
+    v[r] = make_double2(a.x * inv, a.y * inv);
+}
+
+// out_c[r] = sum_i V_i[r] * Y[i + ldy*c], c in [c0, c0+kc)
+constexpr int kRotChunk = 8;
+__global__ void __launch_bounds__(kBlk)
+k_rotate(const double2 *const *__restrict__ vptrs, int m, const double *__restrict__ Y, int ldy,
+         int c0, int kc, double2 *const *__restrict__ outptrs, int64_t D)
+{
+    extern __shared__ double ys[];   // m * kRotChunk
+    for (int t = threadIdx.x; t < m * kRotChunk; t += kBlk) {
+        int i = t / kRotChunk, c = t % kRotChunk;
+        ys[t] = c < kc ? Y[i + (int64_t)ldy * (c0 + c)] : 0.0;
+    }
+    __syncthreads();
+    int64_t r = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (r >= D) return;
+    double ar[kRotChunk], ai[kRotChunk];
+#pragma unroll
+    for (int c = 0; c < kRotChunk; ++c) { ar[c] = 0.0; ai[c] = 0.0; }
+    for (int i = 0; i < m; ++i) {
+        double2 v = vptrs[i][r];
+#pragma unroll
+        for (int c = 0; c < kRotChunk; ++c) {
+            double y = ys[i * kRotChunk + c];
+            ar[c] += v.x * y;
+            ai[c] += v.y * y;
+        }
+    }
+    for (int c = 0; c < kc; ++c) outptrs[c0 + c][r] = make_double2(ar[c], ai[c]);
+}
+
+// SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) as a counter-based generator:
+// value(seed, n) = mix(seed + (n + 1) * golden).
+__device__ __forceinline__ uint64_t splitmix(uint64_t seed, uint64_t n)
+{
+    uint64_t z = seed + (n + 1ULL) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double u01(uint64_t z) { return (double)(z >> 11) * 0x1.0p-53; }
+
+__global__ void __launch_bounds__(kBlk)
+k_fill_random(double2 *__restrict__ v, int64_t D, uint64_t seed, int mode)
+{
+    int64_t r = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (r >= D) return;
+    double a = u01(splitmix(seed, 2 * (uint64_t)r));
+    double b = u01(splitmix(seed, 2 * (uint64_t)r + 1));
+    if (mode == 0) {
+        v[r] = make_double2(a, b);
+    } else {   // Box-Muller: two independent N(0,1) values
+        double rad = sqrt(-2.0 * log(1.0 - a));
+        double s, c;
+        sincospi(2.0 * b, &s, &c);
+        v[r] = make_double2(rad * c, rad * s);
+    }
+}
+
+inline unsigned rows_blocks(int64_t D) { return (unsigned)((D + kBlk - 1) / kBlk); }
+inline unsigned tile_blocks(int64_t D) { return (unsigned)((D + kTile - 1) / kTile); }
+
+}  // namespace
+
+void blas_fill_random(edm_ctx *ctx, double2 *v, int64_t D, uint64_t seed, int mode)
+{
+    if (D == 0) return;
+    hipLaunchKernelGGL(k_fill_random, dim3(rows_blocks(D)), dim3(kBlk), 0, ctx->stream, v, D,
+                       seed, mode);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_multidot(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const double2 *w,
+                   int64_t D, double2 *d_h)
+{
+    EDM_REQUIRE(nvec <= kMaxVec, EDM_EINVAL, "too many vectors in one multi-dot");
+    if (nvec == 0) return;
+    unsigned nb = std::max(1u, tile_blocks(D));
+    double2 *part = (double2 *)ctx->partials((size_t)nb * nvec * sizeof(double2));
+    hipLaunchKernelGGL(k_multidot_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, d_vptrs, nvec,
+                       w, D, part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nvec), dim3(kBlk), 0, ctx->stream, part,
+                       (int64_t)nb, d_h);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_multiaxpy(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const double2 *d_h,
+                    double2 *w, int64_t D)
+{
+    EDM_REQUIRE(nvec <= kMaxVec, EDM_EINVAL, "too many vectors in one multi-axpy");
+    if (nvec == 0 || D == 0) return;
+    hipLaunchKernelGGL(k_multiaxpy, dim3(rows_blocks(D)), dim3(kBlk), 0, ctx->stream, d_vptrs,
+                       nvec, d_h, w, D);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_norm(edm_ctx *ctx, const double2 *w, int64_t D, double *d_out)
+{
+    unsigned nb = std::max(1u, tile_blocks(D));
+    double2 *part = (double2 *)ctx->partials((size_t)(nb + 1) * sizeof(double2));
+    hipLaunchKernelGGL(k_norm2_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, w, D, part);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(kBlk), 0, ctx->stream, part, (int64_t)nb,
+                       part + nb);
+    hipLaunchKernelGGL(k_sqrt_first, dim3(1), dim3(64), 0, ctx->stream, part + nb, d_out);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_scale_inv(edm_ctx *ctx, double2 *v, const double2 *w, const double *d_beta, int64_t D)
+{
+    if (D == 0) return;
+    hipLaunchKernelGGL(k_scale_inv, dim3(rows_blocks(D)), dim3(kBlk), 0, ctx->stream, v, w,
+                       d_beta, D);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_rotate(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const double *d_Y, int ldy,
+                 int k, double2 *const *d_outptrs, int64_t D)
+{
+    if (D == 0 || k == 0) return;
+    size_t sh = (size_t)m * kRotChunk * sizeof(double);
+    EDM_REQUIRE(sh <= 64 * 1024, EDM_EINVAL, "Krylov dimension too large for the rotation kernel");
+    for (int c0 = 0; c0 < k; c0 += kRotChunk) {
+        int kc = std::min(kRotChunk, k - c0);
+        hipLaunchKernelGGL(k_rotate, dim3(rows_blocks(D)), dim3(kBlk), sh, ctx->stream, d_vptrs,
+                           m, d_Y, ldy, c0, kc, d_outptrs, D);
+    }
+    EDM_HIP_CHECK(hipGetLastError());
+}

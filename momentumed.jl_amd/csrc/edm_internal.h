@@ -1,0 +1,177 @@
+// edm_internal.h — shared internal declarations of libedm.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/edm.h"
+
+// ------------------------------------------------------------------ errors
+struct EdmError {
+    int code;
+    std::string msg;
+};
+
+void edm_set_global_error(const std::string &msg);
+
+#define EDM_HIP_CHECK(expr)                                                                 \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            throw EdmError{EDM_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e)};    \
+    } while (0)
+
+#define EDM_REQUIRE(cond, code, msg)                                                        \
+    do {                                                                                    \
+        if (!(cond)) throw EdmError{(code), (msg)};                                         \
+    } while (0)
+
+// ------------------------------------------------------------------ system description
+struct SystemSpec {
+    int dim = 0, nk = 0, ncm = 1, ncc = 1;
+    std::vector<int64_t> k_list;   // dim x nk, column-major
+    std::vector<int64_t> gk;       // dim
+    std::vector<int64_t> n_each;   // ncc
+    // derived
+    int nbits_comp = 0;            // orbitals per conserved component = nk*ncm
+    int nbits = 0;                 // total bits = nk*ncm*ncc
+    int64_t ntot = 0;
+    std::vector<int64_t> kmin, krange;   // per dim sector range
+    int64_t nsec = 0;
+
+    static SystemSpec from(const edm_system *s);
+    // momentum of orbital bit `b` (0-based global bit) in dimension d
+    int64_t orb_k(int b, int d) const { return k_list[d + dim * ((b % nbits_comp) % nk)]; }
+    int comp_of_bit(int b) const { return b / nbits_comp; }
+    // sector index of a state's (unreduced) momentum sums, -1 if outside momentum_list
+    int64_t sector_of_sums(const int64_t *sums) const;
+    void sector_momentum(int64_t sector, int64_t *k) const;
+};
+
+// ------------------------------------------------------------------ context
+struct edm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;     // 0 = unlimited
+    uint64_t used = 0;
+    std::string last_error;
+    // binomial table C(a,b), a<=64, b<=64 on the device (enumeration)
+    uint64_t *d_binom = nullptr;
+    // reduction scratch
+    double *d_partials = nullptr;
+    size_t partials_bytes = 0;
+
+    void *alloc(size_t bytes);
+    void free(void *p, size_t bytes);
+    double *partials(size_t bytes);
+};
+
+// RAII set-device guard
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+// ------------------------------------------------------------------ basis
+uint64_t edm_next_id();
+
+struct edm_basis {
+    uint64_t id = edm_next_id();
+    edm_ctx *ctx = nullptr;
+    int64_t D = 0;
+    int nbits = 0;
+    uint64_t *d_states = nullptr;
+    // rank tables (Lin two-table): index(t) = jhi[t >> nlo] + jlo[t & ((1<<nlo)-1)]
+    int rank_mode = 0;      // 0 = binary search, 1 = rank tables
+    int nlo = 0, nhi = 0;
+    uint32_t *d_jlo = nullptr, *d_jhi = nullptr;
+    bool has_spec = false;
+    SystemSpec spec;
+    int64_t sector = -1;
+};
+
+// ------------------------------------------------------------------ operator
+// Right-apply term table keyed by the creation mask (out): key a (one-body out = {a}) or
+// 64 + b(b-1)/2 + a (two-body out = {a<b}). Entries of a key are in ascending `in` order.
+constexpr int EDM_NKEYS = 64 + 64 * 63 / 2;   // 2080
+
+struct OpTable {
+    uint32_t *d_keyptr = nullptr;   // EDM_NKEYS + 1
+    uint64_t *d_in = nullptr;       // annihilation mask per entry
+    uint64_t *d_par = nullptr;      // parity mask per entry
+    double2 *d_amp = nullptr;       // amplitude per entry
+    int32_t *d_term = nullptr;      // index into the canonical full list
+    int64_t nent = 0;
+};
+
+struct edm_op {
+    uint64_t id = edm_next_id();
+    edm_ctx *ctx = nullptr;
+    // cached op_conserves() results keyed by basis id
+    mutable std::vector<std::pair<uint64_t, bool>> conserve_cache;
+    int nbits = 0;
+    bool upper = false;
+    int64_t S_full = 0;
+    // canonical full list (host copy, (in,out)-sorted)
+    std::vector<double> amp;        // 2*S_full
+    std::vector<uint64_t> out, in, par;
+    bool fast = false;              // every term is 1- or 2-body and number conserving
+    bool has_onebody = false;
+    OpTable fwd;                    // H: right-apply keyed by out
+    OpTable adj;                    // H^dagger: keyed by in (swapped roles, conj amp)
+    // generic fallback: full list SoA on the device
+    double2 *d_amp = nullptr;
+    uint64_t *d_out = nullptr, *d_in = nullptr, *d_par = nullptr;
+};
+
+// ------------------------------------------------------------------ kernels (launchers)
+// enumeration (edm_enum.hip)
+void enum_sector_counts(edm_ctx *ctx, const SystemSpec &sp, std::vector<int64_t> &counts);
+// enumerate one sector; returns a device array of *D states (allocated on ctx)
+uint64_t *enum_sector_states(edm_ctx *ctx, const SystemSpec &sp, int64_t sector, int64_t *D);
+// rank tables (edm_rank.cpp)
+bool build_rank_tables(edm_basis *b);
+// matvec (edm_matvec.hip)
+enum class ApplyMode { Forward, Adjoint };
+void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
+                   double2 *y, ApplyMode mode);
+void launch_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
+                         int64_t nrows, int64_t *d_counts, const int64_t *d_rowptr,
+                         int64_t *d_cols, int32_t *d_terms, int8_t *d_negs);
+bool op_conserves(const edm_op *op, const edm_basis *b);
+void launch_conn_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t *nnz,
+                        int64_t *nnz_off);
+edm_op *build_op(edm_ctx *ctx, int nbits, const double *amp, const uint64_t *out,
+                 const uint64_t *in, int64_t S, bool upper);
+void destroy_op(edm_op *op);
+bool verify_rank_tables(edm_basis *b);
+int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, int krylovdim,
+                 double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
+                 double *vecs, edm_solve_info *info);
+const char *edm_global_error();
+// vector kernels (edm_blas.hip)
+void blas_fill_random(edm_ctx *ctx, double2 *v, int64_t D, uint64_t seed, int mode);
+// h[i] = <V_i, w> for i < nvec (V given by a device pointer table); result on device
+void blas_multidot(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const double2 *w,
+                   int64_t D, double2 *d_h);
+// w -= sum_i V_i h_i (h on device)
+void blas_multiaxpy(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const double2 *d_h,
+                    double2 *w, int64_t D);
+// d_out[0] = ||w||_2
+void blas_norm(edm_ctx *ctx, const double2 *w, int64_t D, double *d_out);
+// v = w / (*d_beta)
+void blas_scale_inv(edm_ctx *ctx, double2 *v, const double2 *w, const double *d_beta, int64_t D);
+// out_c = sum_i V_i Y[i + ldy*c] for c < k (Y real, on device)
+void blas_rotate(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const double *d_Y, int ldy,
+                 int k, double2 *const *d_outptrs, int64_t D);

@@ -1,0 +1,298 @@
+// edm_lanczos.cpp — thick-restart Lanczos for the lowest eigenpairs of a Hermitian map.
+//
+// Stands in for KrylovKit.eigsolve(A, x0, howmany, :SR; ishermitian=true, krylovdim, tol,
+// maxiter) as called by krylov_map_solve (MomentumED/src/method/linear_map.jl:179-191,
+// MomentumED/ext/MomentumEDCUDAExt.jl:159-179). Algorithm: the thick-restart Lanczos method
+// of Wu & Simon (SIAM J. Matrix Anal. Appl. 22, 2000) with full re-orthogonalisation by
+// classical Gram-Schmidt applied twice ("twice is enough", Giraud, Langou & Rozloznik 2005).
+// Every vector lives on the device; per Lanczos step the host reads back only the projection
+// coefficients and the norm (a few hundred bytes). The projected matrix (tridiagonal plus an
+// arrow after each restart) is real symmetric and is diagonalised on the host by the cyclic
+// Jacobi method (Golub & Van Loan, Matrix Computations, 4th ed., section 8.5).
+//
+// Restart: keep the `keep` lowest Ritz vectors, keep = (3*krylovdim + 2*converged) / 5 (at
+// least the number of wanted pairs), rotate them into place with one tall-skinny product,
+// and continue from the normalised residual. Convergence of Ritz pair i: |beta_m * Y[m-1,i]|
+// <= tol (KrylovKit's absolute residual criterion).
+#include "edm_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+
+namespace {
+
+// Cyclic Jacobi for a real symmetric n x n matrix A (row-major, destroyed). On return
+// w[i] ascending and Z[:, i] (column-major, ldz = n) the matching orthonormal eigenvectors.
+void jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z)
+{
+    Z.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+    auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
+    double fro = 0.0;
+    for (double v : A) fro += v * v;
+    fro = std::sqrt(fro);
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) off += a(p, q) * a(p, q);
+        if (std::sqrt(off) <= 1e-300 + 1e-17 * fro) break;
+        for (int p = 0; p < n - 1; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                double apq = a(p, q);
+                if (apq == 0.0) continue;
+                double tau = (a(q, q) - a(p, p)) / (2.0 * apq);
+                double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+                double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < n; ++k) {   // A <- A J (columns p, q)
+                    double akp = a(k, p), akq = a(k, q);
+                    a(k, p) = c * akp - s * akq;
+                    a(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {   // A <- J^T A (rows p, q)
+                    double apk = a(p, k), aqk = a(q, k);
+                    a(p, k) = c * apk - s * aqk;
+                    a(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {   // Z <- Z J
+                    double zkp = Z[(size_t)p * n + k], zkq = Z[(size_t)q * n + k];
+                    Z[(size_t)p * n + k] = c * zkp - s * zkq;
+                    Z[(size_t)q * n + k] = s * zkp + c * zkq;
+                }
+            }
+    }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return a(i, i) < a(j, j); });
+    std::vector<double> Zs((size_t)n * n);
+    w.resize(n);
+    for (int c = 0; c < n; ++c) {
+        w[c] = a(idx[c], idx[c]);
+        std::copy(Z.begin() + (size_t)idx[c] * n, Z.begin() + (size_t)idx[c] * n + n,
+                  Zs.begin() + (size_t)c * n);
+    }
+    Z.swap(Zs);
+}
+
+struct Workspace {
+    edm_ctx *ctx;
+    int64_t D;
+    int m;
+    std::vector<double2 *> V, tmp;
+    double2 **d_vptrs = nullptr, **d_tmpptrs = nullptr;
+    double2 *d_h = nullptr, *d_h2 = nullptr;
+    double *d_beta = nullptr, *d_Y = nullptr;
+    size_t vbytes = 0;
+
+    Workspace(edm_ctx *c, int64_t D_, int m_, int ntmp) : ctx(c), D(D_), m(m_)
+    {
+        vbytes = (size_t)std::max<int64_t>(D, 1) * sizeof(double2);
+        try {
+            for (int i = 0; i <= m; ++i) V.push_back((double2 *)ctx->alloc(vbytes));
+            for (int i = 0; i < ntmp; ++i) tmp.push_back((double2 *)ctx->alloc(vbytes));
+            d_vptrs = (double2 **)ctx->alloc((m + 1) * sizeof(double2 *));
+            d_tmpptrs = (double2 **)ctx->alloc(std::max(ntmp, 1) * sizeof(double2 *));
+            d_h = (double2 *)ctx->alloc((m + 1) * sizeof(double2));
+            d_h2 = (double2 *)ctx->alloc((m + 1) * sizeof(double2));
+            d_beta = (double *)ctx->alloc(sizeof(double) * 2);
+            d_Y = (double *)ctx->alloc((size_t)m * m * sizeof(double));
+        } catch (...) {
+            release();
+            throw;
+        }
+        upload_ptrs();
+    }
+    void upload_ptrs()
+    {
+        EDM_HIP_CHECK(hipMemcpyAsync(d_vptrs, V.data(), V.size() * sizeof(double2 *),
+                                     hipMemcpyHostToDevice, ctx->stream));
+        if (!tmp.empty())
+            EDM_HIP_CHECK(hipMemcpyAsync(d_tmpptrs, tmp.data(), tmp.size() * sizeof(double2 *),
+                                         hipMemcpyHostToDevice, ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    void release()
+    {
+        for (auto p : V) ctx->free(p, vbytes);
+        for (auto p : tmp) ctx->free(p, vbytes);
+        V.clear();
+        tmp.clear();
+        ctx->free(d_vptrs, (m + 1) * sizeof(double2 *));
+        ctx->free(d_tmpptrs, sizeof(double2 *));
+        ctx->free(d_h, (m + 1) * sizeof(double2));
+        ctx->free(d_h2, (m + 1) * sizeof(double2));
+        ctx->free(d_beta, sizeof(double) * 2);
+        ctx->free(d_Y, (size_t)m * m * sizeof(double));
+        d_vptrs = d_tmpptrs = nullptr;
+        d_h = d_h2 = nullptr;
+        d_beta = d_Y = nullptr;
+    }
+    ~Workspace() { release(); }
+};
+
+// Orthogonalise V[target] against V[0..n) twice; returns the first-pass coefficients + second
+// pass corrections summed on the host (h_out[n]).
+void cgs2(Workspace &ws, int n, double2 *target, std::vector<double2> &h_out)
+{
+    edm_ctx *ctx = ws.ctx;
+    h_out.assign(n, make_double2(0.0, 0.0));
+    if (n == 0) return;
+    std::vector<double2> h2(n);
+    blas_multidot(ctx, ws.d_vptrs, n, target, ws.D, ws.d_h);
+    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h, target, ws.D);
+    blas_multidot(ctx, ws.d_vptrs, n, target, ws.D, ws.d_h2);
+    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h2, target, ws.D);
+    EDM_HIP_CHECK(hipMemcpyAsync(h_out.data(), ws.d_h, n * sizeof(double2), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    EDM_HIP_CHECK(hipMemcpyAsync(h2.data(), ws.d_h2, n * sizeof(double2), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) {
+        h_out[i].x += h2[i].x;
+        h_out[i].y += h2[i].y;
+    }
+}
+
+double norm_host(Workspace &ws, const double2 *v)
+{
+    blas_norm(ws.ctx, v, ws.D, ws.d_beta);
+    double b = 0.0;
+    EDM_HIP_CHECK(hipMemcpyAsync(&b, ws.d_beta, sizeof(double), hipMemcpyDeviceToHost,
+                                 ws.ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ws.ctx->stream));
+    return b;
+}
+
+}  // namespace
+
+int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, int krylovdim,
+                 double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
+                 double *vecs, edm_solve_info *info)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const int64_t D = b->D;
+    EDM_REQUIRE(D > 0, EDM_EINVAL, "empty Hilbert subspace");
+    EDM_REQUIRE(nev >= 1, EDM_EINVAL, "N should be at least 1.");
+    EDM_REQUIRE(nev <= D, EDM_EINVAL, "more eigenpairs requested than the subspace dimension");
+    EDM_REQUIRE(tol > 0, EDM_EINVAL, "tol must be positive");
+    EDM_REQUIRE(maxiter >= 1, EDM_EINVAL, "maxiter must be at least 1");
+    int m = std::max(krylovdim, nev + 2);
+    if ((int64_t)m > D) m = (int)D;
+    EDM_REQUIRE(m <= 255, EDM_EINVAL, "krylovdim must be at most 255");
+    int ntmp = m;
+    Workspace ws(ctx, D, m, ntmp);
+    // size the reduction scratch once for the largest multi-dot
+    {
+        int64_t nb = (D + 1023) / 1024;
+        ctx->partials((size_t)(nb + 1) * (m + 1) * sizeof(double2));
+    }
+    hipStream_t st = ctx->stream;
+    int matvecs = 0;
+    uint64_t rng_stream = 0;
+
+    // start vector
+    if (vec0) {
+        EDM_HIP_CHECK(hipMemcpyAsync(ws.V[0], vec0, D * sizeof(double2), hipMemcpyHostToDevice, st));
+    } else {
+        blas_fill_random(ctx, ws.V[0], D, seed, 0);
+    }
+    double b0 = norm_host(ws, ws.V[0]);
+    EDM_REQUIRE(b0 > 0 && std::isfinite(b0), EDM_EINVAL, "start vector is zero or not finite");
+    blas_scale_inv(ctx, ws.V[0], ws.V[0], ws.d_beta, D);
+
+    std::vector<double> T((size_t)m * m, 0.0);
+    auto t = [&](int i, int j) -> double & { return T[(size_t)i * m + j]; };
+    std::vector<double> theta, Y;
+    std::vector<double2> h;
+    int k = 0;            // kept Ritz vectors at the start of a cycle
+    int iters = 0;
+    int nconv = 0;
+    double beta_last = 0.0;
+    double hnorm = 0.0;   // running estimate of ||H|| for the breakdown threshold
+    std::vector<double> res(m, 0.0);
+
+    for (iters = 1; iters <= maxiter; ++iters) {
+        for (int j = k; j < m; ++j) {
+            launch_matvec(ctx, op, b, ws.V[j], ws.V[j + 1], ApplyMode::Forward);
+            ++matvecs;
+            cgs2(ws, j + 1, ws.V[j + 1], h);
+            t(j, j) = h[j].x;
+            if (j > k) {
+                // beta_{j-1} already stored; nothing else to record
+            }
+            double beta = norm_host(ws, ws.V[j + 1]);
+            hnorm = std::max(hnorm, std::fabs(h[j].x) + beta + (j > 0 ? std::fabs(t(j, j - 1)) : 0.0));
+            bool breakdown = !(beta > 1e-14 * std::max(hnorm, 1e-300));
+            if (breakdown) {
+                // invariant subspace: continue from a fresh random direction orthogonal to
+                // the current basis (coupling zero)
+                beta = 0.0;
+                for (int attempt = 0; attempt < 4; ++attempt) {
+                    blas_fill_random(ctx, ws.V[j + 1], D, seed ^ (0xA5A5A5A5ULL + ++rng_stream), 0);
+                    std::vector<double2> hh;
+                    cgs2(ws, j + 1, ws.V[j + 1], hh);
+                    double nb = norm_host(ws, ws.V[j + 1]);
+                    if (nb > 1e-8) break;
+                }
+            }
+            blas_scale_inv(ctx, ws.V[j + 1], ws.V[j + 1], ws.d_beta, D);
+            if (j + 1 < m) {
+                t(j, j + 1) = beta;
+                t(j + 1, j) = beta;
+            }
+            beta_last = beta;
+        }
+        // Rayleigh-Ritz on the projected matrix
+        std::vector<double> A = T;
+        jacobi_eigh(m, A, theta, Y);
+        nconv = 0;
+        for (int i = 0; i < m; ++i) res[i] = std::fabs(beta_last * Y[(size_t)i * m + (m - 1)]);
+        while (nconv < nev && res[nconv] <= tol) ++nconv;
+        if (nconv >= nev || iters == maxiter || (int64_t)m == D) break;
+        // thick restart
+        int keep = (3 * m + 2 * nconv) / 5;
+        keep = std::max(keep, nev);
+        keep = std::min(keep, m - 1);
+        EDM_HIP_CHECK(hipMemcpyAsync(ws.d_Y, Y.data(), (size_t)m * keep * sizeof(double),
+                                     hipMemcpyHostToDevice, st));
+        blas_rotate(ctx, (const double2 *const *)ws.d_vptrs, m, ws.d_Y, m, keep,
+                    (double2 *const *)ws.d_tmpptrs, D);
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int c = 0; c < keep; ++c) std::swap(ws.V[c], ws.tmp[c]);
+        std::swap(ws.V[keep], ws.V[m]);
+        ws.upload_ptrs();
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int c = 0; c < keep; ++c) {
+            t(c, c) = theta[c];
+            double cpl = beta_last * Y[(size_t)c * m + (m - 1)];
+            t(c, keep) = cpl;
+            t(keep, c) = cpl;
+        }
+        k = keep;
+    }
+    if (iters > maxiter) iters = maxiter;
+
+    for (int i = 0; i < nev; ++i) vals[i] = theta[i];
+    double maxres = 0.0;
+    for (int i = 0; i < nev; ++i) maxres = std::max(maxres, res[i]);
+    if (vecs) {
+        EDM_HIP_CHECK(hipMemcpyAsync(ws.d_Y, Y.data(), (size_t)m * nev * sizeof(double),
+                                     hipMemcpyHostToDevice, st));
+        blas_rotate(ctx, (const double2 *const *)ws.d_vptrs, m, ws.d_Y, m, nev,
+                    (double2 *const *)ws.d_tmpptrs, D);
+        for (int i = 0; i < nev; ++i)
+            EDM_HIP_CHECK(hipMemcpyAsync(vecs + 2 * (size_t)i * D, ws.tmp[i], D * sizeof(double2),
+                                         hipMemcpyDeviceToHost, st));
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    if (info) {
+        info->iterations = iters;
+        info->matvecs = matvecs;
+        info->converged = nconv;
+        info->krylovdim = m;
+        info->max_residual = maxres;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return nconv >= nev ? EDM_OK : EDM_ENOCONV;
+}

@@ -1,0 +1,573 @@
+// edm_matvec.hip — matrix-free H x in gather form.
+//
+// Semantics: the CPU map of the reference, MomentumED/src/method/linear_map.jl:100-118 —
+// for every basis row i, y_i = sum over full-list terms s of <i|s|j> x_j, where the right
+// application state_i * s (EDCore/src/multiplication.jl:55-69) needs out(s) ⊆ state_i, gives
+// mid = state_i & ~out, needs mid ∩ in(s) = ∅, target mid | in with sign
+// (-1)^popcount(mid & parity) (multiplication.jl:4-6), and a target outside the basis is
+// dropped (index_fit, hilbert_subspace.jl:70-72). (The reference's GPU kernel,
+// gpu_interface.jl:128-163, left-applies and so computes conj(H) x; this file computes H x.)
+//
+// Instead of testing every term against every row (the reference loops over the whole
+// term list per row), the term list is bucketed by creation mask. A row enumerates only the
+// buckets it can feed: one key per occupied orbital (one-body) and one per occupied pair
+// (two-body), N + N(N-1)/2 keys. Each bucket lists its terms' annihilation masks, parity
+// masks and amplitudes. Partner indices come from the two-table rank of a complete sector
+// (edm_rank.cpp) — two independent loads — or from binary search when the basis is not a
+// complete sector or the operator does not conserve its quantum numbers. One thread owns one
+// row and writes y_i once: no atomics.
+#include "edm_internal.h"
+
+#include <algorithm>
+#include <numeric>
+
+namespace {
+
+constexpr int kRowBlock = 256;
+
+struct OpDev {
+    const uint32_t *keyptr;
+    const uint64_t *tin;
+    const uint64_t *tpar;
+    const double2 *tamp;
+    const int32_t *tterm;
+    int has_onebody;
+};
+
+struct RankLin {
+    const uint32_t *jlo, *jhi;
+    int nlo;
+    uint64_t lomask;
+    __device__ __forceinline__ int64_t operator()(uint64_t t) const
+    {
+        return (int64_t)jhi[t >> nlo] + (int64_t)jlo[t & lomask];
+    }
+};
+
+struct RankSearch {
+    const uint64_t *states;
+    int64_t D;
+    __device__ __forceinline__ int64_t operator()(uint64_t t) const
+    {
+        int64_t lo = 0, hi = D;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (states[mid] < t) lo = mid + 1; else hi = mid;
+        }
+        return (lo < D && states[lo] == t) ? lo : -1;
+    }
+};
+
+// Visit every (term entry, partner) pair of row state s. f(e, j, neg) for off-diagonal
+// contributions, fd(e) for diagonal ones (target = s itself).
+template <class Rank, class F, class FD>
+__device__ __forceinline__ void visit_row(uint64_t s, const OpDev &op, const Rank &rank, F &&f,
+                                          FD &&fd)
+{
+    auto bucket = [&](int key, uint64_t outm) {
+        uint32_t e0 = op.keyptr[key], e1 = op.keyptr[key + 1];
+        if (e0 == e1) return;
+        uint64_t mid = s & ~outm;
+        for (uint32_t e = e0; e < e1; ++e) {
+            uint64_t in = op.tin[e];
+            if (in == outm) { fd(e); continue; }
+            if (mid & in) continue;
+            uint64_t t = mid | in;
+            int neg = __popcll(mid & op.tpar[e]) & 1;
+            int64_t j = rank(t);
+            if (j < 0) continue;
+            f(e, j, neg);
+        }
+    };
+    uint64_t rem = s;
+    while (rem) {
+        int b = __builtin_ctzll(rem);
+        rem &= rem - 1;
+        uint64_t ob = 1ULL << b;
+        if (op.has_onebody) bucket(b, ob);
+        uint64_t low = s & (ob - 1);
+        int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            int a = __builtin_ctzll(low);
+            low &= low - 1;
+            bucket(kb + a, ob | (1ULL << a));
+        }
+    }
+}
+
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_matvec_fast(int64_t D, const uint64_t *__restrict__ states, OpDev op, Rank rank,
+              const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    uint64_t s = states[i];
+    double2 xi = x[i];
+    double yr = 0.0, yi = 0.0;
+    visit_row(
+        s, op, rank,
+        [&](uint32_t e, int64_t j, int neg) {
+            double2 a = op.tamp[e];
+            double2 xv = x[j];
+            double pr = a.x * xv.x - a.y * xv.y;
+            double pi = a.x * xv.y + a.y * xv.x;
+            if (neg) { yr -= pr; yi -= pi; } else { yr += pr; yi += pi; }
+        },
+        [&](uint32_t e) {
+            double2 a = op.tamp[e];
+            yr += a.x * xi.x - a.y * xi.y;
+            yi += a.x * xi.y + a.y * xi.x;
+        });
+    y[i] = make_double2(yr, yi);
+}
+
+// Generic terms (any body count): every full-list term is tried on every row, as the
+// reference does; used only for operators the bucketed table cannot hold.
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_matvec_generic(int64_t D, const uint64_t *__restrict__ states, int64_t S,
+                 const double2 *__restrict__ amp, const uint64_t *__restrict__ tout,
+                 const uint64_t *__restrict__ tin, const uint64_t *__restrict__ tpar, Rank rank,
+                 const double2 *__restrict__ x, double2 *__restrict__ y, int adjoint)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    uint64_t s = states[i];
+    double yr = 0.0, yi = 0.0;
+    for (int64_t e = 0; e < S; ++e) {
+        uint64_t out = adjoint ? tin[e] : tout[e];
+        uint64_t in = adjoint ? tout[e] : tin[e];
+        if ((s & out) != out) continue;
+        double2 a = amp[e];
+        if (adjoint) a.y = -a.y;
+        int64_t j;
+        int neg = 0;
+        if (in == out) {
+            j = i;
+        } else {
+            uint64_t mid = s & ~out;
+            if (mid & in) continue;
+            neg = __popcll(mid & tpar[e]) & 1;
+            j = rank(mid | in);
+            if (j < 0) continue;
+        }
+        double2 xv = x[j];
+        double pr = a.x * xv.x - a.y * xv.y;
+        double pi = a.x * xv.y + a.y * xv.x;
+        if (neg) { yr -= pr; yi -= pi; } else { yr += pr; yi += pi; }
+    }
+    y[i] = make_double2(yr, yi);
+}
+
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_conn_count(int64_t row0, int64_t nrows, const uint64_t *__restrict__ states, OpDev op,
+             Rank rank, int64_t *__restrict__ counts)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int64_t c = 0;
+    visit_row(states[row0 + r], op, rank, [&](uint32_t, int64_t, int) { ++c; },
+              [&](uint32_t) { ++c; });
+    counts[r] = c;
+}
+
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_conn_fill(int64_t row0, int64_t nrows, const uint64_t *__restrict__ states, OpDev op,
+            Rank rank, const int64_t *__restrict__ rowptr, int64_t *__restrict__ cols,
+            int32_t *__restrict__ terms, int8_t *__restrict__ negs)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    int64_t p = rowptr[r];
+    int64_t self = row0 + r;
+    visit_row(
+        states[self], op, rank,
+        [&](uint32_t e, int64_t j, int neg) {
+            cols[p] = j;
+            terms[p] = op.tterm[e];
+            negs[p] = (int8_t)neg;
+            ++p;
+        },
+        [&](uint32_t e) {
+            cols[p] = self;
+            terms[p] = op.tterm[e];
+            negs[p] = 0;
+            ++p;
+        });
+}
+
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_conn_totals(int64_t D, const uint64_t *__restrict__ states, OpDev op, Rank rank,
+              unsigned long long *__restrict__ tot)
+{
+    __shared__ unsigned long long sh[2];
+    if (threadIdx.x < 2) sh[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long all = 0, off = 0;
+    if (i < D)
+        visit_row(states[i], op, rank, [&](uint32_t, int64_t, int) { ++all; ++off; },
+                  [&](uint32_t) { ++all; });
+    atomicAdd(&sh[0], all);
+    atomicAdd(&sh[1], off);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&tot[0], sh[0]);
+        atomicAdd(&tot[1], sh[1]);
+    }
+}
+
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_verify_rank(int64_t D, const uint64_t *__restrict__ states, Rank rank,
+              unsigned int *__restrict__ bad)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    if (rank(states[i]) != i) atomicAdd(bad, 1u);
+}
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + kRowBlock - 1) / kRowBlock); }
+
+OpDev opdev(const edm_op *op, const OpTable &t)
+{
+    return OpDev{t.d_keyptr, t.d_in, t.d_par, t.d_amp, t.d_term, op->has_onebody ? 1 : 0};
+}
+
+bool use_lin(const edm_op *op, const edm_basis *b)
+{
+    return b->rank_mode == 1 && op_conserves(op, b);
+}
+
+RankLin make_lin(const edm_basis *b)
+{
+    return RankLin{b->d_jlo, b->d_jhi, b->nlo, (1ULL << b->nlo) - 1ULL};
+}
+
+RankSearch make_search(const edm_basis *b) { return RankSearch{b->d_states, b->D}; }
+
+}  // namespace
+
+// Every full-list term keeps the sector: same particle number in each conserved component
+// and the same total momentum mod Gk on both sides. Then every target of a sector state is
+// in the sector, and a complete sector's rank tables give its exact index.
+static bool op_conserves_uncached(const edm_op *op, const edm_basis *b);
+
+bool op_conserves(const edm_op *op, const edm_basis *b)
+{
+    for (auto &kv : op->conserve_cache)
+        if (kv.first == b->id) return kv.second;
+    bool r = op_conserves_uncached(op, b);
+    op->conserve_cache.emplace_back(b->id, r);
+    return r;
+}
+
+static bool op_conserves_uncached(const edm_op *op, const edm_basis *b)
+{
+    if (!b->has_spec) return false;
+    const SystemSpec &sp = b->spec;
+    if (op->nbits != sp.nbits) return false;
+    for (int64_t e = 0; e < op->S_full; ++e) {
+        uint64_t o = op->out[e], i = op->in[e];
+        int64_t mo[3] = {0, 0, 0}, mi[3] = {0, 0, 0};
+        int co[8] = {0}, ci[8] = {0};
+        for (uint64_t v = o; v; v &= v - 1) {
+            int bb = __builtin_ctzll(v);
+            if (bb >= sp.nbits) return false;
+            co[sp.comp_of_bit(bb)]++;
+            for (int d = 0; d < sp.dim; ++d) mo[d] += sp.orb_k(bb, d);
+        }
+        for (uint64_t v = i; v; v &= v - 1) {
+            int bb = __builtin_ctzll(v);
+            if (bb >= sp.nbits) return false;
+            ci[sp.comp_of_bit(bb)]++;
+            for (int d = 0; d < sp.dim; ++d) mi[d] += sp.orb_k(bb, d);
+        }
+        for (int c = 0; c < sp.ncc; ++c)
+            if (co[c] != ci[c]) return false;
+        for (int d = 0; d < sp.dim; ++d) {
+            int64_t diff = mo[d] - mi[d];
+            if (sp.gk[d] != 0 ? (diff % sp.gk[d]) != 0 : diff != 0) return false;
+        }
+    }
+    return true;
+}
+
+bool verify_rank_tables(edm_basis *b)
+{
+    edm_ctx *ctx = b->ctx;
+    unsigned int *d_bad = (unsigned int *)ctx->alloc(sizeof(unsigned int));
+    EDM_HIP_CHECK(hipMemsetAsync(d_bad, 0, sizeof(unsigned int), ctx->stream));
+    if (b->D > 0)
+        hipLaunchKernelGGL(k_verify_rank<RankLin>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                           ctx->stream, b->D, b->d_states, make_lin(b), d_bad);
+    EDM_HIP_CHECK(hipGetLastError());
+    unsigned int bad = 0;
+    EDM_HIP_CHECK(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost, ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->free(d_bad, sizeof(unsigned int));
+    return bad == 0;
+}
+
+void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
+                   double2 *y, ApplyMode mode)
+{
+    EDM_REQUIRE(op->nbits == b->nbits, EDM_EINVAL, "operator and basis have different bits");
+    if (b->D == 0) return;
+    if (op->fast) {
+        OpDev od = opdev(op, mode == ApplyMode::Forward ? op->fwd : op->adj);
+        if (use_lin(op, b))
+            hipLaunchKernelGGL(k_matvec_fast<RankLin>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                               ctx->stream, b->D, b->d_states, od, make_lin(b), x, y);
+        else
+            hipLaunchKernelGGL(k_matvec_fast<RankSearch>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                               ctx->stream, b->D, b->d_states, od, make_search(b), x, y);
+    } else {
+        int adj = mode == ApplyMode::Adjoint ? 1 : 0;
+        if (use_lin(op, b))
+            hipLaunchKernelGGL(k_matvec_generic<RankLin>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                               ctx->stream, b->D, b->d_states, op->S_full, op->d_amp, op->d_out,
+                               op->d_in, op->d_par, make_lin(b), x, y, adj);
+        else
+            hipLaunchKernelGGL(k_matvec_generic<RankSearch>, dim3(nblk(b->D)), dim3(kRowBlock),
+                               0, ctx->stream, b->D, b->d_states, op->S_full, op->d_amp,
+                               op->d_out, op->d_in, op->d_par, make_search(b), x, y, adj);
+    }
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
+                         int64_t nrows, int64_t *d_counts, const int64_t *d_rowptr,
+                         int64_t *d_cols, int32_t *d_terms, int8_t *d_negs)
+{
+    EDM_REQUIRE(op->fast, EDM_EINVAL,
+                "connectivity dump supports one- and two-body number-conserving operators");
+    if (nrows == 0) return;
+    OpDev od = opdev(op, op->fwd);
+    bool lin = use_lin(op, b);
+    if (d_counts) {
+        if (lin)
+            hipLaunchKernelGGL(k_conn_count<RankLin>, dim3(nblk(nrows)), dim3(kRowBlock), 0,
+                               ctx->stream, row0, nrows, b->d_states, od, make_lin(b), d_counts);
+        else
+            hipLaunchKernelGGL(k_conn_count<RankSearch>, dim3(nblk(nrows)), dim3(kRowBlock), 0,
+                               ctx->stream, row0, nrows, b->d_states, od, make_search(b),
+                               d_counts);
+    } else {
+        if (lin)
+            hipLaunchKernelGGL(k_conn_fill<RankLin>, dim3(nblk(nrows)), dim3(kRowBlock), 0,
+                               ctx->stream, row0, nrows, b->d_states, od, make_lin(b), d_rowptr,
+                               d_cols, d_terms, d_negs);
+        else
+            hipLaunchKernelGGL(k_conn_fill<RankSearch>, dim3(nblk(nrows)), dim3(kRowBlock), 0,
+                               ctx->stream, row0, nrows, b->d_states, od, make_search(b),
+                               d_rowptr, d_cols, d_terms, d_negs);
+    }
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_conn_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t *nnz,
+                        int64_t *nnz_off)
+{
+    EDM_REQUIRE(op->fast, EDM_EINVAL,
+                "connectivity totals support one- and two-body number-conserving operators");
+    unsigned long long *d = (unsigned long long *)ctx->alloc(2 * sizeof(unsigned long long));
+    EDM_HIP_CHECK(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    OpDev od = opdev(op, op->fwd);
+    if (b->D > 0) {
+        if (use_lin(op, b))
+            hipLaunchKernelGGL(k_conn_totals<RankLin>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                               ctx->stream, b->D, b->d_states, od, make_lin(b), d);
+        else
+            hipLaunchKernelGGL(k_conn_totals<RankSearch>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                               ctx->stream, b->D, b->d_states, od, make_search(b), d);
+        EDM_HIP_CHECK(hipGetLastError());
+    }
+    unsigned long long h[2] = {0, 0};
+    EDM_HIP_CHECK(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->free(d, 2 * sizeof(unsigned long long));
+    *nnz = (int64_t)h[0];
+    *nnz_off = (int64_t)h[1];
+}
+
+// ------------------------------------------------------------------ operator build
+namespace {
+
+uint64_t parity_mask(uint64_t occ)   // EDCore/src/types/scattering.jl:7-18
+{
+    uint64_t mask = 0;
+    while (occ) {
+        int tz1 = __builtin_ctzll(occ);
+        occ &= occ - 1;
+        int tz2 = occ ? __builtin_ctzll(occ) : 64;
+        if (occ) occ &= occ - 1;
+        mask += tz2 >= 64 ? 0ULL : (1ULL << tz2);
+        mask -= tz1 + 1 >= 64 ? 0ULL : (1ULL << (tz1 + 1));
+    }
+    return mask;
+}
+
+int out_key(uint64_t outm)
+{
+    int pc = __builtin_popcountll(outm);
+    if (pc == 1) return __builtin_ctzll(outm);
+    int a = __builtin_ctzll(outm);
+    int b = 63 - __builtin_clzll(outm);
+    return 64 + b * (b - 1) / 2 + a;
+}
+
+void upload_table(edm_ctx *ctx, edm_op *op, OpTable &t, bool adjoint)
+{
+    const int64_t S = op->S_full;
+    std::vector<std::vector<int64_t>> buckets(EDM_NKEYS);
+    for (int64_t e = 0; e < S; ++e) {
+        uint64_t outm = adjoint ? op->in[e] : op->out[e];
+        buckets[out_key(outm)].push_back(e);
+    }
+    std::vector<uint32_t> keyptr(EDM_NKEYS + 1, 0);
+    std::vector<uint64_t> tin;
+    std::vector<uint64_t> tpar;
+    std::vector<double2> tamp;
+    std::vector<int32_t> tterm;
+    tin.reserve(S); tpar.reserve(S); tamp.reserve(S); tterm.reserve(S);
+    for (int k = 0; k < EDM_NKEYS; ++k) {
+        keyptr[k] = (uint32_t)tin.size();
+        auto &bk = buckets[k];
+        // ascending annihilation mask inside a bucket
+        std::stable_sort(bk.begin(), bk.end(), [&](int64_t a, int64_t b) {
+            uint64_t ia = adjoint ? op->out[a] : op->in[a];
+            uint64_t ib = adjoint ? op->out[b] : op->in[b];
+            return ia < ib;
+        });
+        for (int64_t e : bk) {
+            tin.push_back(adjoint ? op->out[e] : op->in[e]);
+            tpar.push_back(op->par[e]);
+            double im = op->amp[2 * e + 1];
+            tamp.push_back(make_double2(op->amp[2 * e], adjoint ? -im : im));
+            tterm.push_back((int32_t)e);
+        }
+    }
+    keyptr[EDM_NKEYS] = (uint32_t)tin.size();
+    t.nent = S;
+    size_t n = std::max<size_t>(1, (size_t)S);
+    t.d_keyptr = (uint32_t *)ctx->alloc(keyptr.size() * sizeof(uint32_t));
+    t.d_in = (uint64_t *)ctx->alloc(n * sizeof(uint64_t));
+    t.d_par = (uint64_t *)ctx->alloc(n * sizeof(uint64_t));
+    t.d_amp = (double2 *)ctx->alloc(n * sizeof(double2));
+    t.d_term = (int32_t *)ctx->alloc(n * sizeof(int32_t));
+    hipStream_t st = ctx->stream;
+    EDM_HIP_CHECK(hipMemcpyAsync(t.d_keyptr, keyptr.data(), keyptr.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, st));
+    if (S) {
+        EDM_HIP_CHECK(hipMemcpyAsync(t.d_in, tin.data(), S * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        EDM_HIP_CHECK(hipMemcpyAsync(t.d_par, tpar.data(), S * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        EDM_HIP_CHECK(hipMemcpyAsync(t.d_amp, tamp.data(), S * sizeof(double2), hipMemcpyHostToDevice, st));
+        EDM_HIP_CHECK(hipMemcpyAsync(t.d_term, tterm.data(), S * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    }
+    EDM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void free_table(edm_ctx *ctx, OpTable &t)
+{
+    size_t n = std::max<size_t>(1, (size_t)t.nent);
+    ctx->free(t.d_keyptr, (EDM_NKEYS + 1) * sizeof(uint32_t));
+    ctx->free(t.d_in, n * sizeof(uint64_t));
+    ctx->free(t.d_par, n * sizeof(uint64_t));
+    ctx->free(t.d_amp, n * sizeof(double2));
+    ctx->free(t.d_term, n * sizeof(int32_t));
+    t = OpTable{};
+}
+
+}  // namespace
+
+edm_op *build_op(edm_ctx *ctx, int nbits, const double *amp, const uint64_t *out,
+                 const uint64_t *in, int64_t S, bool upper)
+{
+    EDM_REQUIRE(nbits >= 0 && nbits <= 64, EDM_EINVAL, "nbits must be 0..64");
+    EDM_REQUIRE(S >= 0, EDM_EINVAL, "negative term count");
+    EDM_REQUIRE(S == 0 || (amp && out && in), EDM_EINVAL, "null term array");
+    uint64_t lim = nbits >= 64 ? ~0ULL : ((1ULL << nbits) - 1ULL);
+    struct T { uint64_t in, out, par; double re, im; };
+    std::vector<T> t;
+    t.reserve(upper ? 2 * S : S);
+    for (int64_t s = 0; s < S; ++s) {
+        EDM_REQUIRE((out[s] & ~lim) == 0 && (in[s] & ~lim) == 0, EDM_EINVAL,
+                    "term mask outside the operator's bits");
+        if (upper)   // EDCore/src/types/scattering.jl:170 isupper; operator.jl:28-31 asserts it
+            EDM_REQUIRE(in[s] > out[s] || (in[s] == out[s] && amp[2 * s + 1] == 0.0), EDM_EINVAL,
+                        "All Scatter terms must be in the upper triangular.");
+        uint64_t p = parity_mask(out[s]) ^ parity_mask(in[s]);   // scattering.jl:60
+        t.push_back(T{in[s], out[s], p, amp[2 * s], amp[2 * s + 1]});
+    }
+    if (upper) {   // linear_map.jl:24-32: append adjoints of non-diagonal terms, sort
+        for (int64_t s = 0; s < S; ++s)
+            if (in[s] != out[s]) t.push_back(T{out[s], in[s], t[s].par, amp[2 * s], -amp[2 * s + 1]});
+        std::stable_sort(t.begin(), t.end(), [](const T &a, const T &b) {
+            return a.in < b.in || (a.in == b.in && a.out < b.out);   // scattering.jl:205-207
+        });
+    }
+    edm_op *op = new edm_op();
+    op->ctx = ctx;
+    op->nbits = nbits;
+    op->upper = upper;
+    op->S_full = (int64_t)t.size();
+    op->amp.resize(2 * t.size());
+    op->out.resize(t.size());
+    op->in.resize(t.size());
+    op->par.resize(t.size());
+    bool fast = true, one = false;
+    for (size_t e = 0; e < t.size(); ++e) {
+        op->amp[2 * e] = t[e].re;
+        op->amp[2 * e + 1] = t[e].im;
+        op->out[e] = t[e].out;
+        op->in[e] = t[e].in;
+        op->par[e] = t[e].par;
+        int po = __builtin_popcountll(t[e].out), pi = __builtin_popcountll(t[e].in);
+        if (po != pi || po < 1 || po > 2) fast = false;
+        if (po == 1) one = true;
+    }
+    op->fast = fast;
+    op->has_onebody = one;
+    try {
+        if (fast) {
+            upload_table(ctx, op, op->fwd, false);
+            upload_table(ctx, op, op->adj, true);
+        }
+        size_t n = std::max<size_t>(1, t.size());
+        op->d_amp = (double2 *)ctx->alloc(n * sizeof(double2));
+        op->d_out = (uint64_t *)ctx->alloc(n * sizeof(uint64_t));
+        op->d_in = (uint64_t *)ctx->alloc(n * sizeof(uint64_t));
+        op->d_par = (uint64_t *)ctx->alloc(n * sizeof(uint64_t));
+        if (!t.empty()) {
+            hipStream_t st = ctx->stream;
+            EDM_HIP_CHECK(hipMemcpyAsync(op->d_amp, op->amp.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice, st));
+            EDM_HIP_CHECK(hipMemcpyAsync(op->d_out, op->out.data(), t.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+            EDM_HIP_CHECK(hipMemcpyAsync(op->d_in, op->in.data(), t.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+            EDM_HIP_CHECK(hipMemcpyAsync(op->d_par, op->par.data(), t.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+            EDM_HIP_CHECK(hipStreamSynchronize(st));
+        }
+    } catch (...) {
+        destroy_op(op);
+        throw;
+    }
+    return op;
+}
+
+void destroy_op(edm_op *op)
+{
+    if (!op) return;
+    edm_ctx *ctx = op->ctx;
+    if (op->fwd.d_keyptr) free_table(ctx, op->fwd);
+    if (op->adj.d_keyptr) free_table(ctx, op->adj);
+    size_t n = std::max<size_t>(1, (size_t)op->S_full);
+    ctx->free(op->d_amp, n * sizeof(double2));
+    ctx->free(op->d_out, n * sizeof(uint64_t));
+    ctx->free(op->d_in, n * sizeof(uint64_t));
+    ctx->free(op->d_par, n * sizeof(uint64_t));
+    delete op;
+}

@@ -1,0 +1,340 @@
+"""Device backend: the MI355X counterpart of the reference's GPU plugin contract.
+
+Mirrors MomentumED/src/method/gpu_interface.jl (flags, activation, the abstract linear-map
+hooks) and the per-vendor extension MomentumED/ext/MomentumEDCUDAExt.jl (device arrays,
+create_gpu_linearmap, the callable map, krylov_map_solve, release_gpu_memory), all backed by
+libedm.so through its C-ABI. The device symbol of this backend is "rocm"; "gpu" resolves to
+DEFAULT_GPU as in the reference (MomentumED.jl:446-453).
+"""
+from __future__ import annotations
+
+import ctypes
+import weakref
+
+import numpy as np
+
+from . import _lib
+from ._lib import EdmSolveInfo, EdmSystem, check
+
+__all__ = [
+    "GPU_AVAILABLE", "DEFAULT_GPU", "activate_gpu_device", "release_gpu_memory", "Context",
+    "get_context", "DeviceBasis", "DeviceOperator", "ROCmLinearMap", "create_gpu_linearmap",
+    "krylov_map_solve", "system_struct",
+]
+
+# gpu_interface.jl:25-30,39
+GPU_AVAILABLE = {"rocm": False}
+DEFAULT_GPU = {"value": "nogpu"}
+
+
+def activate_gpu_device(device: str = "rocm", set_default_gpu: bool = True) -> bool:
+    """gpu_interface.jl:41-60 + MomentumEDCUDAExt.jl:26-39 for the ROCm backend."""
+    if device not in GPU_AVAILABLE:
+        raise ValueError(f"Unknown GPU {device}. Valid options are: {list(GPU_AVAILABLE)}.")
+    L = _lib.load()
+    ok = bool(L.edm_device_available())
+    GPU_AVAILABLE[device] = ok
+    if ok and set_default_gpu:
+        DEFAULT_GPU["value"] = device
+    return ok
+
+
+def throw_gpu_unavailable(device: str):
+    """gpu_interface.jl:63-80."""
+    if GPU_AVAILABLE.get(device):
+        return
+    try:
+        activate_gpu_device(device, set_default_gpu=False)
+    except RuntimeError:
+        raise
+    if not GPU_AVAILABLE.get(device):
+        raise ValueError(f"device=:{device} requires functional GPU(s) of the type. "
+                         "Load the required package(s) and verify GPU availability.")
+
+
+# ------------------------------------------------------------------------- context
+class Context:
+    """One libedm context (one HIP stream) on one device."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self.L = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.L.edm_ctx_create(device, hbm_budget_bytes, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        self._fin = weakref.finalize(self, self.L.edm_ctx_destroy, h)
+
+    def close(self):
+        self._fin()
+
+    def synchronize(self):
+        check(self.L.edm_synchronize(self.h), self.h)
+
+    @property
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        check(self.L.edm_ctx_stream(self.h, ctypes.byref(s)), self.h)
+        return s.value or 0
+
+    def memory(self):
+        u, b = ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.L.edm_ctx_memory(self.h, ctypes.byref(u), ctypes.byref(b)), self.h)
+        return u.value, b.value
+
+
+_CONTEXTS: dict[int, Context] = {}
+
+
+def get_context(device: int = 0) -> Context:
+    ctx = _CONTEXTS.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        _CONTEXTS[device] = ctx
+    return ctx
+
+
+def release_gpu_memory(device: str = "rocm", level: int = 2):
+    """gpu_interface.jl:87 / MomentumEDCUDAExt.jl:43-47: drop cached device state."""
+    import gc
+    if level >= 1:
+        gc.collect()
+    if level >= 2:
+        for ctx in list(_CONTEXTS.values()):
+            ctx.synchronize()
+
+
+def system_struct(k_list, Gk, Nc_mix, Nc_conserve, N_each):
+    """Build an edm_system (arrays are kept alive on the returned object)."""
+    k_list = np.asarray(k_list, dtype=np.int64)
+    if k_list.ndim == 1:
+        k_list = k_list.reshape(1, -1)
+    dim, nk = k_list.shape
+    kl = np.ascontiguousarray(k_list.T.reshape(-1))          # column-major
+    gk = np.ascontiguousarray(np.asarray(Gk, dtype=np.int64).reshape(-1))
+    ne = np.ascontiguousarray(np.asarray(N_each, dtype=np.int64).reshape(-1))
+    s = EdmSystem(dim, nk, Nc_mix, Nc_conserve,
+                  kl.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                  gk.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                  ne.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    s._keep = (kl, gk, ne)
+    return s
+
+
+# ------------------------------------------------------------------------- basis
+class DeviceBasis:
+    """A sorted basis resident on the device (with rank tables when it is a complete sector)."""
+
+    def __init__(self, ctx: Context, handle: ctypes.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+        D = ctypes.c_int64()
+        check(ctx.L.edm_basis_size(handle, ctypes.byref(D)), ctx.h)
+        self.D = D.value
+        self._fin = weakref.finalize(self, ctx.L.edm_basis_destroy, handle)
+
+    @classmethod
+    def enumerate(cls, ctx: Context, system, sector: int) -> "DeviceBasis":
+        h = ctypes.c_void_p()
+        check(ctx.L.edm_basis_enumerate(ctx.h, ctypes.byref(system), sector, ctypes.byref(h)),
+              ctx.h)
+        return cls(ctx, h)
+
+    @classmethod
+    def upload(cls, ctx: Context, states: np.ndarray, nbits: int, system=None,
+               sector: int = -1) -> "DeviceBasis":
+        states = np.ascontiguousarray(states, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        sysp = ctypes.byref(system) if system is not None else None
+        check(ctx.L.edm_basis_upload(ctx.h, states.ctypes.data if states.size else None,
+                                     states.size, nbits, sysp, sector, ctypes.byref(h)), ctx.h)
+        return cls(ctx, h)
+
+    def download(self) -> np.ndarray:
+        out = np.empty(max(self.D, 1), dtype=np.uint64)
+        check(self.ctx.L.edm_basis_download(self.h, out.ctypes.data), self.ctx.h)
+        return out[: self.D]
+
+    @property
+    def rank_mode(self) -> int:
+        m = ctypes.c_int32()
+        check(self.ctx.L.edm_basis_rank_mode(self.h, ctypes.byref(m)), self.ctx.h)
+        return m.value
+
+    @property
+    def device_ptr(self) -> int:
+        p = ctypes.c_void_p()
+        check(self.ctx.L.edm_basis_device_ptr(self.h, ctypes.byref(p)), self.ctx.h)
+        return p.value or 0
+
+    def close(self):
+        self._fin()
+
+
+# ------------------------------------------------------------------------- operator
+class DeviceOperator:
+    """A term list on the device, expanded to the full list when upper-Hermitian
+    (linear_map.jl:22-41) and bucketed by creation mask."""
+
+    def __init__(self, ctx: Context, nbits: int, amp, out, inn, upper_hermitian: bool):
+        amp = np.ascontiguousarray(amp, dtype=np.complex128)
+        out = np.ascontiguousarray(out, dtype=np.uint64)
+        inn = np.ascontiguousarray(inn, dtype=np.uint64)
+        h = ctypes.c_void_p()
+        S = amp.size
+        check(ctx.L.edm_op_upload(ctx.h, nbits, amp.ctypes.data if S else None,
+                                  out.ctypes.data if S else None, inn.ctypes.data if S else None,
+                                  S, int(bool(upper_hermitian)), ctypes.byref(h)), ctx.h)
+        self.ctx = ctx
+        self.h = h
+        self.nbits = nbits
+        self.upper_hermitian = bool(upper_hermitian)
+        n = ctypes.c_int64()
+        check(ctx.L.edm_op_size(h, ctypes.byref(n)), ctx.h)
+        self.S_full = n.value
+        self._fin = weakref.finalize(self, ctx.L.edm_op_destroy, h)
+
+    @classmethod
+    def from_mboperator(cls, ctx: Context, op, nbits: int) -> "DeviceOperator":
+        amp, out, inn = op.arrays()
+        return cls(ctx, nbits, amp, out, inn, op.upper_hermitian)
+
+    def full_list(self):
+        n = self.S_full
+        amp = np.empty(max(n, 1), dtype=np.complex128)
+        out = np.empty(max(n, 1), dtype=np.uint64)
+        inn = np.empty(max(n, 1), dtype=np.uint64)
+        par = np.empty(max(n, 1), dtype=np.uint64)
+        check(self.ctx.L.edm_op_full_list(self.h, amp.ctypes.data, out.ctypes.data,
+                                          inn.ctypes.data, par.ctypes.data), self.ctx.h)
+        return amp[:n], out[:n], inn[:n], par[:n]
+
+    def close(self):
+        self._fin()
+
+
+# ------------------------------------------------------------------------- linear map
+class ROCmLinearMap:
+    """The device linear map (the CuLinearMap counterpart, MomentumEDCUDAExt.jl:55-63,119-139).
+
+    ``A(y, x)`` computes y = H x for device tensors (torch, complex128, on the GPU) or host
+    numpy arrays; ``A.adjoint()`` gives the H^dagger map."""
+
+    def __init__(self, op: DeviceOperator, basis: DeviceBasis, space=None, adjoint: bool = False):
+        self.op = op
+        self.basis = basis
+        self.space = space
+        self.is_adjoint = adjoint
+
+    @property
+    def ctx(self) -> Context:
+        return self.op.ctx
+
+    def size(self):
+        return (self.basis.D, self.basis.D)
+
+    @property
+    def dtype(self):
+        return np.complex128
+
+    def adjoint(self) -> "ROCmLinearMap":
+        return ROCmLinearMap(self.op, self.basis, self.space, not self.is_adjoint)
+
+    def _check(self, y, x):
+        D = self.basis.D
+        if len(x) != D or len(y) != D:
+            raise ValueError("Dimension of Hamiltonian linear map mismatches vector length.")
+
+    def matvec_device(self, y_ptr: int, x_ptr: int):
+        """Stream-ordered y = H x on raw device pointers (no synchronisation)."""
+        L = self.ctx.L
+        f = L.edm_matvec_adjoint if self.is_adjoint else L.edm_matvec
+        check(f(self.ctx.h, self.op.h, self.basis.h, x_ptr, y_ptr), self.ctx.h)
+
+    def __call__(self, y, x=None):
+        if x is None:
+            x, y = y, None
+        if isinstance(x, np.ndarray):
+            x = np.ascontiguousarray(x, dtype=np.complex128)
+            if y is None:
+                y = np.empty_like(x)
+            self._check(y, x)
+            if self.is_adjoint:
+                import torch
+                xd = torch.from_numpy(x).to(f"cuda:{self.ctx.device}")
+                yd = torch.empty_like(xd)
+                self(yd, xd)
+                y[:] = yd.cpu().numpy()
+                return y
+            check(self.ctx.L.edm_matvec_host(self.ctx.h, self.op.h, self.basis.h,
+                                             x.ctypes.data, y.ctypes.data), self.ctx.h)
+            return y
+        import torch
+        if y is None:
+            y = torch.empty_like(x)
+        self._check(y, x)
+        if x.dtype != torch.complex128 or y.dtype != torch.complex128:
+            raise TypeError("device vectors must be complex128")
+        if not (x.is_contiguous() and y.is_contiguous()):
+            raise ValueError("device vectors must be contiguous")
+        torch.cuda.current_stream(x.device).synchronize()
+        self.matvec_device(y.data_ptr(), x.data_ptr())
+        self.ctx.synchronize()   # as gpu_matvec! synchronizes after each launch (:218)
+        return y
+
+
+def create_gpu_linearmap(op, space, device: str = "rocm", ctx: Context | None = None):
+    """gpu_interface.jl:85 / MomentumEDCUDAExt.jl:94-113: upload the operator (and the basis,
+    unless the subspace is already on the device) and return the device map."""
+    if device != "rocm":
+        raise ValueError(f"Unknown GPU {device}.")
+    ctx = ctx or get_context(0)
+    basis = device_basis_of(space, ctx)
+    dop = DeviceOperator.from_mboperator(ctx, op, space.bits)
+    return ROCmLinearMap(dop, basis, space)
+
+
+def device_basis_of(space, ctx: Context) -> DeviceBasis:
+    """The device copy of a HilbertSubspace (uploaded once and cached on the subspace)."""
+    dev = getattr(space, "_device", None)
+    if dev is not None and dev.ctx is ctx:
+        return dev
+    sysinfo = getattr(space, "sector_info", None)
+    if sysinfo is not None:
+        system, sector = sysinfo
+        dev = DeviceBasis.upload(ctx, space.list, space.bits, system, sector)
+    else:
+        dev = DeviceBasis.upload(ctx, space.list, space.bits)
+    space._device = dev
+    return dev
+
+
+def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True, vec0=None,
+                     krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100,
+                     seed: int = 0x5EED, want_vectors: bool = True, **_ignored):
+    """gpu_interface.jl:92-94 / MomentumEDCUDAExt.jl:159-179 with KrylovKit's defaults
+    (krylovdim 30, tol 1e-12, maxiter 100). Returns (vals, vecs, info); vecs is a list of
+    host complex128 arrays (or None)."""
+    if not ishermitian:
+        raise ValueError("the device eigensolver implements the Hermitian (:SR) case")
+    if H.is_adjoint:
+        H = H.adjoint()   # H is Hermitian: the adjoint map has the same spectrum
+    D = H.basis.D
+    N_eigen = min(N_eigen, D)
+    vals = np.empty(N_eigen, dtype=np.float64)
+    vecs = np.empty((N_eigen, D), dtype=np.complex128) if want_vectors else None
+    v0 = None
+    if vec0 is not None:
+        v0 = np.ascontiguousarray(np.asarray(vec0, dtype=np.complex128).reshape(-1))
+        if v0.size != D:
+            raise ValueError("Dimension of Hamiltonian linear map mismatches vector length.")
+    info = EdmSolveInfo()
+    ctx = H.ctx
+    rc = ctx.L.edm_eigsolve(ctx.h, H.op.h, H.basis.h, N_eigen, krylovdim, float(tol), maxiter,
+                            seed, v0.ctypes.data if v0 is not None else None, vals.ctypes.data,
+                            vecs.ctypes.data if vecs is not None else None, ctypes.byref(info))
+    if rc not in (_lib.EDM_OK, _lib.EDM_ENOCONV):
+        check(rc, ctx.h)
+    out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
+                "converged": info.converged, "krylovdim": info.krylovdim,
+                "max_residual": info.max_residual, "seconds": info.seconds}
+    return vals, (list(vecs) if vecs is not None else None), out_info

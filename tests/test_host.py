@@ -1,0 +1,217 @@
+"""Host-side logic of the momentumed package (no GPU): the Scatter / MBOperator /
+HilbertSubspace semantics against the reference's own unit tests, the term builders against
+the oracle's restatement, argument validation, and the C-ABI library's exports."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import momentumed as me
+from momentumed import (EDPara, ED_scatterlist_onebody, ED_scatterlist_twobody, HilbertSubspace,
+                        MBOperator, MBS64, MBS64Vector, Scatter, get_body, index_fit,
+                        isdiagonal, isupper, sort_merge_scatlist)
+from momentumed import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def M(bits, *occ):
+    return MBS64(bits, list(occ)).n
+
+
+# ------------------------------------------------ EDCore/test/test_scattering.jl:1-69
+def test_scatter_basic():
+    s1 = Scatter(1.0, 1, 2, bits=10)
+    assert s1.Amp == 1.0 and s1.out == M(10, 1) and s1.in_ == M(10, 2)
+    assert get_body(s1) == (1, 1)
+    s2 = Scatter(0.5, 1, 2, 4, 3, bits=10)
+    assert s2.Amp == 0.5 and s2.out == M(10, 1, 2) and s2.in_ == M(10, 3, 4)
+    assert get_body(s2) == (2, 2)
+    assert s1.adjoint() == Scatter(1.0, 2, 1, bits=10)
+    assert isdiagonal(Scatter(1.0, 1, 1, bits=10)) and not isdiagonal(s1)
+
+
+def test_scatter_normal_ordering():
+    s = Scatter(1.0 + 1.0j, 2, 1, bits=10, upper_hermitian=True)
+    assert s.in_ == M(10, 2) and s.out == M(10, 1) and s.Amp == 1.0 - 1.0j
+    s = Scatter(1.0 + 0.0j, 2, 1, 3, 4, bits=10, upper_hermitian=True)
+    assert s.in_ == M(10, 4, 3) and s.out == M(10, 2, 1) and s.Amp == 1.0
+    s = Scatter(1.0 + 0.0j, 1, 2, 3, 4, bits=10, upper_hermitian=True)
+    assert s.in_ == M(10, 4, 3) and s.out == M(10, 2, 1) and s.Amp == -1.0
+
+
+def test_scatter_operations_and_merge():
+    s1 = Scatter(1.0, 1, 2, bits=10)
+    s2 = Scatter(2.0, 1, 2, bits=10)
+    s3 = Scatter(1.0, 2, 1, bits=10)
+    s4 = Scatter(1.0, 1, 1, bits=10, upper_hermitian=True)
+    assert s1 == s2 and s1 != s3 and s3 < s1
+    assert (s1 + s2).Amp == 3.0 and (2 * s1).Amp == 2.0
+    merged = sort_merge_scatlist([s1, s2], check_upper=False)
+    assert len(merged) == 1 and merged[0].Amp == 3.0
+    merged = sort_merge_scatlist([s1, s3], check_upper=False)
+    assert len(merged) == 2 and merged[0] < merged[1]
+    with pytest.raises(AssertionError):
+        sort_merge_scatlist([s3], check_upper=True)
+    assert sort_merge_scatlist([s4], check_upper=True) == [s4]
+
+
+def test_scatter_nbody_normal_ordering():
+    """N=3 path (scattering.jl:63-100): reordering creation/annihilation indices into
+    descending order flips the sign once per transposition."""
+    a = Scatter(1.0, 5, 3, 1, 2, 4, 6, bits=8)   # c†5 c†3 c†1 c6 c4 c2
+    b = Scatter(1.0, 3, 5, 1, 2, 4, 6, bits=8)   # one swap among the creations
+    c = Scatter(1.0, 3, 5, 1, 4, 2, 6, bits=8)   # and one among the annihilations
+    assert a.Amp == 1.0 and b.Amp == -1.0 and c.Amp == 1.0
+    assert a.out == M(8, 1, 3, 5) and a.in_ == M(8, 2, 4, 6)
+    assert Scatter(1.0, 1, 1, 2, 3, 4, 5, bits=8).Amp == 0
+
+
+# ------------------------------------------------ test_operator.jl, test_hilbert_subspace.jl
+def test_mboperator_reference_vectors():
+    s1 = Scatter(1.0 + 0.0j, 1, 2, bits=10)
+    s2 = Scatter(0.5 + 0.0j, 1, 2, 4, 3, bits=10)
+    op = MBOperator([s1], [s2], upper_hermitian=False)
+    assert not isupper(op) and len(op.scats) == 2
+    assert get_body(op.scats[0]) == (1, 1) and get_body(op.scats[1]) == (2, 2)
+    with pytest.raises(AssertionError):
+        MBOperator([Scatter(1.0, 2, 1, bits=4)], upper_hermitian=True)
+
+
+def test_hilbert_subspace_reference_vectors():
+    space = HilbertSubspace([i for i in range(5)], bits=8)
+    assert len(space) == 5 and space.bits == 8
+    assert space.get(2) == 3
+    assert not index_fit(space.get(5), space, 5)
+    v = np.arange(5) + 1j
+    mv = MBS64Vector(v, space)
+    assert mv.dot(mv) == pytest.approx(np.vdot(v, v))
+    with pytest.warns(UserWarning):
+        HilbertSubspace([3, 1, 2], bits=4)
+
+
+def test_mbs64_reference_vectors():
+    # EDCore/test/test_manybodystate_basis.jl:1-36
+    assert MBS64(8, n=10).n == 10
+    assert (MBS64(2, n=1) * MBS64(3, n=2)) == MBS64(5, n=10)
+    assert me.occ_list(10) == [2, 4] and me.make_mask64([2, 4]) == 10
+    with pytest.raises(AssertionError):
+        MBS64(4, n=16)
+
+
+def test_parity_mask_matches_oracle(oracle_mod):
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        v = int(rng.integers(0, 2 ** 63)) | (int(rng.integers(0, 2)) << 63)
+        assert me.parity_mask(v) == oracle_mod.parity_mask(v)
+
+
+# ------------------------------------------------ EDPara validation
+def test_edpara_validation():
+    k = np.zeros((2, 33), dtype=np.int64)
+    with pytest.raises(AssertionError):
+        EDPara(k_list=k, Gk=(1, 1), Nc_conserve=2)           # 66 orbitals > 64 bits
+    with pytest.raises(AssertionError):
+        EDPara(k_list=np.zeros((2, 3), dtype=np.int64), Gk=(-1, 1))
+    p = EDPara(k_list=np.zeros((2, 3), dtype=np.int64), Gk=(1, 1))
+    assert p.Nk == 3 and p.Nc == 1 and p.bits == 3
+    assert me.momentum_residue((7, -1), (3, 0)) == (1, -1)
+
+
+# ------------------------------------------------ term builders vs the oracle restatement
+def _oracle_terms(O, para):
+    t2 = O.scatterlist_twobody(para.k_list, para.Gk, para.Nc_mix, para.Nc_conserve, para.H_two,
+                               para.two_momentum_coordinate)
+    t1 = []
+    if para._H_one_given:
+        t1 = O.scatterlist_onebody(para.k_list, para.Gk, para.Nc_mix, para.Nc_conserve,
+                                   para.H_one, para.one_momentum_coordinate)
+    return O._sort_merge(t1 + t2)
+
+
+@pytest.mark.parametrize("name", ["small", "medium"])
+def test_term_builder_matches_oracle(oracle_mod, name):
+    from momentumed import workloads
+    w = workloads.get(name)
+    want = _oracle_terms(oracle_mod, w.para)
+    got = w.operator().scats
+    assert len(got) == len(want)
+    for s, (a, o, i) in zip(got, want):
+        assert (s.out, s.in_) == (o, i) and s.Amp == a
+
+
+def test_term_builder_multicomponent_matches_oracle(oracle_mod):
+    from momentumed.workloads import CoulombGaussian, mesh
+    k = mesh(2, 3)
+
+    def h1(kc, cf, ci):
+        return 0.25 + 0.5j if (cf, ci) == (1, 2) else (0.25 - 0.5j if (cf, ci) == (2, 1) else 0j)
+    para = EDPara(k_list=k, Gk=(2, 3), Nc_mix=2, Nc_conserve=2, H_one=h1,
+                  H_two=CoulombGaussian(k, (2, 3), perturb=0.02, seed=5),
+                  two_momentum_coordinate=False)
+    want = _oracle_terms(oracle_mod, para)
+    got = MBOperator(ED_scatterlist_onebody(para), ED_scatterlist_twobody(para),
+                     upper_hermitian=True).scats
+    assert [(s.Amp, s.out, s.in_) for s in got] == [tuple(t) for t in want]
+
+
+def test_kshift_coordinates_are_exact():
+    from fractions import Fraction
+    seen = []
+    para = EDPara(k_list=np.array([[0, 1, 2], [0, 0, 0]]), Gk=(3, 1),
+                  H_one=lambda k, cf, ci: seen.append(k) or 1.0)
+    ED_scatterlist_onebody(para, kshift=(1, 0))
+    assert seen[1] == (Fraction(2, 3), Fraction(0))
+
+
+# ------------------------------------------------ solve argument handling (no GPU needed)
+def test_edsolve_argument_errors():
+    sp = HilbertSubspace([1, 2], bits=2)
+    op = MBOperator([Scatter(1.0, 1, 2, bits=2, upper_hermitian=True)], upper_hermitian=True)
+    with pytest.raises(AssertionError):
+        me.EDsolve(sp, op, N=0)
+    with pytest.raises(NotImplementedError):
+        me.EDsolve(sp, op, N=1, method="sparse")
+    with pytest.raises(NotImplementedError):
+        me.EDsolve(sp, op, N=1, device="cpu", method_info=False)
+    non_upper = MBOperator([Scatter(1.0, 1, 2, bits=2)], upper_hermitian=False)
+    with pytest.raises(AssertionError):
+        me.EDsolve(sp, non_upper, N=1, method_info=False)
+    with pytest.raises(ValueError):
+        me.EDsolve(sp, [Scatter(1.0, 1, 1, bits=2)], N=1)
+
+
+# ------------------------------------------------ the C-ABI library
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    header = open(os.path.join(ROOT, "include", "edm.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*(edm_[a-z0-9_]+)\(", header, re.M))
+    assert declared, "no declarations parsed"
+    assert declared == set(_lib.EXPORTED)
+    for name in declared:
+        assert isinstance(getattr(L, name), ctypes._CFuncPtr)
+
+
+def test_library_reports_errors_without_gpu():
+    L = _lib.load()
+    # bad system description: no device work is needed to reject it
+    import momentumed.device as dev
+    sysd = dev.system_struct(np.zeros((2, 40), dtype=np.int64), (1, 1), 1, 2, [1, 1])
+    n = ctypes.c_int64()
+    rc = L.edm_num_sectors(ctypes.byref(sysd), ctypes.byref(n))
+    assert rc == _lib.EDM_EINVAL
+    assert b"64 bits" in L.edm_last_error(None)
+    sysd = dev.system_struct(np.array([[0, 1, 2], [0, 0, 0]]), (3, 0), 1, 1, [2])
+    assert L.edm_num_sectors(ctypes.byref(sysd), ctypes.byref(n)) == 0
+    assert n.value == 3 * 1          # Gk=0 direction: range [0*2, 0*2] -> one value
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "momentumed.jl_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".h", ".jl")):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in src and "liboracle" not in src and "orc_" not in src, f

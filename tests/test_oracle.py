@@ -1,0 +1,226 @@
+"""Pins of the CPU oracle (oracle/edm_oracle.c, oracle/oracle.py, oracle/llt.py).
+
+Each oracle function is checked against something it does not contain: the reference's own
+unit-test vectors, numbers its notebooks print, an independent Jordan-Wigner Hamiltonian, or
+brute force. A planted-mistake test shows the Jordan-Wigner check catches a sign error.
+"""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from momentumed import EDPara, ED_scatterlist_onebody, ED_scatterlist_twobody, MBOperator
+from momentumed.workloads import CoulombGaussian, mesh
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "reference_printed_values.json")))
+
+
+# ------------------------------------------------------------------ bit rules
+def test_parity_mask_reference_vectors(oracle_mod):
+    O = oracle_mod
+    # EDCore/test/test_manybodystate_basis.jl:37-38 via scat_occ_number, which pairs bits
+    # the same way parity_mask does (manybodystate_basis.jl:397-408)
+    mbs = 0b1010                        # orbitals 2, 4
+    assert bin(mbs & O.parity_mask(0b1010)).count("1") == 0   # (2, 4)
+    assert bin(mbs & O.parity_mask(0b10001)).count("1") == 2  # (1, 5)
+
+
+def test_parity_mask_closed_form(oracle_mod):
+    O = oracle_mod
+    M = (1 << 64) - 1
+    for p in range(64):                  # unpaired top bit p -> bits p+1..63
+        assert O.parity_mask(1 << p) == (M ^ ((1 << (p + 1)) - 1) if p < 63 else 0)
+    for a in range(20):
+        for b in range(a + 1, 24):       # pair (a<b) -> bits a+1..b-1
+            assert O.parity_mask((1 << a) | (1 << b)) == ((1 << b) - (1 << (a + 1)))
+
+
+def test_apply_reference_vectors(oracle_mod):
+    O = oracle_mod
+    # EDCore/test/test_multiplication.jl:1-13 — Scatter(2.0, 1, 2) * |0010> = (2, |0001>)
+    out, inn = 0b0001, 0b0010
+    ok, t, neg = O.apply_left(0b0010, out, inn, O.parity_mask(out) ^ O.parity_mask(inn))
+    assert ok and t == 0b0001 and neg == 0
+    # Scatter(1.0, 2, 1) * |0101> = (1, |0110>)
+    out, inn = 0b0010, 0b0001
+    ok, t, neg = O.apply_left(0b0101, out, inn, O.parity_mask(out) ^ O.parity_mask(inn))
+    assert ok and t == 0b0110 and neg == 0
+
+
+def test_upper_hermitian_action_reference_vectors(oracle_mod):
+    # EDCore/test/test_multiplication.jl:39-61: Scatter(1+1im, 1, 2; upper_hermitian) acting on
+    # |01> gives (1-1im)|10>; on |10> gives (1+1im)|01>.
+    O = oracle_mod
+    terms = O.TermList([1 + 1j], [0b01], [0b10])
+    full = O.full_list(terms, True)
+    basis = np.array([0b01, 0b10], dtype=np.uint64)
+    y = O.matvec(basis, full, np.array([1, 0], dtype=np.complex128))
+    assert y[0] == 0 and y[1] == 1 - 1j
+    y = O.matvec(basis, full, np.array([0, 1], dtype=np.complex128))
+    assert y[0] == 1 + 1j and y[1] == 0
+
+
+# ------------------------------------------------------------------ enumeration
+def _enum(O, n, t):
+    k = np.zeros((1, n), dtype=np.int64)
+    lists, _ = O.enumerate_sectors(k, (1,), 1, 1, [t])
+    return [int(v) for v in lists[0]] if lists else []
+
+
+def test_colex_order_reference_vectors(oracle_mod):
+    O = oracle_mod
+    # EDCore/test/test_colex_mbslist.jl:10-17: (1,2),(1,3),(2,3),(1,4),(2,4),(3,4)
+    assert _enum(O, 4, 2) == [0b0011, 0b0101, 0b0110, 0b1001, 0b1010, 0b1100]
+    assert _enum(O, 3, 0) == [0]          # :22-24
+    assert _enum(O, 3, 3) == [0b111]      # :27-29
+    s = _enum(O, 5, 3)
+    assert len(s) == 10 and s == sorted(s)
+
+
+@pytest.mark.parametrize("entry", GOLDEN["sector_sizes"], ids=lambda e: e["source"][-30:])
+def test_sector_sizes_reference_notebooks(oracle_mod, entry):
+    k = mesh(*entry["mesh"])
+    lists, ks = oracle_mod.enumerate_sectors(k, tuple(entry["Gk"]), entry["Nc_mix"],
+                                             entry["Nc_conserve"], entry["N"])
+    sizes = [len(v) for v in lists]
+    if "sizes" in entry:
+        assert sizes == entry["sizes"]
+    else:
+        for idx, n in entry["sizes_known"].items():
+            assert sizes[int(idx)] == n
+    for v in lists:
+        assert np.all(v[1:] > v[:-1])
+
+
+def test_enumeration_bruteforce(oracle_mod):
+    """All 2^n states, filtered by particle numbers and total momentum, two components."""
+    k = mesh(2, 3)
+    Gk = (2, 3)
+    Ncm, Ncc, N = 1, 2, [2, 1]
+    lists, ks = oracle_mod.enumerate_sectors(k, Gk, Ncm, Ncc, N)
+    n = 6
+    want = {}
+    for s in range(1 << (2 * n)):
+        if bin(s & 0x3F).count("1") != N[0] or bin(s >> 6).count("1") != N[1]:
+            continue
+        kk = [0, 0]
+        for b in range(12):
+            if s >> b & 1:
+                kk[0] += int(k[0, b % 6])
+                kk[1] += int(k[1, b % 6])
+        want.setdefault((kk[0] % 2, kk[1] % 3), []).append(s)
+    assert len(lists) == len(want)
+    for v, kk in zip(lists, ks):
+        assert [int(x) for x in v] == want[kk]
+
+
+# ------------------------------------------------------------------ Hamiltonian physics
+def _dense_from_terms(O, para, N_each, ops):
+    amp, out, inn = MBOperator(*ops, upper_hermitian=True).arrays()
+    full = O.full_list(O.TermList(amp, out, inn), True)
+    lists, ks = O.enumerate_sectors(para.k_list, para.Gk, para.Nc_mix, para.Nc_conserve, N_each)
+    return lists, ks, full
+
+
+def _check_jw(O, para, N_each, with_one_body, coord_one=True):
+    ops = [ED_scatterlist_twobody(para)]
+    if with_one_body:
+        ops.insert(0, ED_scatterlist_onebody(para))
+    lists, ks, full = _dense_from_terms(O, para, N_each, ops)
+    Hjw = O.jw_hamiltonian(para.k_list, para.Gk, para.Nc_mix, para.Nc_conserve,
+                           H_one=para.H_one if with_one_body else None, H_two=para.H_two,
+                           coordinate_one=coord_one, coordinate_two=para.two_momentum_coordinate)
+    worst = 0.0
+    for basis in lists:
+        Hs = O.dense_matrix(basis, full)
+        Hb = O.jw_sector_block(Hjw, basis)
+        worst = max(worst, np.abs(Hs - Hb).max())
+        assert np.abs(Hs - Hs.conj().T).max() < 1e-14
+    return worst
+
+
+def test_jw_single_component(oracle_mod):
+    k = mesh(2, 5)
+    para = EDPara(k_list=k, Gk=(2, 5), H_one=lambda kc, cf, ci: complex(np.cos(2 * np.pi * kc[1])),
+                  H_two=CoulombGaussian(k, (2, 5), c=0.25, perturb=0.05, seed=7),
+                  two_momentum_coordinate=False)
+    assert _check_jw(oracle_mod, para, [4], True) < 1e-13
+
+
+def test_jw_two_conserved_components(oracle_mod):
+    k = mesh(2, 3)
+    para = EDPara(k_list=k, Gk=(2, 3), Nc_conserve=2,
+                  H_two=CoulombGaussian(k, (2, 3), c=0.3, perturb=0.05, seed=8, inter_ratio=0.7),
+                  two_momentum_coordinate=False)
+    assert _check_jw(oracle_mod, para, [2, 2], False) < 1e-13
+
+
+def test_jw_mixed_components(oracle_mod):
+    k = mesh(1, 5)
+
+    def h1(kc, cf, ci):   # Hermitian in (cf, ci): tunnelling 0.3 + 0.2i between components
+        if cf == ci:
+            return complex(0.5 * np.cos(2 * np.pi * float(kc[1])))
+        return 0.3 + 0.2j if cf < ci else 0.3 - 0.2j
+    para = EDPara(k_list=k, Gk=(1, 5), Nc_mix=2, H_one=h1,
+                  H_two=CoulombGaussian(k, (1, 5), c=0.3, perturb=0.05, seed=9),
+                  two_momentum_coordinate=False)
+    assert _check_jw(oracle_mod, para, [3], True) < 1e-13
+
+
+def test_jw_catches_a_planted_sign_error(oracle_mod):
+    """Plant a mistake in the oracle (drop the fermion sign) and check the JW test sees it."""
+    O = oracle_mod
+    k = mesh(2, 4)
+    para = EDPara(k_list=k, Gk=(2, 4), H_two=CoulombGaussian(k, (2, 4), c=0.25),
+                  two_momentum_coordinate=False)
+    lists, ks, full = _dense_from_terms(O, para, [3], [ED_scatterlist_twobody(para)])
+    broken = O.TermList(full.amp, full.out, full.inn, np.zeros_like(full.parity))
+    Hjw = O.jw_hamiltonian(para.k_list, para.Gk, 1, 1, H_two=para.H_two, coordinate_two=False)
+    diff = max(np.abs(O.dense_matrix(b, broken) - O.jw_sector_block(Hjw, b)).max() for b in lists)
+    assert diff > 1e-3
+
+
+# ------------------------------------------------------------------ golden end-to-end values
+def _llt_para(mesh_dims):
+    import llt
+    V = llt.LandauInteraction(llt.ReciprocalLattice.named("triangular"), (1, 0, 1, 0))
+    V.D_l = 10.0
+    V.mix = 0.0
+    return EDPara(k_list=mesh(*mesh_dims), Gk=tuple(mesh_dims), H_two=V)
+
+
+def test_term_count_reference_notebook(oracle_mod):
+    g = GOLDEN["term_counts"][0]
+    para = _llt_para(g["mesh"])
+    op = MBOperator(ED_scatterlist_twobody(para), upper_hermitian=True)
+    assert len(op.scats) == g["upper_twobody_terms"]
+
+
+def test_laughlin_energies_reference_notebook(oracle_mod):
+    g = GOLDEN["energies_per_Nk"][0]
+    para = _llt_para(g["mesh"])
+    lists, ks, full = _dense_from_terms(oracle_mod, para, [g["N"]], [ED_scatterlist_twobody(para)])
+    basis = lists[ks.index(tuple(g["sector"]))]
+    e = np.linalg.eigvalsh(oracle_mod.dense_matrix(basis, full))[:10] / para.Nk
+    np.testing.assert_allclose(e, g["values"], rtol=1e-10, atol=0)
+
+
+# ------------------------------------------------------------------ matvec consistency
+def test_oracle_matvec_equals_dense(oracle_mod):
+    from momentumed.workloads import small
+    w = small()
+    amp, out, inn = w.operator().arrays()
+    full = oracle_mod.full_list(oracle_mod.TermList(amp, out, inn), True)
+    lists, _ = oracle_mod.enumerate_sectors(w.para.k_list, w.para.Gk, 1, 1, [5])
+    rng = np.random.default_rng(3)
+    for basis in lists[:3]:
+        x = rng.normal(size=len(basis)) + 1j * rng.normal(size=len(basis))
+        H = oracle_mod.dense_matrix(basis, full)
+        y = oracle_mod.matvec(basis, full, x)
+        np.testing.assert_allclose(y, H @ x, rtol=0, atol=1e-13 * np.abs(y).max())
+        rows = np.array([0, 5, len(basis) - 1])
+        np.testing.assert_array_equal(oracle_mod.matvec(basis, full, x, rows=rows), y[rows])
