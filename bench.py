@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--solve", action="store_true", help="also time the lowest-nev eigensolve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, run 3 plain device copies of one vector "
+                         "(16*D bytes read + written each) to calibrate rocprof byte counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -136,6 +139,12 @@ def main():
     ms_events = ev0.elapsed_time(ev1)
     ms_per = ms_events / args.steps
     log(f"[bench] {args.steps} matvecs: {ms_per:.3f} ms each (events), wall {wall:.3f} s")
+
+    if args.copy_calibration:
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 copies of {16 * D} bytes")
 
     S = dop.S_full
     B_mv = D * 40 + nnz_off.value * 16 + S * 40

@@ -376,6 +376,22 @@ int edm_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t
         EDM_REQUIRE(ctx && op && b && rowptr && cols && terms && negs, EDM_EINVAL, "NULL argument");
         EDM_REQUIRE(row0 >= 0 && nrows >= 0 && row0 + nrows <= b->D, EDM_EINVAL, "row range");
         int64_t nnz = rowptr[nrows];
+        // the fill kernel writes rowptr[r]..rowptr[r+1]-1 for row r: check the caller's
+        // offsets against freshly computed counts before any device write
+        {
+            std::vector<int64_t> cnt((size_t)std::max<int64_t>(nrows, 1));
+            size_t cbytes = cnt.size() * sizeof(int64_t);
+            int64_t *d_cnt = (int64_t *)ctx->alloc(cbytes);
+            launch_connectivity(ctx, op, b, row0, nrows, d_cnt, nullptr, nullptr, nullptr, nullptr);
+            EDM_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_cnt, nrows * sizeof(int64_t),
+                                         hipMemcpyDeviceToHost, ctx->stream));
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            ctx->free(d_cnt, cbytes);
+            EDM_REQUIRE(rowptr[0] == 0, EDM_EINVAL, "rowptr[0] must be 0");
+            for (int64_t r = 0; r < nrows; ++r)
+                EDM_REQUIRE(rowptr[r + 1] - rowptr[r] == cnt[r], EDM_EINVAL,
+                            "rowptr is not the exclusive prefix of edm_connectivity_counts");
+        }
         size_t rb = (size_t)(nrows + 1) * sizeof(int64_t);
         size_t cb = (size_t)std::max<int64_t>(nnz, 1) * sizeof(int64_t);
         size_t tb = (size_t)std::max<int64_t>(nnz, 1) * sizeof(int32_t);

@@ -119,6 +119,8 @@ void edm_ctx::free(void *p, size_t bytes)
 double *edm_ctx::partials(size_t bytes)
 {
     if (bytes > partials_bytes) {
+        // kernels queued on the stream may still read the old buffer
+        EDM_HIP_CHECK(hipStreamSynchronize(stream));
         if (d_partials) free(d_partials, partials_bytes);
         d_partials = (double *)alloc(bytes);
         partials_bytes = bytes;

@@ -115,6 +115,23 @@ struct OpTable {
     int64_t nent = 0;
 };
 
+// Compact form of the same table for the LDS-staged kernel: off-diagonal entries only, each
+// annihilation mask coded in 16 bits (c | d << 8, d = 0xFF for one-body; c < d), bucket
+// offsets in 16 bits, and the diagonal amplitude of every key in its own array. Parity masks
+// are recomputed from the four bit positions, so an entry costs 2 bytes of LDS plus its
+// amplitude in global memory.
+struct CompactTable {
+    bool ok = false;
+    int nkeys = 0;                  // keys >= nkeys have no entries
+    int64_t noff = 0;               // off-diagonal entries
+    uint16_t *d_keyptr = nullptr;   // nkeys + 1
+    uint16_t *d_code = nullptr;     // noff
+    double2 *d_amp = nullptr;       // noff
+    double2 *d_diag = nullptr;      // nkeys (zero where a key has no diagonal term)
+    size_t lds_bytes = 0;
+    size_t off_diag = 0, off_code = 0;   // byte offsets inside the LDS image
+};
+
 struct edm_op {
     uint64_t id = edm_next_id();
     edm_ctx *ctx = nullptr;
@@ -130,6 +147,7 @@ struct edm_op {
     bool has_onebody = false;
     OpTable fwd;                    // H: right-apply keyed by out
     OpTable adj;                    // H^dagger: keyed by in (swapped roles, conj amp)
+    CompactTable cfwd, cadj;        // LDS-staged forms of fwd / adj
     // generic fallback: full list SoA on the device
     double2 *d_amp = nullptr;
     uint64_t *d_out = nullptr, *d_in = nullptr, *d_par = nullptr;

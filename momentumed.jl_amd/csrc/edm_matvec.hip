@@ -19,6 +19,8 @@
 #include "edm_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <numeric>
 
 namespace {
@@ -231,6 +233,114 @@ k_verify_rank(int64_t D, const uint64_t *__restrict__ states, Rank rank,
     if (rank(states[i]) != i) atomicAdd(bad, 1u);
 }
 
+// ---------------------------------------------------------------- LDS-staged matvec
+// The bucket offsets, 16-bit annihilation codes and per-key diagonal amplitudes are copied
+// into LDS once per workgroup; a row then reads its candidates from LDS (no L2 traffic for
+// the term table) and touches global memory only for the accepted entries' amplitudes, the
+// rank tables and the x gather.
+constexpr int kLdsBlock = 512;
+constexpr int kNumXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin across them
+
+struct CompactDev {
+    const uint16_t *keyptr;
+    const uint16_t *code;
+    const double2 *amp;
+    const double2 *diag;
+    int nkeys;
+    int noff;
+    int off_diag;   // byte offsets in the LDS image
+    int off_code;
+    int lds_bytes;
+};
+
+// bits p+1..q-1 (pair p<q) — parity mask of a two-orbital set (scattering.jl:7-18)
+__device__ __forceinline__ uint64_t pm_pair(int p, int q)
+{
+    return (1ULL << q) - (2ULL << p);
+}
+// bits p+1..63 — parity mask of a single orbital
+__device__ __forceinline__ uint64_t pm_one(int p)
+{
+    return p >= 63 ? 0ULL : (~0ULL << (p + 1));
+}
+
+// Consecutive workgroups land on different XCDs; remap so each XCD works through one
+// contiguous range of rows (its L2 then holds one neighbourhood of x, not eight).
+__device__ __forceinline__ int64_t xcd_block(int swz)
+{
+    int64_t bid = blockIdx.x;
+    if (!swz) return bid;
+    int64_t nb = gridDim.x;
+    int64_t per = nb / kNumXcd, rem = nb % kNumXcd;
+    int64_t xcd = bid % kNumXcd, idx = bid / kNumXcd;
+    return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+template <class Rank, bool HighFirst>
+__global__ void __launch_bounds__(kLdsBlock)
+k_matvec_lds(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
+             const double2 *__restrict__ x, double2 *__restrict__ y, int swz)
+{
+    extern __shared__ uint4 lds_u4[];
+    {   // stage the contiguous table image (keyptr | diag | code), 16 bytes per thread per step
+        const uint4 *g = (const uint4 *)t.keyptr;
+        int n16 = t.lds_bytes / 16;
+        for (int k = threadIdx.x; k < n16; k += blockDim.x) lds_u4[k] = g[k];
+    }
+    __syncthreads();
+    const unsigned char *base = (const unsigned char *)lds_u4;
+    const uint16_t *s_keyptr = (const uint16_t *)base;
+    const double2 *s_diag = (const double2 *)(base + t.off_diag);
+    const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
+
+    int64_t i = xcd_block(swz) * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint64_t s = states[i];
+    double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
+
+    auto bucket = [&](int key, uint64_t outm, uint64_t pm_out) {
+        double2 dg = s_diag[key];
+        dr += dg.x;
+        di += dg.y;
+        int e0 = s_keyptr[key], e1 = s_keyptr[key + 1];
+        uint64_t mid = s & ~outm;
+        for (int e = e0; e < e1; ++e) {
+            unsigned code = s_code[e];
+            int c = code & 0xFF, d = code >> 8;
+            uint64_t inm = (1ULL << c) | (d != 0xFF ? (1ULL << d) : 0ULL);
+            if (mid & inm) continue;
+            uint64_t pm_in = d != 0xFF ? pm_pair(c, d) : pm_one(c);
+            int neg = __popcll(mid & (pm_out ^ pm_in)) & 1;
+            int64_t j = rank(mid | inm);
+            if (j < 0) continue;
+            double2 a = t.amp[e];
+            double2 xv = x[j];
+            double pr = a.x * xv.x - a.y * xv.y;
+            double pi = a.x * xv.y + a.y * xv.x;
+            if (neg) { yr -= pr; yi -= pi; } else { yr += pr; yi += pi; }
+        }
+    };
+    uint64_t rem = s;
+    while (rem) {
+        int b = HighFirst ? 63 - __builtin_clzll(rem) : __builtin_ctzll(rem);
+        rem ^= 1ULL << b;
+        uint64_t ob = 1ULL << b;
+        if (b < t.nkeys) bucket(b, ob, pm_one(b));
+        uint64_t low = s & (ob - 1ULL);
+        int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            int a = HighFirst ? 63 - __builtin_clzll(low) : __builtin_ctzll(low);
+            low ^= 1ULL << a;
+            int key = kb + a;
+            if (key < t.nkeys) bucket(key, ob | (1ULL << a), pm_pair(a, b));
+        }
+    }
+    double2 xi = x[i];
+    yr += dr * xi.x - di * xi.y;
+    yi += dr * xi.y + di * xi.x;
+    y[i] = make_double2(yr, yi);
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kRowBlock - 1) / kRowBlock); }
 
 OpDev opdev(const edm_op *op, const OpTable &t)
@@ -313,12 +423,57 @@ bool verify_rank_tables(edm_basis *b)
     return bad == 0;
 }
 
+// Kernel selection for the bucketed path. EDM_MATVEC=basic forces the global-memory table
+// kernel; EDM_MATVEC=lo selects low-bit-first pair order in the LDS kernel; EDM_XCD_SWIZZLE=0
+// keeps the hardware's workgroup order (EDM_XCD_SWIZZLE=1 remaps blocks per XCD). Defaults: LDS
+// kernel, high-bit-first, hardware order (measured fastest on the 6x6 N=12 sector).
+static int matvec_variant()
+{
+    const char *v = getenv("EDM_MATVEC");
+    if (!v) return 2;
+    if (!strcmp(v, "basic")) return 0;
+    if (!strcmp(v, "lo")) return 1;
+    return 2;
+}
+
+static int xcd_swizzle_enabled()
+{
+    const char *v = getenv("EDM_XCD_SWIZZLE");
+    return (v && v[0] == '1') ? 1 : 0;
+}
+
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
                    double2 *y, ApplyMode mode)
 {
     EDM_REQUIRE(op->nbits == b->nbits, EDM_EINVAL, "operator and basis have different bits");
     if (b->D == 0) return;
     if (op->fast) {
+        const CompactTable &ct = (mode == ApplyMode::Adjoint && !op->upper) ? op->cadj : op->cfwd;
+        int variant = matvec_variant();
+        if (ct.ok && variant != 0) {
+            CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
+                          (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+            unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
+            int swz = xcd_swizzle_enabled();
+            bool lin = use_lin(op, b);
+            if (variant == 1) {
+                if (lin)
+                    hipLaunchKernelGGL((k_matvec_lds<RankLin, false>), dim3(nb), dim3(kLdsBlock),
+                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_lin(b), x, y, swz);
+                else
+                    hipLaunchKernelGGL((k_matvec_lds<RankSearch, false>), dim3(nb), dim3(kLdsBlock),
+                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_search(b), x, y, swz);
+            } else {
+                if (lin)
+                    hipLaunchKernelGGL((k_matvec_lds<RankLin, true>), dim3(nb), dim3(kLdsBlock),
+                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_lin(b), x, y, swz);
+                else
+                    hipLaunchKernelGGL((k_matvec_lds<RankSearch, true>), dim3(nb), dim3(kLdsBlock),
+                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_search(b), x, y, swz);
+            }
+            EDM_HIP_CHECK(hipGetLastError());
+            return;
+        }
         OpDev od = opdev(op, mode == ApplyMode::Forward ? op->fwd : op->adj);
         if (use_lin(op, b))
             hipLaunchKernelGGL(k_matvec_fast<RankLin>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
@@ -472,6 +627,78 @@ void upload_table(edm_ctx *ctx, edm_op *op, OpTable &t, bool adjoint)
     EDM_HIP_CHECK(hipStreamSynchronize(st));
 }
 
+// Build the LDS image of the right-apply table (adjoint: keyed by `in`, conjugated amplitudes).
+void upload_compact(edm_ctx *ctx, edm_op *op, CompactTable &t, bool adjoint)
+{
+    t = CompactTable{};
+    struct Ent { int key; uint64_t inm; uint16_t code; double re, im; };
+    std::vector<Ent> ents;
+    std::vector<double2> diag(EDM_NKEYS, make_double2(0.0, 0.0));
+    int maxkey = -1;
+    for (int64_t e = 0; e < op->S_full; ++e) {
+        uint64_t outm = adjoint ? op->in[e] : op->out[e];
+        uint64_t inm = adjoint ? op->out[e] : op->in[e];
+        double re = op->amp[2 * e], im = adjoint ? -op->amp[2 * e + 1] : op->amp[2 * e + 1];
+        int key = out_key(outm);
+        maxkey = std::max(maxkey, key);
+        if (inm == outm) {
+            diag[key].x += re;
+            diag[key].y += im;
+            continue;
+        }
+        int c = __builtin_ctzll(inm);
+        int d = __builtin_popcountll(inm) == 2 ? 63 - __builtin_clzll(inm) : 0xFF;
+        ents.push_back(Ent{key, inm, (uint16_t)(c | (d << 8)), re, im});
+    }
+    if (ents.size() > 65535) return;   // 16-bit offsets
+    int nkeys = maxkey + 1;
+    if (nkeys < 1) nkeys = 1;
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent &a, const Ent &b) {
+        return a.key < b.key || (a.key == b.key && a.inm < b.inm);
+    });
+    size_t off_diag = (((size_t)(nkeys + 1) * 2) + 15) & ~(size_t)15;
+    size_t off_code = off_diag + (size_t)nkeys * 16;
+    size_t lds = (off_code + ents.size() * 2 + 15) & ~(size_t)15;
+    if (lds > 64 * 1024) return;
+    std::vector<unsigned char> img(lds, 0);
+    uint16_t *kp = (uint16_t *)img.data();
+    double2 *dg = (double2 *)(img.data() + off_diag);
+    uint16_t *cd = (uint16_t *)(img.data() + off_code);
+    std::vector<double2> amp(std::max<size_t>(ents.size(), 1));
+    size_t p = 0;
+    for (int k = 0; k < nkeys; ++k) {
+        kp[k] = (uint16_t)p;
+        dg[k] = diag[k];
+        while (p < ents.size() && ents[p].key == k) {
+            cd[p] = ents[p].code;
+            amp[p] = make_double2(ents[p].re, ents[p].im);
+            ++p;
+        }
+    }
+    kp[nkeys] = (uint16_t)p;
+    t.nkeys = nkeys;
+    t.noff = (int64_t)ents.size();
+    t.off_diag = off_diag;
+    t.off_code = off_code;
+    t.lds_bytes = lds;
+    t.d_keyptr = (uint16_t *)ctx->alloc(lds);
+    t.d_diag = (double2 *)((unsigned char *)t.d_keyptr + off_diag);
+    t.d_code = (uint16_t *)((unsigned char *)t.d_keyptr + off_code);
+    t.d_amp = (double2 *)ctx->alloc(amp.size() * sizeof(double2));
+    EDM_HIP_CHECK(hipMemcpyAsync(t.d_keyptr, img.data(), lds, hipMemcpyHostToDevice, ctx->stream));
+    EDM_HIP_CHECK(hipMemcpyAsync(t.d_amp, amp.data(), amp.size() * sizeof(double2),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    t.ok = true;
+}
+
+void free_compact(edm_ctx *ctx, CompactTable &t)
+{
+    if (t.d_keyptr) ctx->free(t.d_keyptr, t.lds_bytes);
+    if (t.d_amp) ctx->free(t.d_amp, std::max<int64_t>(t.noff, 1) * sizeof(double2));
+    t = CompactTable{};
+}
+
 void free_table(edm_ctx *ctx, OpTable &t)
 {
     size_t n = std::max<size_t>(1, (size_t)t.nent);
@@ -537,6 +764,8 @@ edm_op *build_op(edm_ctx *ctx, int nbits, const double *amp, const uint64_t *out
         if (fast) {
             upload_table(ctx, op, op->fwd, false);
             upload_table(ctx, op, op->adj, true);
+            upload_compact(ctx, op, op->cfwd, false);
+            if (!upper) upload_compact(ctx, op, op->cadj, true);
         }
         size_t n = std::max<size_t>(1, t.size());
         op->d_amp = (double2 *)ctx->alloc(n * sizeof(double2));
@@ -564,6 +793,8 @@ void destroy_op(edm_op *op)
     edm_ctx *ctx = op->ctx;
     if (op->fwd.d_keyptr) free_table(ctx, op->fwd);
     if (op->adj.d_keyptr) free_table(ctx, op->adj);
+    free_compact(ctx, op->cfwd);
+    free_compact(ctx, op->cadj);
     size_t n = std::max<size_t>(1, (size_t)op->S_full);
     ctx->free(op->d_amp, n * sizeof(double2));
     ctx->free(op->d_out, n * sizeof(uint64_t));

@@ -81,6 +81,13 @@ def load(path: str | None = None) -> ctypes.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # PyTorch-ROCm ships its own libamdhip64.so.7 and loads it by path; libedm.so asks for the
+    # soname. Loading torch first makes libedm.so bind to the runtime torch already holds, so
+    # a process that uses both sees one HIP runtime (two runtimes cannot share the device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(p):
         raise RuntimeError(f"libedm.so is not built ({p}); run `python -c \"import "
                            f"__graft_entry__ as g; g.build()\"` or `make -C "

@@ -187,6 +187,22 @@ def test_matvec_matches_oracle(gpu_ctx, oracle_mod, name):
         np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b2, x), y)
 
 
+@pytest.mark.parametrize("variant", ["basic", "lo", "hi"])
+@pytest.mark.parametrize("swizzle", ["0", "1"])
+def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizzle):
+    """Every bucketed-kernel variant (global table; LDS table with low- or high-bit-first pair
+    order; with and without the XCD block remap) against the oracle."""
+    monkeypatch.setenv("EDM_MATVEC", variant)
+    monkeypatch.setenv("EDM_XCD_SWIZZLE", swizzle)
+    w = W.medium()
+    op = w.operator()
+    dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, w.para.bits)
+    b = dev.DeviceBasis.enumerate(gpu_ctx, _system(w.para, w.N_each), 0)
+    x = _rand(b.D, 31)
+    yo = oracle_mod.matvec(b.download(), _oracle_full(oracle_mod, op), x)
+    assert np.abs(_gpu_matvec(gpu_ctx, dop, b, x) - yo).max() <= Y_TOL * np.abs(yo).max()
+
+
 def test_matvec_partial_basis_drops_targets(gpu_ctx, oracle_mod):
     w = W.small()
     op = w.operator()
