@@ -35,6 +35,7 @@ void free_basis(edm_basis *b)
     ctx->free(b->d_states, (size_t)std::max<int64_t>(b->D, 1) * sizeof(uint64_t));
     if (b->d_jlo) ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
     if (b->d_jhi) ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
+    if (b->d_jlo16) ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
     delete b;
 }
 
@@ -47,7 +48,9 @@ void attach_rank(edm_basis *b)
     } else {
         b->ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
         b->ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
+        if (b->d_jlo16) b->ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
         b->d_jlo = b->d_jhi = nullptr;
+        b->d_jlo16 = nullptr;
     }
 }
 

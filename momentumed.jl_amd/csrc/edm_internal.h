@@ -96,6 +96,7 @@ struct edm_basis {
     int rank_mode = 0;      // 0 = binary search, 1 = rank tables
     int nlo = 0, nhi = 0;
     uint32_t *d_jlo = nullptr, *d_jhi = nullptr;
+    uint16_t *d_jlo16 = nullptr;   // 16-bit copy of d_jlo when every entry fits
     bool has_spec = false;
     SystemSpec spec;
     int64_t sector = -1;

@@ -40,6 +40,7 @@ struct RankLin {
     const uint32_t *jlo, *jhi;
     int nlo;
     uint64_t lomask;
+    const uint16_t *jlo16;   // 16-bit copy of jlo (null when an entry exceeds 65535)
     __device__ __forceinline__ int64_t operator()(uint64_t t) const
     {
         return (int64_t)jhi[t >> nlo] + (int64_t)jlo[t & lomask];
@@ -341,6 +342,146 @@ k_matvec_lds(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank 
     y[i] = make_double2(yr, yi);
 }
 
+// Two-phase partner lookup: issue() starts the table loads, finish() combines them, so a lane
+// can have the lookups of several candidates in flight before it needs any of them.
+struct LinPending { uint32_t h, l; };
+// Per-row view of the rank tables: a target whose high half equals the row's own reuses the
+// row's J_hi entry instead of loading it; J_lo is read from the 16-bit copy when one exists.
+struct LinRow {
+    const uint32_t *jhi;
+    const uint32_t *jlo;
+    const uint16_t *jlo16;
+    int nlo;
+    uint64_t lomask;
+    uint64_t hs;
+    uint32_t jhs;
+};
+__device__ __forceinline__ LinRow rank_row(const RankLin &r, uint64_t s)
+{
+    uint64_t hs = s >> r.nlo;
+    return LinRow{r.jhi, r.jlo, r.jlo16, r.nlo, r.lomask, hs, r.jhi[hs]};
+}
+__device__ __forceinline__ LinPending rank_issue(const LinRow &r, uint64_t t)
+{
+    uint64_t h = t >> r.nlo;
+    uint32_t hv = h == r.hs ? r.jhs : r.jhi[h];
+    uint32_t lv = r.jlo16 ? (uint32_t)r.jlo16[t & r.lomask] : r.jlo[t & r.lomask];
+    return LinPending{hv, lv};
+}
+__device__ __forceinline__ int64_t rank_finish(const LinRow &, LinPending p)
+{
+    return (int64_t)p.h + (int64_t)p.l;
+}
+__device__ __forceinline__ const RankSearch &rank_row(const RankSearch &r, uint64_t) { return r; }
+__device__ __forceinline__ int64_t rank_issue(const RankSearch &r, uint64_t t) { return r(t); }
+__device__ __forceinline__ int64_t rank_finish(const RankSearch &, int64_t j) { return j; }
+
+// Same traversal as k_matvec_lds, with U candidates of a bucket resolved together (U table
+// lookups in flight, then U gathers in flight) and an optional partner window: only partners
+// j in [jmin, jmax) are gathered, and the result is added to y when `accumulate` is set. A
+// matvec split into passes over windows of x keeps each pass's gather footprint small.
+template <class Rank, bool HighFirst, int U>
+__global__ void __launch_bounds__(kLdsBlock)
+k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
+              const double2 *__restrict__ x, double2 *__restrict__ y, int64_t jmin, int64_t jmax,
+              int accumulate, int with_diag)
+{
+    extern __shared__ uint4 lds_u4[];
+    {
+        const uint4 *g = (const uint4 *)t.keyptr;
+        int n16 = t.lds_bytes / 16;
+        for (int k = threadIdx.x; k < n16; k += blockDim.x) lds_u4[k] = g[k];
+    }
+    __syncthreads();
+    const unsigned char *base = (const unsigned char *)lds_u4;
+    const uint16_t *s_keyptr = (const uint16_t *)base;
+    const double2 *s_diag = (const double2 *)(base + t.off_diag);
+    const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
+
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint64_t s = states[i];
+    double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
+    const auto rrow = rank_row(rank, s);
+    using Pend = decltype(rank_issue(rrow, 0ULL));
+
+    auto bucket = [&](int key, uint64_t outm, uint64_t pm_out) {
+        if (with_diag) {
+            double2 dg = s_diag[key];
+            dr += dg.x;
+            di += dg.y;
+        }
+        int e0 = s_keyptr[key], e1 = s_keyptr[key + 1];
+        uint64_t mid = s & ~outm;
+        for (int e = e0; e < e1; e += U) {
+            Pend pend[U];
+            bool live[U];
+            int neg[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                live[u] = false;
+                neg[u] = 0;
+                if (e + u < e1) {
+                    unsigned code = s_code[e + u];
+                    int c = code & 0xFF, d = code >> 8;
+                    uint64_t inm = (1ULL << c) | (d != 0xFF ? (1ULL << d) : 0ULL);
+                    if (!(mid & inm)) {
+                        uint64_t pm_in = d != 0xFF ? pm_pair(c, d) : pm_one(c);
+                        neg[u] = __popcll(mid & (pm_out ^ pm_in)) & 1;
+                        pend[u] = rank_issue(rrow, mid | inm);
+                        live[u] = true;
+                    }
+                }
+            }
+            int64_t j[U];
+            double2 a[U], xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                j[u] = live[u] ? rank_finish(rrow, pend[u]) : -1;
+                live[u] = live[u] && j[u] >= jmin && j[u] < jmax;
+                if (live[u]) {
+                    a[u] = t.amp[e + u];
+                    xv[u] = x[j[u]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (live[u]) {
+                    double pr = a[u].x * xv[u].x - a[u].y * xv[u].y;
+                    double pi = a[u].x * xv[u].y + a[u].y * xv[u].x;
+                    if (neg[u]) { yr -= pr; yi -= pi; } else { yr += pr; yi += pi; }
+                }
+            }
+        }
+    };
+    uint64_t rem = s;
+    while (rem) {
+        int b = HighFirst ? 63 - __builtin_clzll(rem) : __builtin_ctzll(rem);
+        rem ^= 1ULL << b;
+        uint64_t ob = 1ULL << b;
+        if (b < t.nkeys) bucket(b, ob, pm_one(b));
+        uint64_t low = s & (ob - 1ULL);
+        int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            int a = HighFirst ? 63 - __builtin_clzll(low) : __builtin_ctzll(low);
+            low ^= 1ULL << a;
+            int key = kb + a;
+            if (key < t.nkeys) bucket(key, ob | (1ULL << a), pm_pair(a, b));
+        }
+    }
+    if (with_diag) {
+        double2 xi = x[i];
+        yr += dr * xi.x - di * xi.y;
+        yi += dr * xi.y + di * xi.x;
+    }
+    if (accumulate) {
+        double2 o = y[i];
+        yr += o.x;
+        yi += o.y;
+    }
+    y[i] = make_double2(yr, yi);
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kRowBlock - 1) / kRowBlock); }
 
 OpDev opdev(const edm_op *op, const OpTable &t)
@@ -355,7 +496,7 @@ bool use_lin(const edm_op *op, const edm_basis *b)
 
 RankLin make_lin(const edm_basis *b)
 {
-    return RankLin{b->d_jlo, b->d_jhi, b->nlo, (1ULL << b->nlo) - 1ULL};
+    return RankLin{b->d_jlo, b->d_jhi, b->nlo, (1ULL << b->nlo) - 1ULL, b->d_jlo16};
 }
 
 RankSearch make_search(const edm_basis *b) { return RankSearch{b->d_states, b->D}; }
@@ -423,23 +564,75 @@ bool verify_rank_tables(edm_basis *b)
     return bad == 0;
 }
 
-// Kernel selection for the bucketed path. EDM_MATVEC=basic forces the global-memory table
-// kernel; EDM_MATVEC=lo selects low-bit-first pair order in the LDS kernel; EDM_XCD_SWIZZLE=0
-// keeps the hardware's workgroup order (EDM_XCD_SWIZZLE=1 remaps blocks per XCD). Defaults: LDS
-// kernel, high-bit-first, hardware order (measured fastest on the 6x6 N=12 sector).
+// Kernel selection for the bucketed path (all variants compute the same y; the choice only
+// changes speed). Default: the LDS-staged kernel with per-row rank view, one candidate at a
+// time (u1) — the fastest measured on the 6x6 N=12 sector. EDM_MATVEC=u2|u4 resolves 2|4
+// candidates together; hi|lo select the LDS kernel without the per-row rank view, with
+// high- or low-bit-first pair order (EDM_XCD_SWIZZLE=1 then remaps workgroups per XCD);
+// basic forces the global-memory table kernel. EDM_MATVEC_SPLIT=n splits the u* kernels into
+// n passes over windows of x.
 static int matvec_variant()
 {
     const char *v = getenv("EDM_MATVEC");
-    if (!v) return 2;
+    if (!v) return 11;
     if (!strcmp(v, "basic")) return 0;
     if (!strcmp(v, "lo")) return 1;
-    return 2;
+    if (!strcmp(v, "hi")) return 2;
+    if (!strcmp(v, "u1")) return 11;
+    if (!strcmp(v, "u2")) return 12;
+    if (!strcmp(v, "u4")) return 14;
+    return 11;
 }
 
 static int xcd_swizzle_enabled()
 {
     const char *v = getenv("EDM_XCD_SWIZZLE");
     return (v && v[0] == '1') ? 1 : 0;
+}
+
+static int matvec_passes()
+{
+    const char *v = getenv("EDM_MATVEC_SPLIT");
+    int n = v ? atoi(v) : 1;
+    return n < 1 ? 1 : (n > 16 ? 16 : n);
+}
+
+template <class Rank, int U>
+static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                        const double2 *x, double2 *y)
+{
+    CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
+                  (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
+    int passes = matvec_passes();
+    for (int p = 0; p < passes; ++p) {
+        int64_t jmin = b->D * p / passes, jmax = b->D * (p + 1) / passes;
+        hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, U>), dim3(nb), dim3(kLdsBlock), ct.lds_bytes,
+                           ctx->stream, b->D, b->d_states, cd, rank, x, y, jmin, jmax, p > 0 ? 1 : 0,
+                           p == 0 ? 1 : 0);
+    }
+}
+
+template <class Rank>
+static void launch_compact(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                           const double2 *x, double2 *y, int variant)
+{
+    if (variant >= 11) {
+        if (variant == 12) launch_ldsu<Rank, 2>(ctx, b, ct, rank, x, y);
+        else if (variant == 14) launch_ldsu<Rank, 4>(ctx, b, ct, rank, x, y);
+        else launch_ldsu<Rank, 1>(ctx, b, ct, rank, x, y);
+        return;
+    }
+    CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
+                  (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
+    int swz = xcd_swizzle_enabled();
+    if (variant == 1)
+        hipLaunchKernelGGL((k_matvec_lds<Rank, false>), dim3(nb), dim3(kLdsBlock), ct.lds_bytes,
+                           ctx->stream, b->D, b->d_states, cd, rank, x, y, swz);
+    else
+        hipLaunchKernelGGL((k_matvec_lds<Rank, true>), dim3(nb), dim3(kLdsBlock), ct.lds_bytes,
+                           ctx->stream, b->D, b->d_states, cd, rank, x, y, swz);
 }
 
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
@@ -451,26 +644,10 @@ void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const dou
         const CompactTable &ct = (mode == ApplyMode::Adjoint && !op->upper) ? op->cadj : op->cfwd;
         int variant = matvec_variant();
         if (ct.ok && variant != 0) {
-            CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
-                          (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
-            unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
-            int swz = xcd_swizzle_enabled();
-            bool lin = use_lin(op, b);
-            if (variant == 1) {
-                if (lin)
-                    hipLaunchKernelGGL((k_matvec_lds<RankLin, false>), dim3(nb), dim3(kLdsBlock),
-                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_lin(b), x, y, swz);
-                else
-                    hipLaunchKernelGGL((k_matvec_lds<RankSearch, false>), dim3(nb), dim3(kLdsBlock),
-                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_search(b), x, y, swz);
-            } else {
-                if (lin)
-                    hipLaunchKernelGGL((k_matvec_lds<RankLin, true>), dim3(nb), dim3(kLdsBlock),
-                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_lin(b), x, y, swz);
-                else
-                    hipLaunchKernelGGL((k_matvec_lds<RankSearch, true>), dim3(nb), dim3(kLdsBlock),
-                                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, make_search(b), x, y, swz);
-            }
+            if (use_lin(op, b))
+                launch_compact(ctx, b, ct, make_lin(b), x, y, variant);
+            else
+                launch_compact(ctx, b, ct, make_search(b), x, y, variant);
             EDM_HIP_CHECK(hipGetLastError());
             return;
         }

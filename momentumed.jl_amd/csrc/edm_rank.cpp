@@ -13,6 +13,7 @@
 // searchsortedfirst position bit for bit. Two independent loads replace ~log2(D) dependent ones.
 #include "edm_internal.h"
 
+#include <algorithm>
 #include <unordered_map>
 
 namespace {
@@ -111,6 +112,15 @@ bool build_rank_tables(edm_basis *b)
                                  hipMemcpyHostToDevice, ctx->stream));
     EDM_HIP_CHECK(hipMemcpyAsync(b->d_jhi, jhi.data(), nhi_vals * sizeof(uint32_t),
                                  hipMemcpyHostToDevice, ctx->stream));
+    uint32_t mx = 0;
+    for (uint32_t v : jlo) mx = std::max(mx, v);
+    std::vector<uint16_t> j16;   // must outlive the asynchronous copy below
+    if (mx <= 0xFFFF) {
+        j16.assign(jlo.begin(), jlo.end());
+        b->d_jlo16 = (uint16_t *)ctx->alloc(nlo_vals * sizeof(uint16_t));
+        EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo16, j16.data(), nlo_vals * sizeof(uint16_t),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    }
     EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return true;
 }

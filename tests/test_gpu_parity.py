@@ -187,7 +187,7 @@ def test_matvec_matches_oracle(gpu_ctx, oracle_mod, name):
         np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b2, x), y)
 
 
-@pytest.mark.parametrize("variant", ["basic", "lo", "hi"])
+@pytest.mark.parametrize("variant", ["basic", "lo", "hi", "u1", "u2", "u4"])
 @pytest.mark.parametrize("swizzle", ["0", "1"])
 def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizzle):
     """Every bucketed-kernel variant (global table; LDS table with low- or high-bit-first pair
@@ -199,6 +199,20 @@ def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizz
     dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, w.para.bits)
     b = dev.DeviceBasis.enumerate(gpu_ctx, _system(w.para, w.N_each), 0)
     x = _rand(b.D, 31)
+    yo = oracle_mod.matvec(b.download(), _oracle_full(oracle_mod, op), x)
+    assert np.abs(_gpu_matvec(gpu_ctx, dop, b, x) - yo).max() <= Y_TOL * np.abs(yo).max()
+
+
+@pytest.mark.parametrize("passes", ["2", "3"])
+def test_matvec_windowed_passes(gpu_ctx, oracle_mod, monkeypatch, passes):
+    """The matvec split into passes over windows of x (accumulating into y)."""
+    monkeypatch.setenv("EDM_MATVEC", "u2")
+    monkeypatch.setenv("EDM_MATVEC_SPLIT", passes)
+    w = W.medium()
+    op = w.operator()
+    dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, w.para.bits)
+    b = dev.DeviceBasis.enumerate(gpu_ctx, _system(w.para, w.N_each), 0)
+    x = _rand(b.D, 32)
     yo = oracle_mod.matvec(b.download(), _oracle_full(oracle_mod, op), x)
     assert np.abs(_gpu_matvec(gpu_ctx, dop, b, x) - yo).max() <= Y_TOL * np.abs(yo).max()
 
