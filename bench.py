@@ -28,6 +28,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _kernel_name(rank_mode: int) -> str:
+    """Name of the matvec kernel libedm.so launches for the bucketed path (EDM_MATVEC)."""
+    rank = "RankLin" if rank_mode == 1 else "RankSearch"
+    v = os.environ.get("EDM_MATVEC", "u1")
+    if v == "basic":
+        return f"k_matvec_fast<{rank}>"
+    if v in ("hi", "lo"):
+        return f"k_matvec_lds<{rank}, {str(v == 'hi').lower()}>"
+    u = {"u2": 2, "u4": 4}.get(v, 1)
+    return f"k_matvec_ldsu<{rank}, true, {u}>"
+
+
 def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
     """The oracle's row-gather matvec (restatement of linear_map.jl:100-118: every term tried
     on every row, binary search for partners) on a random row sample, scaled to matvecs/s."""
@@ -151,7 +163,7 @@ def main():
     achieved = B_mv / (ms_per * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_matvec_fast<RankLin>" if basis.rank_mode == 1 else "k_matvec_fast<RankSearch>",
+                "kernel": _kernel_name(basis.rank_mode),
                 "bytes_model": "D*40 + nnz_off*16 + S_full*40", "algorithmic_bytes": B_mv}
     tr_path = os.path.join(ROOT, "profiles", f"matvec_{args.workload}_traffic.json")
     if os.path.exists(tr_path):
