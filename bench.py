@@ -8,7 +8,13 @@ library's stream, the CPU baseline on a bounded row sample, setup times). Progre
 stderr.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|medium|small]
-                    [--solve] [--no-cpu-baseline] [--cpu-seconds S]
+                    [--solve] [--no-cpu-baseline] [--cpu-seconds S] [--no-batched]
+                    [--no-served] [--batch-ratios R] [--serve-seconds T]
+
+Beside the matvec line it reports, in the same JSON object, the batched sweep (3,840
+two-component sector solves, batched vs a plain loop of single solves) and the served stream
+(Poisson requests of mixed sizes, continuous batching vs one solve at a time, p50/p99
+latency).
 """
 import argparse
 import ctypes
@@ -72,6 +78,105 @@ def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
     }, err
 
 
+def batched_section(args):
+    """Sector-solves/s of the batched sweep against the plain loop of single solves.
+
+    Problems: every (ratio, filling, sector) of the two-component 3x4 system (64 ratios x 5
+    fillings x 12 sectors), lowest 4 eigenvalues, krylovdim 30, tol 1e-12. The plain loop
+    (krylov_map_solve per problem, operators and bases already on the device) is timed on the
+    first ratio's 60 problems, which span every filling and sector; the batch solves all
+    problems. The sample's eigenvalues must be bitwise equal in both."""
+    import numpy as np
+    import momentumed as me
+    from momentumed import batch as B
+    from momentumed import device as dev
+    from momentumed import workloads as W
+    ctx = dev.get_context(0)
+    t0 = time.perf_counter()
+    spaces = []
+    for f in W.BATCH_FILLINGS:
+        subs, ks = me.ED_momentum_subspaces(W.batched_para(1.0), f)
+        for k, sp in zip(ks, subs):
+            spaces.append((f, k, sp))
+    bases = [dev.device_basis_of(sp, ctx) for _, _, sp in spaces]
+    t_enum = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ratios = W.batched_ratios(args.batch_ratios)
+    ops = []
+    for r in ratios:
+        op = me.MBOperator(me.ED_scatterlist_twobody(W.batched_para(r)), upper_hermitian=True)
+        ops.append(dev.DeviceOperator.from_mboperator(ctx, op, 24))
+    t_ops = time.perf_counter() - t0
+    maps = [dev.ROCmLinearMap(o, b) for o in ops for b in bases]
+    dims = [b.D for b in bases]
+    nev, kd, tol = W.BATCH_NEV, 30, 1e-12
+    log(f"[batched] {len(maps)} problems (D {min(dims)}..{max(dims)}), bases {t_enum:.2f} s, "
+        f"operators {t_ops:.2f} s")
+    nplain = min(len(bases), len(maps))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plain_vals, plain_mv = [], 0
+    for m in maps[:nplain]:
+        v, _, info = dev.krylov_map_solve(m, nev, krylovdim=kd, tol=tol, want_vectors=False)
+        plain_vals.append(v)
+        plain_mv += info["matvecs"]
+    t_plain = time.perf_counter() - t0
+    log(f"[batched] plain loop: {nplain} solves in {t_plain:.2f} s ({nplain / t_plain:.1f}/s)")
+    used0, _ = ctx.memory()
+    peak = used0
+    t0 = time.perf_counter()
+    with B.BatchSolver(ctx) as bs:
+        ids = [bs.add(m, nev, krylovdim=kd, tol=tol) for m in maps]
+        t_add = time.perf_counter() - t0
+        peak = max(peak, ctx.memory()[0])
+        res = bs.run()
+    t_batch = time.perf_counter() - t0
+    nconv = sum(1 for r in res.values() if r.converged)
+    mv = sum(r.info["matvecs"] for r in res.values())
+    same = all(np.array_equal(res[ids[i]].values, plain_vals[i]) for i in range(nplain))
+    log(f"[batched] batch: {len(maps)} solves in {t_batch:.2f} s ({len(maps) / t_batch:.1f}/s), "
+        f"{nconv} converged, sample bitwise equal: {same}")
+    sps_b, sps_p = len(maps) / t_batch, nplain / t_plain
+    return {
+        "workload": f"two-component 3x4 mesh, fillings {list(W.BATCH_FILLINGS)}, all 12 sectors, "
+                    f"{len(ratios)} inter-component ratios ~U(0.5,2) seed 99",
+        "problems": len(maps), "D_min": min(dims), "D_max": max(dims), "nev": nev,
+        "krylovdim": kd, "tol": tol, "converged": nconv,
+        "sector_solves_per_s": round(sps_b, 2), "wall_s": round(t_batch, 3),
+        "add_s": round(t_add, 3), "matvecs": mv,
+        "plain_sector_solves_per_s": round(sps_p, 2), "plain_sample": nplain,
+        "plain_sample_s": round(t_plain, 3), "plain_matvecs": plain_mv,
+        "speedup_vs_plain": round(sps_b / sps_p, 2),
+        "sample_bitwise_equal": bool(same),
+        "workspace_bytes": int(peak - used0), "setup_s": round(t_enum + t_ops, 3),
+    }
+
+
+def served_section(args):
+    """The served stream through the continuous-batching server and through the plain
+    one-at-a-time server, same requests, same HBM budget and queue bound."""
+    from momentumed import serve as S
+    reqs = S.served_stream(rate=50.0, duration=args.serve_seconds, seed=7)
+    out = {"stream": f"Poisson 50 req/s for {args.serve_seconds:g} s (seed 7), meshes "
+                     f"{['%dx%d' % m for m in S.MESHES]}, fillings 1/3, 2/5 (non-integer N "
+                     f"redrawn), one random sector, screening ~U(1,4), nev 4",
+           "hbm_budget_bytes": 200 * 10 ** 9, "max_queue": 256, "drain_s": args.serve_drain}
+    for mode in ("batched", "plain"):
+        srv = S.SolveServer(hbm_budget_bytes=200 * 10 ** 9, max_queue=256, mode=mode)
+        try:
+            rep = srv.run(reqs, drain_s=args.serve_drain)
+        finally:
+            srv.close()
+        summ = rep.summary()
+        log(f"[served] {mode}: {summ['completed']}/{summ['offered']} done, "
+            f"{summ['rejected']} rejected, {summ['unfinished']} unfinished, "
+            f"{summ['solves_per_s']}/s, p50 {summ['p50_s']}, p99 {summ['p99_s']}")
+        out[mode] = summ
+    b, p = out["batched"]["solves_per_s"], out["plain"]["solves_per_s"]
+    out["speedup_vs_plain"] = round(b / p, 2) if b and p else None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,14 +186,19 @@ def main():
     ap.add_argument("--solve", action="store_true", help="also time the lowest-nev eigensolve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--batch-ratios", type=int, default=64)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-drain", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, run 3 plain device copies of one vector "
                          "(16*D bytes read + written each) to calibrate rocprof byte counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one momentum "
-              "sector per GPU; see DESIGN.md for how sectors shard). Run with --gpus 1.",
+        print("bench.py measures one GPU: this build has no multi-GPU path (sectors and "
+              "parameter points would shard one problem set per GPU). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -212,6 +322,10 @@ def main():
         log(f"[bench] cpu baseline {cb['value']:.3e} matvec/s; GPU/CPU rows max rel err {err:.2e}")
     else:
         out["cpu_baseline"] = None
+    if not args.no_batched:
+        out["batched"] = batched_section(args)
+    if not args.no_served:
+        out["served"] = served_section(args)
     print(json.dumps(out), flush=True)
 
 

@@ -38,6 +38,7 @@ extern "C" {
 typedef struct edm_ctx edm_ctx;
 typedef struct edm_basis edm_basis;
 typedef struct edm_op edm_op;
+typedef struct edm_batch edm_batch;
 
 /* Description of a momentum-conserved system and particle numbers
  * (EDPara, MomentumED/src/preparation/init_parameter.jl:62-146, and the N_each_component
@@ -162,6 +163,49 @@ int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, 
 int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
                  const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+
+/* ----------------------------------------------------------------- batched / served solves */
+/* The reference has no batch object: a sweep over sectors and parameter points is a host loop
+ * of EDsolve calls (MomentumED.jl:375-480; e.g. the per-sector loop in
+ * examples/landau level torus/topological order/4. cuda support.ipynb, cell source lines
+ * 299-301). A batch advances many independent krylov_map_solve
+ * problems together: each problem is exactly the solve edm_eigsolve runs (same arguments,
+ * same iteration, bitwise the same eigenvalues and vectors), but one step moves every running
+ * problem by one Lanczos iteration with one launch per kernel for all of them. Problems can be
+ * added and collected between steps. The operator and basis of a problem must stay alive
+ * until the problem is collected or removed. A batch lives on its context's stream. */
+int edm_batch_create(edm_ctx *ctx, edm_batch **out);
+int edm_batch_destroy(edm_batch *bt);
+/* Admit a problem (arguments as edm_eigsolve; want_vectors keeps the nev eigenvectors on the
+ * device for edm_batch_result). Allocates the problem's workspace (edm_batch_problem_bytes)
+ * from the context: EDM_ENOMEM when the context's HBM budget does not hold it. *id names it. */
+int edm_batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int32_t nev,
+                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
+                  const double *vec0, int32_t want_vectors, int64_t *id);
+/* Run up to max_steps Lanczos steps of all running problems, returning early after a step in
+ * which any problem finished. *n_running: problems still running; *n_finished: problems that
+ * finished during this call (either pointer may be NULL). */
+int edm_batch_step(edm_batch *bt, int32_t max_steps, int32_t *n_running, int32_t *n_finished);
+/* Ids of finished, not yet collected problems (up to cap written; *n = total count). */
+int edm_batch_finished(edm_batch *bt, int64_t *ids, int32_t cap, int32_t *n);
+/* *state: 0 running, 1 converged, 2 finished without all nev pairs converged; info may be NULL. */
+int edm_batch_status(edm_batch *bt, int64_t id, int32_t *state, edm_solve_info *info);
+/* Collect a finished problem: vals[nev], vecs (host, nev*D complex, vector-major; NULL or the
+ * problem must have been added with want_vectors), info; releases its workspace. Returns
+ * EDM_OK or EDM_ENOCONV as edm_eigsolve would have. */
+int edm_batch_result(edm_batch *bt, int64_t id, double *vals, double *vecs, edm_solve_info *info);
+/* Drop a problem, running or finished, and release its workspace. */
+int edm_batch_remove(edm_batch *bt, int64_t id);
+/* Problems held (running + finished, not collected) and running. */
+int edm_batch_count(edm_batch *bt, int32_t *n_total, int32_t *n_running);
+/* Step scheduling. rows = 0 (default): every running problem takes part in every step.
+ * rows > 0: deficit round robin over rows; each step credits every running problem with rows,
+ * and a problem of dimension D takes part when its credit reaches D (then pays D), so small
+ * problems step often while large ones step about once per D / rows steps. Only the pace
+ * changes; each problem's iteration and result do not. */
+int edm_batch_set_quantum(edm_batch *bt, int64_t rows);
+/* Device bytes edm_batch_add allocates for a problem of dimension D. */
+int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t *bytes);
 
 /* ----------------------------------------------------------------- device vector helpers */
 /* Fill D complex values at a device pointer: mode 0 = uniform [0,1) in re and im

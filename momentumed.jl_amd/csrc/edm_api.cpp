@@ -56,6 +56,14 @@ void attach_rank(edm_basis *b)
 
 }  // namespace
 
+#define EDM_NULL_BATCH(bt)                                   \
+    do {                                                     \
+        if (!(bt)) {                                         \
+            edm_set_global_error("batch handle is NULL");    \
+            return EDM_EINVAL;                               \
+        }                                                    \
+    } while (0)
+
 extern "C" {
 
 int edm_device_available(void)
@@ -470,6 +478,90 @@ int edm_fill_random(edm_ctx *ctx, void *v_dev, int64_t D, uint64_t seed, int32_t
     return guarded(ctx, [&]() -> int {
         EDM_REQUIRE(ctx && (v_dev || D == 0), EDM_EINVAL, "NULL argument");
         blas_fill_random(ctx, (double2 *)v_dev, D, seed, mode);
+        return EDM_OK;
+    });
+}
+
+int edm_batch_create(edm_ctx *ctx, edm_batch **out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && out, EDM_EINVAL, "NULL argument");
+        *out = batch_create(ctx);
+        return EDM_OK;
+    });
+}
+
+int edm_batch_destroy(edm_batch *bt)
+{
+    if (!bt) return EDM_OK;
+    return guarded(batch_ctx(bt), [&]() -> int {
+        batch_destroy(bt);
+        return EDM_OK;
+    });
+}
+
+int edm_batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int32_t nev,
+                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
+                  const double *vec0, int32_t want_vectors, int64_t *id)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int {
+        return batch_add(bt, op, b, nev, krylovdim, tol, maxiter, seed, vec0, want_vectors, id);
+    });
+}
+
+int edm_batch_step(edm_batch *bt, int32_t max_steps, int32_t *n_running, int32_t *n_finished)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int {
+        return batch_step(bt, max_steps, n_running, n_finished);
+    });
+}
+
+int edm_batch_finished(edm_batch *bt, int64_t *ids, int32_t cap, int32_t *n)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int {
+        EDM_REQUIRE(n && cap >= 0, EDM_EINVAL, "bad argument");
+        return batch_finished(bt, ids, cap, n);
+    });
+}
+
+int edm_batch_status(edm_batch *bt, int64_t id, int32_t *state, edm_solve_info *info)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int { return batch_status(bt, id, state, info); });
+}
+
+int edm_batch_result(edm_batch *bt, int64_t id, double *vals, double *vecs, edm_solve_info *info)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int { return batch_result(bt, id, vals, vecs, info); });
+}
+
+int edm_batch_remove(edm_batch *bt, int64_t id)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int { return batch_remove(bt, id); });
+}
+
+int edm_batch_count(edm_batch *bt, int32_t *n_total, int32_t *n_running)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int { return batch_count(bt, n_total, n_running); });
+}
+
+int edm_batch_set_quantum(edm_batch *bt, int64_t rows)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int { return batch_set_quantum(bt, rows); });
+}
+
+int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t *bytes)
+{
+    return guarded(nullptr, [&]() -> int {
+        EDM_REQUIRE(bytes && D > 0 && nev >= 1, EDM_EINVAL, "bad argument");
+        *bytes = batch_problem_bytes(D, nev, krylovdim);
         return EDM_OK;
     });
 }

@@ -189,6 +189,145 @@ k_fill_random(double2 *__restrict__ v, int64_t D, uint64_t seed, int mode)
     }
 }
 
+// ---------------------------------------------------------------- batched forms
+// The same operations for many problems in one launch. A tile (kTile rows) or a row block
+// (kBlk rows) belongs to exactly one problem, found through a table; per-problem partial sums
+// live in the problem's own region and are reduced in tile order, so every problem's result
+// is bitwise what the single-problem kernels above produce, whatever else shares the launch.
+__global__ void __launch_bounds__(kBlk)
+k_bmultidot_partial(const BatchVecDesc *__restrict__ descs, const int *__restrict__ tile2prob)
+{
+    __shared__ double2 sh[kBlk / 64];
+    const BatchVecDesc &P = descs[tile2prob[blockIdx.x]];
+    int64_t lt = (int64_t)blockIdx.x - P.tile0;
+    int64_t base = lt * kTile + threadIdx.x;
+    double2 wv[kRpt];
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
+        int64_t r = base + (int64_t)k * kBlk;
+        wv[k] = r < P.D ? P.w[r] : make_double2(0.0, 0.0);
+    }
+    for (int i = 0; i < P.nvec; ++i) {
+        const double2 *v = P.V[i];
+        double ar = 0.0, ai = 0.0;
+#pragma unroll
+        for (int k = 0; k < kRpt; ++k) {
+            int64_t r = base + (int64_t)k * kBlk;
+            if (r < P.D) {
+                double2 a = v[r];
+                ar += a.x * wv[k].x + a.y * wv[k].y;
+                ai += a.x * wv[k].y - a.y * wv[k].x;
+            }
+        }
+        double2 s = block_sum2(ar, ai, sh);
+        if (threadIdx.x == 0) P.partial[(int64_t)i * P.ntiles + lt] = s;
+    }
+}
+
+// out[i] = sum of partials, block (problem, i); which = 0 -> h, 1 -> h2
+__global__ void __launch_bounds__(kBlk)
+k_breduce(const BatchVecDesc *__restrict__ descs, int which)
+{
+    __shared__ double2 sh[kBlk / 64];
+    const BatchVecDesc &P = descs[blockIdx.x];
+    int i = blockIdx.y;
+    if (i >= P.nvec) return;
+    const double2 *p = P.partial + (int64_t)i * P.ntiles;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t b = threadIdx.x; b < P.ntiles; b += kBlk) { ar += p[b].x; ai += p[b].y; }
+    double2 s = block_sum2(ar, ai, sh);
+    if (threadIdx.x == 0) (which ? P.h2 : P.h)[i] = s;
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_baxpy(const BatchVecDesc *__restrict__ descs, const int *__restrict__ blk2prob, int which)
+{
+    __shared__ double2 hs[kMaxVec];
+    const BatchVecDesc &P = descs[blk2prob[blockIdx.x]];
+    const double2 *c = which ? P.h2 : P.h;
+    for (int i = threadIdx.x; i < P.nvec; i += kBlk) hs[i] = c[i];
+    __syncthreads();
+    int64_t r = ((int64_t)blockIdx.x - P.blk0) * kBlk + threadIdx.x;
+    if (r >= P.D) return;
+    double2 acc = P.w[r];
+    for (int i = 0; i < P.nvec; ++i) {
+        double2 a = P.V[i][r];
+        double2 h = hs[i];
+        acc.x -= a.x * h.x - a.y * h.y;
+        acc.y -= a.x * h.y + a.y * h.x;
+    }
+    P.w[r] = acc;
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_bnorm_partial(const BatchVecDesc *__restrict__ descs, const int *__restrict__ tile2prob)
+{
+    __shared__ double2 sh[kBlk / 64];
+    const BatchVecDesc &P = descs[tile2prob[blockIdx.x]];
+    int64_t lt = (int64_t)blockIdx.x - P.tile0;
+    int64_t base = lt * kTile + threadIdx.x;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
+        int64_t r = base + (int64_t)k * kBlk;
+        if (r < P.D) { double2 a = P.w[r]; s += a.x * a.x + a.y * a.y; }
+    }
+    double2 t = block_sum2(s, 0.0, sh);
+    if (threadIdx.x == 0) P.partial[lt] = t;
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_bnorm_reduce(const BatchVecDesc *__restrict__ descs)
+{
+    __shared__ double2 sh[kBlk / 64];
+    const BatchVecDesc &P = descs[blockIdx.x];
+    double ar = 0.0, ai = 0.0;
+    for (int64_t b = threadIdx.x; b < P.ntiles; b += kBlk) { ar += P.partial[b].x; ai += P.partial[b].y; }
+    double2 s = block_sum2(ar, ai, sh);
+    if (threadIdx.x == 0) P.beta[0] = sqrt(s.x);
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_bscale(const BatchVecDesc *__restrict__ descs, const int *__restrict__ blk2prob)
+{
+    const BatchVecDesc &P = descs[blk2prob[blockIdx.x]];
+    int64_t r = ((int64_t)blockIdx.x - P.blk0) * kBlk + threadIdx.x;
+    if (r >= P.D) return;
+    double inv = 1.0 / P.beta[0];
+    double2 a = P.w[r];
+    P.w[r] = make_double2(a.x * inv, a.y * inv);
+}
+
+__global__ void __launch_bounds__(kBlk)
+k_brotate(const BatchRotDesc *__restrict__ jobs, const int *__restrict__ blk2job)
+{
+    extern __shared__ double ys[];   // m * kRotChunk
+    const BatchRotDesc &P = jobs[blk2job[blockIdx.x]];
+    for (int t = threadIdx.x; t < P.m * kRotChunk; t += kBlk) {
+        int i = t / kRotChunk, c = t % kRotChunk;
+        ys[t] = c < P.kc ? P.Y[i + (int64_t)P.ldy * (P.c0 + c)] : 0.0;
+    }
+    __syncthreads();
+    int64_t r = ((int64_t)blockIdx.x - P.blk0) * kBlk + threadIdx.x;
+    if (r >= P.D) return;
+    double ar[kRotChunk], ai[kRotChunk];
+#pragma unroll
+    for (int c = 0; c < kRotChunk; ++c) { ar[c] = 0.0; ai[c] = 0.0; }
+    for (int i = 0; i < P.m; ++i) {
+        double2 v = P.V[i][r];
+#pragma unroll
+        for (int c = 0; c < kRotChunk; ++c) {
+            double y = ys[i * kRotChunk + c];
+            ar[c] += v.x * y;
+            ai[c] += v.y * y;
+        }
+    }
+    for (int c = 0; c < P.kc; ++c) P.out[P.c0 + c][r] = make_double2(ar[c], ai[c]);
+}
+
+static_assert(kBlk == kBatchRowBlock && kTile == kBatchTile && kRotChunk == kBatchRotChunk,
+              "batched launch geometry shared with the host engine");
+
 inline unsigned rows_blocks(int64_t D) { return (unsigned)((D + kBlk - 1) / kBlk); }
 inline unsigned tile_blocks(int64_t D) { return (unsigned)((D + kTile - 1) / kTile); }
 
@@ -242,6 +381,45 @@ void blas_scale_inv(edm_ctx *ctx, double2 *v, const double2 *w, const double *d_
     if (D == 0) return;
     hipLaunchKernelGGL(k_scale_inv, dim3(rows_blocks(D)), dim3(kBlk), 0, ctx->stream, v, w,
                        d_beta, D);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void bblas_cgs2_norm(edm_ctx *ctx, const BatchVecDesc *d_descs, int nprob, int maxvec,
+                     const int *d_tile2prob, int64_t ntiles, const int *d_blk2prob, int64_t nblk)
+{
+    EDM_REQUIRE(maxvec <= kMaxVec, EDM_EINVAL, "too many vectors in one batched multi-dot");
+    if (nprob == 0) return;
+    hipStream_t st = ctx->stream;
+    for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_bmultidot_partial, dim3((unsigned)ntiles), dim3(kBlk), 0, st, d_descs,
+                           d_tile2prob);
+        hipLaunchKernelGGL(k_breduce, dim3((unsigned)nprob, (unsigned)maxvec), dim3(kBlk), 0, st,
+                           d_descs, pass);
+        hipLaunchKernelGGL(k_baxpy, dim3((unsigned)nblk), dim3(kBlk), 0, st, d_descs, d_blk2prob,
+                           pass);
+    }
+    hipLaunchKernelGGL(k_bnorm_partial, dim3((unsigned)ntiles), dim3(kBlk), 0, st, d_descs,
+                       d_tile2prob);
+    hipLaunchKernelGGL(k_bnorm_reduce, dim3((unsigned)nprob), dim3(kBlk), 0, st, d_descs);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void bblas_scale(edm_ctx *ctx, const BatchVecDesc *d_descs, const int *d_blk2prob, int64_t nblk)
+{
+    if (nblk == 0) return;
+    hipLaunchKernelGGL(k_bscale, dim3((unsigned)nblk), dim3(kBlk), 0, ctx->stream, d_descs,
+                       d_blk2prob);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void bblas_rotate(edm_ctx *ctx, const BatchRotDesc *d_jobs, const int *d_blk2job, int64_t nblk,
+                  int maxm)
+{
+    if (nblk == 0) return;
+    size_t sh = (size_t)maxm * kRotChunk * sizeof(double);
+    EDM_REQUIRE(sh <= 64 * 1024, EDM_EINVAL, "Krylov dimension too large for the rotation kernel");
+    hipLaunchKernelGGL(k_brotate, dim3((unsigned)nblk), dim3(kBlk), sh, ctx->stream, d_jobs,
+                       d_blk2job);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

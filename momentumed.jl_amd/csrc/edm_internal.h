@@ -154,6 +154,36 @@ struct edm_op {
     uint64_t *d_out = nullptr, *d_in = nullptr, *d_par = nullptr;
 };
 
+// ------------------------------------------------------------------ batched execution
+// One problem's view for the batched vector kernels (edm_blas.hip).
+struct BatchVecDesc {
+    int64_t D;
+    const double2 *const *V;   // device pointer table of the problem's basis vectors
+    int nvec;                  // orthogonalise against V[0..nvec)
+    double2 *w;                // the vector being orthogonalised / normalised
+    double2 *partial;          // the problem's partial-sum region, [nvec][ntiles]
+    int64_t ntiles;            // reduction tiles (1024 rows each)
+    int64_t tile0;             // first global tile of the problem
+    int64_t blk0;              // first global 256-row block of the problem
+    double2 *h, *h2;           // projection coefficients of the two Gram-Schmidt passes
+    double *beta;              // norm of w
+};
+
+// One (problem, column chunk) job of the batched basis rotation (edm_blas.hip):
+// out[c0 + c] = sum_i V[i] * Y[i + ldy * (c0 + c)], c < kc.
+struct BatchRotDesc {
+    int64_t D;
+    const double2 *const *V;
+    double2 *const *out;
+    const double *Y;
+    int m, ldy, c0, kc;
+    int64_t blk0;              // first global 256-row block of the job
+};
+
+constexpr int kBatchRowBlock = 256;    // rows per block of the batched vector kernels
+constexpr int kBatchTile = 1024;       // rows per reduction tile of the batched vector kernels
+constexpr int kBatchRotChunk = 8;      // output columns per rotation job
+
 // ------------------------------------------------------------------ kernels (launchers)
 // enumeration (edm_enum.hip)
 void enum_sector_counts(edm_ctx *ctx, const SystemSpec &sp, std::vector<int64_t> &counts);
@@ -179,6 +209,32 @@ int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, in
                  double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
                  double *vecs, edm_solve_info *info);
 const char *edm_global_error();
+// cyclic Jacobi eigensolver for a small real symmetric matrix (edm_lanczos.cpp)
+void jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z);
+// batched matvec over many problems (edm_matvec.hip). batch_mv_desc writes the problem's
+// descriptor (batch_mv_desc_size() bytes) for y = H x and returns false when the operator has
+// no LDS-staged table (the caller then runs launch_matvec for it); blocks are 512 rows.
+size_t batch_mv_desc_size();
+int64_t batch_mv_blocks(int64_t D);
+bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, double2 *y,
+                   int64_t blk0, void *out, size_t *lds_bytes);
+void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, int64_t nblocks,
+                     size_t lds_bytes);
+// multi-problem Lanczos engine (edm_batch.cpp)
+edm_batch *batch_create(edm_ctx *ctx);
+void batch_destroy(edm_batch *bt);
+edm_ctx *batch_ctx(edm_batch *bt);
+int batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int nev, int krylovdim,
+              double tol, int maxiter, uint64_t seed, const double *vec0, int want_vectors,
+              int64_t *id);
+int batch_step(edm_batch *bt, int max_steps, int *n_running, int *n_finished);
+int batch_finished(edm_batch *bt, int64_t *ids, int cap, int *n);
+int batch_status(edm_batch *bt, int64_t id, int *state, edm_solve_info *info);
+int batch_result(edm_batch *bt, int64_t id, double *vals, double *vecs, edm_solve_info *info);
+int batch_remove(edm_batch *bt, int64_t id);
+int batch_count(edm_batch *bt, int *n_total, int *n_running);
+uint64_t batch_problem_bytes(int64_t D, int nev, int krylovdim);
+int batch_set_quantum(edm_batch *bt, int64_t rows);
 // vector kernels (edm_blas.hip)
 void blas_fill_random(edm_ctx *ctx, double2 *v, int64_t D, uint64_t seed, int mode);
 // h[i] = <V_i, w> for i < nvec (V given by a device pointer table); result on device
@@ -191,6 +247,15 @@ void blas_multiaxpy(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const
 void blas_norm(edm_ctx *ctx, const double2 *w, int64_t D, double *d_out);
 // v = w / (*d_beta)
 void blas_scale_inv(edm_ctx *ctx, double2 *v, const double2 *w, const double *d_beta, int64_t D);
+// batched CGS2 + norm for every problem in d_descs (nprob problems, at most maxvec vectors):
+// h = V^H w, w -= V h, h2 = V^H w, w -= V h2, beta = ||w||
+void bblas_cgs2_norm(edm_ctx *ctx, const BatchVecDesc *d_descs, int nprob, int maxvec,
+                     const int *d_tile2prob, int64_t ntiles, const int *d_blk2prob, int64_t nblk);
+// w /= beta for every problem
+void bblas_scale(edm_ctx *ctx, const BatchVecDesc *d_descs, const int *d_blk2prob, int64_t nblk);
+// batched rotation jobs (blk2job maps each 256-row block to its job)
+void bblas_rotate(edm_ctx *ctx, const BatchRotDesc *d_jobs, const int *d_blk2job, int64_t nblk,
+                  int maxm);
 // out_c = sum_i V_i Y[i + ldy*c] for c < k (Y real, on device)
 void blas_rotate(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const double *d_Y, int ldy,
                  int k, double2 *const *d_outptrs, int64_t D);

@@ -21,8 +21,6 @@
 #include <cmath>
 #include <numeric>
 
-namespace {
-
 // Cyclic Jacobi for a real symmetric n x n matrix A (row-major, destroyed). On return
 // w[i] ascending and Z[:, i] (column-major, ldz = n) the matching orthonormal eigenvectors.
 void jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z)
@@ -74,6 +72,8 @@ void jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vec
     }
     Z.swap(Zs);
 }
+
+namespace {
 
 struct Workspace {
     edm_ctx *ctx;

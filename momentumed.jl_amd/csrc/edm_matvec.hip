@@ -380,27 +380,17 @@ __device__ __forceinline__ int64_t rank_finish(const RankSearch &, int64_t j) { 
 // lookups in flight, then U gathers in flight) and an optional partner window: only partners
 // j in [jmin, jmax) are gathered, and the result is added to y when `accumulate` is set. A
 // matvec split into passes over windows of x keeps each pass's gather footprint small.
+// One row of the LDS-staged matvec: y_i (with the diagonal when with_diag) over partners in
+// [jmin, jmax). The term image must already be in LDS at `base`.
 template <class Rank, bool HighFirst, int U>
-__global__ void __launch_bounds__(kLdsBlock)
-k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
-              const double2 *__restrict__ x, double2 *__restrict__ y, int64_t jmin, int64_t jmax,
-              int accumulate, int with_diag)
+__device__ __forceinline__ double2 ldsu_row(int64_t i, uint64_t s, const unsigned char *base,
+                                            const CompactDev &t, const Rank &rank,
+                                            const double2 *__restrict__ x, int64_t jmin,
+                                            int64_t jmax, int with_diag)
 {
-    extern __shared__ uint4 lds_u4[];
-    {
-        const uint4 *g = (const uint4 *)t.keyptr;
-        int n16 = t.lds_bytes / 16;
-        for (int k = threadIdx.x; k < n16; k += blockDim.x) lds_u4[k] = g[k];
-    }
-    __syncthreads();
-    const unsigned char *base = (const unsigned char *)lds_u4;
     const uint16_t *s_keyptr = (const uint16_t *)base;
     const double2 *s_diag = (const double2 *)(base + t.off_diag);
     const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
-
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= D) return;
-    const uint64_t s = states[i];
     double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
     const auto rrow = rank_row(rank, s);
     using Pend = decltype(rank_issue(rrow, 0ULL));
@@ -474,12 +464,72 @@ k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
         yr += dr * xi.x - di * xi.y;
         yi += dr * xi.y + di * xi.x;
     }
+    return make_double2(yr, yi);
+}
+
+__device__ __forceinline__ void stage_image(const CompactDev &t, uint4 *lds)
+{
+    const uint4 *g = (const uint4 *)t.keyptr;
+    int n16 = t.lds_bytes / 16;
+    for (int k = threadIdx.x; k < n16; k += blockDim.x) lds[k] = g[k];
+}
+
+// Same traversal as k_matvec_lds, with U candidates of a bucket resolved together (U table
+// lookups in flight, then U gathers in flight), a per-row rank view, and an optional partner
+// window: only partners j in [jmin, jmax) are gathered, and the result is added to y when
+// `accumulate` is set. A matvec split into passes over windows of x keeps each pass's gather
+// footprint small.
+template <class Rank, bool HighFirst, int U>
+__global__ void __launch_bounds__(kLdsBlock)
+k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
+              const double2 *__restrict__ x, double2 *__restrict__ y, int64_t jmin, int64_t jmax,
+              int accumulate, int with_diag)
+{
+    extern __shared__ uint4 lds_u4[];
+    stage_image(t, lds_u4);
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    double2 v = ldsu_row<Rank, HighFirst, U>(i, states[i], (const unsigned char *)lds_u4, t, rank,
+                                             x, jmin, jmax, with_diag);
     if (accumulate) {
         double2 o = y[i];
-        yr += o.x;
-        yi += o.y;
+        v.x += o.x;
+        v.y += o.y;
     }
-    y[i] = make_double2(yr, yi);
+    y[i] = v;
+}
+
+// ---------------------------------------------------------------- batched matvec
+// Many independent problems (own basis, own term image, own vectors) in one launch: the
+// workgroup -> problem table routes each 512-row block to its problem, which stages its own
+// term image. Each problem's rows are computed exactly as by k_matvec_ldsu<., true, 1>.
+struct BatchMvDesc {
+    int64_t D;
+    const uint64_t *states;
+    const double2 *x;
+    double2 *y;
+    CompactDev t;
+    RankLin lin;
+    RankSearch search;
+    int use_lin;
+    int64_t blk0;
+};
+
+__global__ void __launch_bounds__(kLdsBlock)
+k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ blk2prob)
+{
+    extern __shared__ uint4 lds_u4[];
+    const BatchMvDesc &P = descs[blk2prob[blockIdx.x]];
+    stage_image(P.t, lds_u4);
+    __syncthreads();
+    int64_t i = ((int64_t)blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
+    if (i >= P.D) return;
+    const unsigned char *base = (const unsigned char *)lds_u4;
+    double2 v = P.use_lin
+        ? ldsu_row<RankLin, true, 1>(i, P.states[i], base, P.t, P.lin, P.x, 0, P.D, 1)
+        : ldsu_row<RankSearch, true, 1>(i, P.states[i], base, P.t, P.search, P.x, 0, P.D, 1);
+    P.y[i] = v;
 }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kRowBlock - 1) / kRowBlock); }
@@ -669,6 +719,41 @@ void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const dou
                                0, ctx->stream, b->D, b->d_states, op->S_full, op->d_amp,
                                op->d_out, op->d_in, op->d_par, make_search(b), x, y, adj);
     }
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+size_t batch_mv_desc_size() { return sizeof(BatchMvDesc); }
+
+int64_t batch_mv_blocks(int64_t D) { return (D + kLdsBlock - 1) / kLdsBlock; }
+
+bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, double2 *y,
+                   int64_t blk0, void *out, size_t *lds_bytes)
+{
+    EDM_REQUIRE(op->nbits == b->nbits, EDM_EINVAL, "operator and basis have different bits");
+    const CompactTable &ct = op->cfwd;
+    if (!op->fast || !ct.ok || matvec_variant() != 11 || matvec_passes() != 1) return false;
+    BatchMvDesc d{};
+    d.D = b->D;
+    d.states = b->d_states;
+    d.x = x;
+    d.y = y;
+    d.t = CompactDev{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
+                     (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    d.use_lin = use_lin(op, b) ? 1 : 0;
+    if (d.use_lin) d.lin = make_lin(b);
+    d.search = make_search(b);
+    d.blk0 = blk0;
+    memcpy(out, &d, sizeof(d));
+    *lds_bytes = ct.lds_bytes;
+    return true;
+}
+
+void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, int64_t nblocks,
+                     size_t lds_bytes)
+{
+    if (nblocks == 0) return;
+    hipLaunchKernelGGL(k_batch_matvec, dim3((unsigned)nblocks), dim3(kLdsBlock), lds_bytes,
+                       ctx->stream, (const BatchMvDesc *)d_descs, d_blk2prob);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

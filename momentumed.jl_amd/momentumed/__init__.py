@@ -16,6 +16,7 @@ from .preparation import (EDPara, ED_momentum_subspaces, ED_scatterlist_onebody,
                           ED_scatterlist_twobody, ED_sortedScatterList_onebody,
                           ED_sortedScatterList_twobody, MBS_totalmomentum, momentum_residue)
 from .solve import EDsolve
+from .batch import BatchSolver, EDsolve_batch
 
 __all__ = [
     "EDPara", "ED_momentum_subspaces", "ED_scatterlist_onebody", "ED_scatterlist_twobody",
@@ -23,5 +24,6 @@ __all__ = [
     "momentum_residue", "HilbertSubspace", "MBOperator", "MBS64", "MBS64Vector", "Scatter",
     "get_body", "index_fit", "isdiagonal", "isupper", "make_mask64", "occ_list", "parity_mask",
     "sort_merge_scatlist", "EDsolve", "GPU_AVAILABLE", "DEFAULT_GPU", "activate_gpu_device",
-    "create_gpu_linearmap", "krylov_map_solve", "release_gpu_memory",
+    "create_gpu_linearmap", "krylov_map_solve", "release_gpu_memory", "BatchSolver",
+    "EDsolve_batch",
 ]

@@ -61,6 +61,18 @@ _SIGS = [
     ("edm_connectivity_totals", c_i32, [c_vp, c_vp, c_vp, P(c_i64), P(c_i64)]),
     ("edm_eigsolve", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_vp,
                              c_vp, P(EdmSolveInfo)]),
+    ("edm_batch_create", c_i32, [c_vp, P(c_vp)]),
+    ("edm_batch_destroy", c_i32, [c_vp]),
+    ("edm_batch_add", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_i32,
+                              P(c_i64)]),
+    ("edm_batch_step", c_i32, [c_vp, c_i32, P(c_i32), P(c_i32)]),
+    ("edm_batch_finished", c_i32, [c_vp, P(c_i64), c_i32, P(c_i32)]),
+    ("edm_batch_status", c_i32, [c_vp, c_i64, P(c_i32), P(EdmSolveInfo)]),
+    ("edm_batch_result", c_i32, [c_vp, c_i64, c_vp, c_vp, P(EdmSolveInfo)]),
+    ("edm_batch_remove", c_i32, [c_vp, c_i64]),
+    ("edm_batch_count", c_i32, [c_vp, P(c_i32), P(c_i32)]),
+    ("edm_batch_set_quantum", c_i32, [c_vp, c_i64]),
+    ("edm_batch_problem_bytes", c_i32, [c_i64, c_i32, c_i32, P(c_u64)]),
     ("edm_fill_random", c_i32, [c_vp, c_vp, c_i64, c_u64, c_i32]),
 ]
 

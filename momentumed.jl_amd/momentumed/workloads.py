@@ -1,5 +1,6 @@
 """Synthetic benchmark systems: spinless (or two-component) fermions on a k-mesh with a
-Coulomb-like interaction V(q) = 2*pi/|q| * exp(-c |q|^2) * (1 + a cos 6 theta_q).
+Coulomb-like interaction V(q) = 2*pi/|q| * exp(-c |q|^2) * (1 + a cos 6 theta_q), or its
+screened form with 1/|q| -> 1/sqrt(|q|^2 + lambda^-2).
 
 Geometry: a square torus of area 2*pi*Nk, so the momentum step is 2*pi/L with
 L = sqrt(2*pi*Nk); the transfer q = k_f1 - k_i1 is taken at its minimum image on the
@@ -13,6 +14,13 @@ Workloads (sizes computed by enumeration):
   medium  3x9 mesh, N=9, flat band, V with c=1/4; sector (0,0) has 173,613 states.
   large   6x6 mesh, N=12, V with c=1/2, a=0.1, then a phase e^{i phi}, phi ~ U(-pi,pi)*1e-3
           (seed 4321), on every non-diagonal upper term; sector (0,0) has 34,770,492 states.
+  batched two conserved components on a 3x4 mesh each (24 orbitals), V with c=1/4 within a
+          component and r*V between components; fillings (N1,N2) in (2,3), (2,4), (3,3),
+          (3,4), (4,4), all 12 momentum sectors of each (1,210 ... 20,419 states), and 64
+          ratios r ~ U(0.5, 2.0) (numpy default_rng seed 99): 3,840 solves, lowest 4 each.
+          The 4x6-mesh, N1=N2=4 system has 4.7 million states per sector, not the
+          1.5k-20k range this sweep is meant to cover, so the mesh and fillings are chosen to
+          give that range.
 """
 from __future__ import annotations
 
@@ -24,7 +32,8 @@ import numpy as np
 from .core import MBOperator, Scatter
 from .preparation import EDPara, ED_scatterlist_onebody, ED_scatterlist_twobody
 
-__all__ = ["mesh", "CoulombGaussian", "Workload", "small", "medium", "large", "get"]
+__all__ = ["mesh", "CoulombGaussian", "Workload", "small", "medium", "large", "get",
+           "BATCH_FILLINGS", "batched_para", "batched_ratios"]
 
 _M64 = (1 << 64) - 1
 
@@ -58,7 +67,8 @@ class CoulombGaussian:
     """Index-signature H_two(kf1, kf2, ki2, ki1, cf1, cf2, ci2, ci1) (1-based indices)."""
 
     def __init__(self, k_list: np.ndarray, dims: tuple, c: float = 0.25, aniso: float = 0.0,
-                 perturb: float = 0.0, seed: int = 0, inter_ratio: float = 1.0):
+                 perturb: float = 0.0, seed: int = 0, inter_ratio: float = 1.0,
+                 screening: float | None = None):
         self.k = np.asarray(k_list, dtype=np.int64)
         self.dims = dims
         self.Nk = self.k.shape[1]
@@ -68,6 +78,7 @@ class CoulombGaussian:
         self.perturb = perturb
         self.seed = seed
         self.inter_ratio = inter_ratio
+        self.screening = screening     # length lambda: 1/|q| -> 1/sqrt(|q|^2 + lambda^-2)
         self._cache: dict = {}
 
     def vq(self, kf: int, ki: int) -> float:
@@ -84,7 +95,8 @@ class CoulombGaussian:
             if qq == 0.0:
                 v = 0.0
             else:
-                v = 2 * math.pi / qq * math.exp(-self.c * qq * qq)
+                den = qq if self.screening is None else math.sqrt(qq * qq + self.screening ** -2)
+                v = 2 * math.pi / den * math.exp(-self.c * qq * qq)
                 if self.aniso:
                     v *= 1.0 + self.aniso * math.cos(6.0 * math.atan2(q[1], q[0]))
             self._cache[key] = v
@@ -172,6 +184,23 @@ def large() -> Workload:
                   two_momentum_coordinate=False)
     return Workload("large_6x6_N12", para, (12,), (0, 0), nev=6, krylovdim=40,
                     phase_disorder=1e-3, phase_seed=4321)
+
+
+BATCH_FILLINGS = ((2, 3), (2, 4), (3, 3), (3, 4), (4, 4))
+BATCH_MESH = (3, 4)
+BATCH_NEV = 4
+
+
+def batched_para(ratio: float) -> EDPara:
+    """The two-component 3x4 system of the batched sweep at inter-component ratio r."""
+    k = mesh(*BATCH_MESH)
+    return EDPara(k_list=k, Gk=BATCH_MESH, Nc_conserve=2,
+                  H_two=CoulombGaussian(k, BATCH_MESH, c=0.25, inter_ratio=float(ratio)),
+                  two_momentum_coordinate=False)
+
+
+def batched_ratios(n: int = 64, seed: int = 99) -> np.ndarray:
+    return np.random.default_rng(seed).uniform(0.5, 2.0, n)
 
 
 def get(name: str) -> Workload:

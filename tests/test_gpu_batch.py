@@ -1,0 +1,217 @@
+"""Batched and continuously batched solves (libedm's batch engine) through the C-ABI.
+
+Bar: every problem's eigenvalues and eigenvectors are bitwise those of the same solve run
+alone by edm_eigsolve (the batch engine runs the identical iteration with kernels of the same
+per-problem arithmetic and reduction order), whatever shares the batch and whenever the
+problem was admitted; against dense LAPACK on the oracle's matrix within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import momentumed as me
+from momentumed import batch as B
+from momentumed import device as dev
+from momentumed import workloads as W
+from momentumed._lib import EDM_ENOMEM, EDMError
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_full(O, op):
+    amp, out, inn = op.arrays()
+    return O.full_list(O.TermList(amp, out, inn), op.upper_hermitian)
+
+
+def _alone(H, nev, **kw):
+    vals, vecs, info = dev.krylov_map_solve(H, nev, **kw)
+    return vals, np.array(vecs), info
+
+
+def _two_component_problems(ctx, ratios, fillings, sectors):
+    """(map, label) for each (ratio, filling, sector): operators shared per ratio, bases per
+    (filling, sector)."""
+    out = []
+    spaces = {}
+    for f in fillings:
+        subs, ks = me.ED_momentum_subspaces(W.batched_para(1.0), f)
+        for s in sectors:
+            spaces[(f, s)] = subs[s]
+    for r in ratios:
+        para = W.batched_para(r)
+        op = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+        dop = dev.DeviceOperator.from_mboperator(ctx, op, 24)
+        for f in fillings:
+            for s in sectors:
+                sp = spaces[(f, s)]
+                out.append((dev.ROCmLinearMap(dop, dev.device_basis_of(sp, ctx), sp),
+                            (r, f, s), op))
+    return out
+
+
+def test_batch_small_all_sectors_bitwise_equal_alone(gpu_ctx, oracle_mod):
+    print("[batch] small 3x5 N=5, 15 sectors, nev 8", flush=True)
+    w = W.small()
+    op = w.operator()
+    subspaces, _ = me.ED_momentum_subspaces(w.para, 5)
+    maps = [dev.create_gpu_linearmap(op, sp) for sp in subspaces]
+    with B.BatchSolver(gpu_ctx) as bs:
+        ids = [bs.add(m, w.nev, want_vectors=True, seed=100 + i) for i, m in enumerate(maps)]
+        res = bs.run()
+    full = _oracle_full(oracle_mod, op)
+    for i, (m, pid) in enumerate(zip(maps, ids)):
+        r = res[pid]
+        vals, vecs, info = _alone(m, w.nev, seed=100 + i)
+        assert r.converged
+        np.testing.assert_array_equal(r.values, vals)
+        np.testing.assert_array_equal(np.array(r.vectors), vecs)
+        assert r.info["matvecs"] == info["matvecs"]
+        assert r.info["iterations"] == info["iterations"]
+        ev = np.linalg.eigvalsh(oracle_mod.dense_matrix(subspaces[i].list, full))
+        np.testing.assert_allclose(r.values, ev[: w.nev], rtol=1e-10, atol=1e-12)
+
+
+def test_batch_two_component_uneven_sizes(gpu_ctx, oracle_mod):
+    """Different operators (inter-component ratio) and bases (fillings, sectors) in one batch."""
+    print("[batch] two-component 3x4, 3 ratios x 3 fillings x 2 sectors, nev 4", flush=True)
+    probs = _two_component_problems(gpu_ctx, [0.6, 1.0, 1.9], [(2, 3), (3, 3), (4, 4)], [0, 7])
+    with B.BatchSolver(gpu_ctx) as bs:
+        ids = [bs.add(m, W.BATCH_NEV, want_vectors=True) for m, _, _ in probs]
+        res = bs.run()
+    checked = 0
+    for (m, label, op), pid in zip(probs, ids):
+        r = res[pid]
+        vals, vecs, _ = _alone(m, W.BATCH_NEV)
+        assert r.converged, label
+        np.testing.assert_array_equal(r.values, vals)
+        np.testing.assert_array_equal(np.array(r.vectors), vecs)
+        if label[1] == (2, 3) and checked < 2:     # dense check on the smallest sectors
+            ev = np.linalg.eigvalsh(oracle_mod.dense_matrix(m.space.list, _oracle_full(oracle_mod, op)))
+            np.testing.assert_allclose(r.values, ev[: W.BATCH_NEV], rtol=1e-10, atol=1e-12)
+            checked += 1
+    assert checked == 2
+
+
+def test_batch_continuous_admission_and_removal(gpu_ctx):
+    """Problems admitted while others are mid-solve, one removed mid-run: every collected
+    result equals its solve alone."""
+    print("[batch] continuous admission", flush=True)
+    probs = _two_component_problems(gpu_ctx, [0.8, 1.5], [(2, 4), (3, 4)], [1, 5])
+    with B.BatchSolver(gpu_ctx) as bs:
+        ids = {}
+        for i in range(3):
+            ids[i] = bs.add(probs[i][0], W.BATCH_NEV, want_vectors=True, seed=i)
+        bs.step(7)
+        for i in range(3, 6):
+            ids[i] = bs.add(probs[i][0], W.BATCH_NEV, want_vectors=True, seed=i)
+        bs.step(3)
+        bs.remove(ids.pop(4))
+        for i in range(6, len(probs)):
+            ids[i] = bs.add(probs[i][0], W.BATCH_NEV, want_vectors=True, seed=i)
+        res = bs.run(max_steps_per_call=5)
+        assert bs.count() == (0, 0)
+    assert sorted(res) == sorted(ids.values())
+    for i, pid in ids.items():
+        vals, vecs, _ = _alone(probs[i][0], W.BATCH_NEV, seed=i)
+        np.testing.assert_array_equal(res[pid].values, vals)
+        np.testing.assert_array_equal(np.array(res[pid].vectors), vecs)
+
+
+def test_batch_not_converged_and_vec0_match_alone(gpu_ctx):
+    w = W.small()
+    op = w.operator()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][3]
+    H = dev.create_gpu_linearmap(op, sp)
+    rng = np.random.default_rng(5)
+    v0 = rng.normal(size=sp.length) + 1j * rng.normal(size=sp.length)
+    with B.BatchSolver(gpu_ctx) as bs:
+        a = bs.add(H, 6, krylovdim=10, maxiter=1, want_vectors=True)
+        b = bs.add(H, 3, vec0=v0, want_vectors=False)
+        res = bs.run()
+    assert not res[a].converged and res[a].info["iterations"] == 1
+    vals, vecs, info = _alone(H, 6, krylovdim=10, maxiter=1)
+    assert info["converged"] < 6
+    np.testing.assert_array_equal(res[a].values, vals)
+    np.testing.assert_array_equal(np.array(res[a].vectors), vecs)
+    vals, _, _ = dev.krylov_map_solve(H, 3, vec0=v0, want_vectors=False)
+    np.testing.assert_array_equal(res[b].values, vals)
+    assert res[b].vectors is None
+
+
+def _map_on(ctx, op, sp):
+    system, sector = sp.sector_info
+    basis = dev.DeviceBasis.upload(ctx, sp.list, sp.bits, system, sector)
+    return dev.ROCmLinearMap(dev.DeviceOperator.from_mboperator(ctx, op, sp.bits), basis, sp)
+
+
+def _drop(H):
+    H.op.close()
+    H.basis.close()
+
+
+def test_batch_budget_admission(gpu_ctx):
+    """A context with an HBM budget refuses a problem that does not fit, and admits it once
+    a finished problem has been collected."""
+    print("[batch] HBM budget admission", flush=True)
+    w = W.small()
+    op = w.operator()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][0]
+    ctx = dev.Context(0)
+    used0, _ = ctx.memory()
+    H = _map_on(ctx, op, sp)
+    used1, _ = ctx.memory()
+    _drop(H)
+    ctx.close()
+    need = B.problem_bytes(sp.length, 4, 30)
+    # room for the map, one problem and the batch's small staging buffers, not two problems
+    budget = used1 + need + need // 2
+    ctx = dev.Context(0, hbm_budget_bytes=budget)
+    H = _map_on(ctx, op, sp)
+    try:
+        with B.BatchSolver(ctx) as bs:
+            a = bs.add(H, 4)
+            with pytest.raises(EDMError) as e:
+                bs.add(H, 4)
+            assert e.value.code == EDM_ENOMEM
+            while bs.step(50)[0]:
+                pass
+            r = bs.result(a)
+            b = bs.add(H, 4)
+            res = bs.run()
+        np.testing.assert_array_equal(res[b].values, r.values)
+        vals, _, _ = dev.krylov_map_solve(H, 4, want_vectors=False)
+        np.testing.assert_array_equal(r.values, vals)
+    finally:
+        _drop(H)
+        ctx.close()
+
+
+def test_edsolve_batch_api(gpu_ctx):
+    w = W.small()
+    op = w.operator()
+    subspaces, _ = me.ED_momentum_subspaces(w.para, 5)
+    pairs = [(sp, op) for sp in subspaces[:4]]
+    out = me.EDsolve_batch(pairs, N=4)
+    for (sp, _), (E, V) in zip(pairs, out):
+        E1, V1 = me.EDsolve(sp, op, N=4, method="map", device="gpu", method_info=False)
+        np.testing.assert_array_equal(E, E1)
+        for a, b in zip(V, V1):
+            np.testing.assert_array_equal(a.vec, b.vec)
+
+
+def test_served_results_equal_alone(gpu_ctx):
+    """Continuous batching: requests arriving while others run; each served result is bitwise
+    the request solved alone."""
+    from momentumed import serve as S
+    print("[serve] 0.6 s Poisson stream of 3x5/4x5/4x6 requests at 150/s", flush=True)
+    reqs = S.served_stream(rate=150.0, duration=0.6, seed=11, meshes=((3, 5), (4, 5), (4, 6)))
+    srv = S.SolveServer(hbm_budget_bytes=8 << 30, max_queue=1000)
+    try:
+        rep = srv.run(reqs, drain_s=120.0)
+    finally:
+        srv.close()
+    summ = rep.summary()
+    print(summ, flush=True)
+    assert rep.completed == len(reqs) and rep.rejected == 0 and rep.unfinished == 0
+    assert rep.steps > 0
+    for r in reqs:
+        np.testing.assert_array_equal(rep.results[r.rid], S.solve_alone(r, gpu_ctx))
