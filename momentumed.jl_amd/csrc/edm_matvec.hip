@@ -376,10 +376,6 @@ __device__ __forceinline__ const RankSearch &rank_row(const RankSearch &r, uint6
 __device__ __forceinline__ int64_t rank_issue(const RankSearch &r, uint64_t t) { return r(t); }
 __device__ __forceinline__ int64_t rank_finish(const RankSearch &, int64_t j) { return j; }
 
-// Same traversal as k_matvec_lds, with U candidates of a bucket resolved together (U table
-// lookups in flight, then U gathers in flight) and an optional partner window: only partners
-// j in [jmin, jmax) are gathered, and the result is added to y when `accumulate` is set. A
-// matvec split into passes over windows of x keeps each pass's gather footprint small.
 // One row of the LDS-staged matvec: y_i (with the diagonal when with_diag) over partners in
 // [jmin, jmax). The term image must already be in LDS at `base`.
 template <class Rank, bool HighFirst, int U>
