@@ -207,6 +207,13 @@ int edm_batch_set_quantum(edm_batch *bt, int64_t rows);
 /* Device bytes edm_batch_add allocates for a problem of dimension D. */
 int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t *bytes);
 
+/* Host eigen-decomposition of a real symmetric n x n matrix (row-major A, not modified):
+ * w[n] ascending, Z[n*n] with eigenvector i in Z[i*n .. i*n+n). The Rayleigh-Ritz step of
+ * every Lanczos restart (edm_eigsolve, batches) uses it on the projected matrix; KrylovKit
+ * does the same with LAPACK (the Lanczos factorization's eigen-decomposition inside
+ * eigsolve). No device work. */
+int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z);
+
 /* ----------------------------------------------------------------- device vector helpers */
 /* Fill D complex values at a device pointer: mode 0 = uniform [0,1) in re and im
  * (rand(ComplexF64), linear_map.jl:187), mode 1 = standard complex normal. Deterministic in

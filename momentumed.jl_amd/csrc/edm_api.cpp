@@ -482,6 +482,18 @@ int edm_fill_random(edm_ctx *ctx, void *v_dev, int64_t D, uint64_t seed, int32_t
     });
 }
 
+int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z)
+{
+    return guarded(nullptr, [&]() -> int {
+        EDM_REQUIRE(n >= 1 && A && w && Z, EDM_EINVAL, "bad argument");
+        std::vector<double> a(A, A + (size_t)n * n), ww, zz;
+        sym_eigh(n, a, ww, zz);
+        std::copy(ww.begin(), ww.end(), w);
+        std::copy(zz.begin(), zz.end(), Z);
+        return EDM_OK;
+    });
+}
+
 int edm_batch_create(edm_ctx *ctx, edm_batch **out)
 {
     return guarded(ctx, [&]() -> int {

@@ -408,7 +408,7 @@ static int host_cycle_end(edm_problem *p, bool *finishing)
 {
     const int m = p->m;
     std::vector<double> A = p->T;
-    jacobi_eigh(m, A, p->theta, p->Y);
+    sym_eigh(m, A, p->theta, p->Y);
     p->nconv = 0;
     for (int i = 0; i < m; ++i) p->res[i] = std::fabs(p->beta_last * p->Y[(size_t)i * m + (m - 1)]);
     while (p->nconv < p->nev && p->res[p->nconv] <= p->tol) ++p->nconv;
@@ -515,11 +515,17 @@ static int batch_step_once(edm_batch *bt)
 {
     edm_ctx *ctx = bt->ctx;
     hipStream_t st = ctx->stream;
-    std::vector<edm_problem *> act;
-    for (auto &kv : bt->probs)
-        if (kv.second->state == kRunning) act.push_back(kv.second);
-    if (act.empty()) return 0;
-    if (bt->quantum > 0) {
+    // Problems whose last step completed a cycle (j == m) are parked: their Rayleigh-Ritz
+    // runs on the host thread pool while this step's device work runs for the others, and
+    // their rotation is queued behind it; they take part again in the next step.
+    std::vector<edm_problem *> act, parked;
+    for (auto &kv : bt->probs) {
+        edm_problem *p = kv.second;
+        if (p->state != kRunning) continue;
+        (p->j == p->m ? parked : act).push_back(p);
+    }
+    if (act.empty() && parked.empty()) return 0;
+    if (bt->quantum > 0 && !act.empty()) {
         // deficit round robin over rows: each step credits every running problem with
         // `quantum` rows; a problem takes part when its credit covers its dimension, so
         // a problem of D rows steps about once every D / quantum steps and every problem
@@ -608,30 +614,54 @@ static int batch_step_once(edm_batch *bt)
         d.h2 = d.h + (act[q]->m + 1);
         d.beta = (double *)(d.h2 + (act[q]->m + 1));
     }
-    memcpy(pl.h.data() + vd_off, vd.data(), n * sizeof(BatchVecDesc));
-    bt->h_plan.ensure(st, pl.h.size());
-    memcpy(bt->h_plan.p, pl.h.data(), pl.h.size());
-    EDM_HIP_CHECK(hipMemcpyAsync(bt->d_plan.p, bt->h_plan.p, pl.h.size(), hipMemcpyHostToDevice, st));
+    if (n > 0) {
+        memcpy(pl.h.data() + vd_off, vd.data(), n * sizeof(BatchVecDesc));
+        bt->h_plan.ensure(st, pl.h.size());
+        memcpy(bt->h_plan.p, pl.h.data(), pl.h.size());
+        EDM_HIP_CHECK(hipMemcpyAsync(bt->d_plan.p, bt->h_plan.p, pl.h.size(),
+                                     hipMemcpyHostToDevice, st));
+        // device work of the step
+        for (edm_problem *p : single_mv)
+            launch_matvec(ctx, p->op, p->b, p->V[p->j], p->V[p->j + 1], ApplyMode::Forward);
+        launch_batch_mv(ctx, dplan_base + mv_off, (const int *)(dplan_base + mvblk_off),
+                        (int64_t)mvblk.size(), lds);
+        const BatchVecDesc *dvd = (const BatchVecDesc *)(dplan_base + vd_off);
+        bblas_cgs2_norm(ctx, dvd, n, maxvec, (const int *)(dplan_base + t2p_off),
+                        (int64_t)tile2prob.size(), (const int *)(dplan_base + b2p_off),
+                        (int64_t)blk2prob.size());
+        bblas_scale(ctx, dvd, (const int *)(dplan_base + b2p_off), (int64_t)blk2prob.size());
+        EDM_HIP_CHECK(hipMemcpyAsync(bt->h_coef.p, bt->d_coef.p, coef_bytes,
+                                     hipMemcpyDeviceToHost, st));
+        ++bt->steps;
+    }
 
-    // device work of the step
-    for (edm_problem *p : single_mv)
-        launch_matvec(ctx, p->op, p->b, p->V[p->j], p->V[p->j + 1], ApplyMode::Forward);
-    launch_batch_mv(ctx, dplan_base + mv_off, (const int *)(dplan_base + mvblk_off),
-                    (int64_t)mvblk.size(), lds);
-    const BatchVecDesc *dvd = (const BatchVecDesc *)(dplan_base + vd_off);
-    bblas_cgs2_norm(ctx, dvd, n, maxvec, (const int *)(dplan_base + t2p_off),
-                    (int64_t)tile2prob.size(), (const int *)(dplan_base + b2p_off),
-                    (int64_t)blk2prob.size());
-    bblas_scale(ctx, dvd, (const int *)(dplan_base + b2p_off), (int64_t)blk2prob.size());
-    EDM_HIP_CHECK(hipMemcpyAsync(bt->h_coef.p, bt->d_coef.p, coef_bytes, hipMemcpyDeviceToHost, st));
+    // while the device works: Rayleigh-Ritz of the parked problems (independent, in
+    // parallel), then their restart / final-vector rotations, queued behind the step
+    std::vector<std::pair<edm_problem *, int>> rots, restarts;
+    std::vector<edm_problem *> finishing;
+    std::vector<int> ncols(parked.size());
+    std::vector<char> fins(parked.size());
+    bt->pool.run((int)parked.size(), [&](int i) {
+        bool fin = false;
+        ncols[i] = host_cycle_end(parked[i], &fin);
+        fins[i] = fin;
+    });
+    for (size_t i = 0; i < parked.size(); ++i) {
+        edm_problem *p = parked[i];
+        if (fins[i]) {
+            finishing.push_back(p);
+            if (ncols[i] > 0) rots.push_back({p, ncols[i]});
+        } else {
+            rots.push_back({p, ncols[i]});
+            restarts.push_back({p, ncols[i]});
+        }
+    }
+    queue_rotations(bt, rots);
+    for (auto &r : restarts) host_restart(r.first, r.second);
     EDM_HIP_CHECK(hipStreamSynchronize(st));
-    ++bt->steps;
+    for (edm_problem *p : finishing) finish_problem(p);
 
-    // host side: coefficients of every problem (serial: a breakdown launches device work),
-    // then the Rayleigh-Ritz of every problem at the end of a cycle (independent, in parallel)
-    std::vector<std::pair<edm_problem *, int>> rots;
-    std::vector<std::pair<edm_problem *, int>> restarts;
-    std::vector<edm_problem *> finishing, cycle_end;
+    // coefficients of the stepped problems (serial: a breakdown launches device work)
     std::vector<double2> h;
     for (int q = 0; q < n; ++q) {
         edm_problem *p = act[q];
@@ -643,30 +673,6 @@ static int batch_step_once(edm_batch *bt)
         for (int i = 0; i < nv; ++i) h[i] = make_double2(hc[i].x + h2c[i].x, hc[i].y + h2c[i].y);
         p->matvecs += 1;
         host_step(ctx, p, h, beta);
-        if (p->j == p->m) cycle_end.push_back(p);
-    }
-    std::vector<int> ncols(cycle_end.size());
-    std::vector<char> fins(cycle_end.size());
-    bt->pool.run((int)cycle_end.size(), [&](int i) {
-        bool fin = false;
-        ncols[i] = host_cycle_end(cycle_end[i], &fin);
-        fins[i] = fin;
-    });
-    for (size_t i = 0; i < cycle_end.size(); ++i) {
-        edm_problem *p = cycle_end[i];
-        if (fins[i]) {
-            finishing.push_back(p);
-            if (ncols[i] > 0) rots.push_back({p, ncols[i]});
-        } else {
-            rots.push_back({p, ncols[i]});
-            restarts.push_back({p, ncols[i]});
-        }
-    }
-    queue_rotations(bt, rots);
-    for (auto &r : restarts) host_restart(r.first, r.second);
-    if (!finishing.empty()) {
-        EDM_HIP_CHECK(hipStreamSynchronize(st));
-        for (edm_problem *p : finishing) finish_problem(p);
     }
     return (int)finishing.size();
 }

@@ -209,8 +209,8 @@ int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, in
                  double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
                  double *vecs, edm_solve_info *info);
 const char *edm_global_error();
-// cyclic Jacobi eigensolver for a small real symmetric matrix (edm_lanczos.cpp)
-void jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z);
+// eigensolver for a small real symmetric matrix (edm_lanczos.cpp)
+void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z);
 // batched matvec over many problems (edm_matvec.hip). batch_mv_desc writes the problem's
 // descriptor (batch_mv_desc_size() bytes) for y = H x and returns false when the operator has
 // no LDS-staged table (the caller then runs launch_matvec for it); blocks are 512 rows.

@@ -7,8 +7,8 @@
 // classical Gram-Schmidt applied twice ("twice is enough", Giraud, Langou & Rozloznik 2005).
 // Every vector lives on the device; per Lanczos step the host reads back only the projection
 // coefficients and the norm (a few hundred bytes). The projected matrix (tridiagonal plus an
-// arrow after each restart) is real symmetric and is diagonalised on the host by the cyclic
-// Jacobi method (Golub & Van Loan, Matrix Computations, 4th ed., section 8.5).
+// arrow after each restart) is real symmetric and is diagonalised on the host (sym_eigh:
+// Householder tridiagonalisation + implicit QL).
 //
 // Restart: keep the `keep` lowest Ritz vectors, keep = (3*krylovdim + 2*converged) / 5 (at
 // least the number of wanted pairs), rotate them into place with one tall-skinny product,
@@ -21,56 +21,134 @@
 #include <cmath>
 #include <numeric>
 
-// Cyclic Jacobi for a real symmetric n x n matrix A (row-major, destroyed). On return
-// w[i] ascending and Z[:, i] (column-major, ldz = n) the matching orthonormal eigenvectors.
-void jacobi_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z)
+// Eigen-decomposition of a real symmetric n x n matrix A (row-major, destroyed): w[i]
+// ascending and Z[:, i] (column-major, ldz = n) the matching orthonormal eigenvectors.
+// Householder reduction to tridiagonal form with the reflections accumulated (Golub & Van
+// Loan, Matrix Computations, 4th ed., Alg. 8.3.1), then the implicit QL iteration with
+// Wilkinson shifts on the tridiagonal matrix, each plane rotation applied to the accumulated
+// basis (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11, 1968). O(n^3) with small
+// constants: 0.03 ms at n = 30, the Rayleigh-Ritz step of every Lanczos restart.
+void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z)
 {
-    Z.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
     auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
-    double fro = 0.0;
-    for (double v : A) fro += v * v;
-    fro = std::sqrt(fro);
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0;
-        for (int p = 0; p < n; ++p)
-            for (int q = p + 1; q < n; ++q) off += a(p, q) * a(p, q);
-        if (std::sqrt(off) <= 1e-300 + 1e-17 * fro) break;
-        for (int p = 0; p < n - 1; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                double apq = a(p, q);
-                if (apq == 0.0) continue;
-                double tau = (a(q, q) - a(p, p)) / (2.0 * apq);
-                double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
-                double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
-                for (int k = 0; k < n; ++k) {   // A <- A J (columns p, q)
-                    double akp = a(k, p), akq = a(k, q);
-                    a(k, p) = c * akp - s * akq;
-                    a(k, q) = s * akp + c * akq;
+    // work on A / 2^e with 2^e >= max|A_ij| (an exact scaling) so that plain
+    // sqrt(f^2 + g^2) neither overflows nor underflows
+    double amax = 0.0;
+    for (double x : A) amax = std::max(amax, std::fabs(x));
+    int ex = 0;
+    if (amax > 0.0) (void)std::frexp(amax, &ex);
+    amax = std::ldexp(1.0, ex);
+    const double inv = std::ldexp(1.0, -ex);
+    for (double &x : A) x *= inv;
+    auto hyp = [](double x, double y) { return std::sqrt(x * x + y * y); };
+    // the accumulated orthogonal basis, stored by columns: qt(c, r) = Q[r][c]
+    std::vector<double> Qt((size_t)n * n, 0.0);
+    auto qt = [&](int c, int r) -> double & { return Qt[(size_t)c * n + r]; };
+    for (int i = 0; i < n; ++i) qt(i, i) = 1.0;
+    std::vector<double> v(n), pv(n), sr(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        // reflect x = A[k+1:, k] onto -sign(x0) |x| e1
+        double xn = 0.0;
+        for (int i = k + 1; i < n; ++i) xn += a(i, k) * a(i, k);
+        xn = std::sqrt(xn);
+        double tail = 0.0;
+        for (int i = k + 2; i < n; ++i) tail += a(i, k) * a(i, k);
+        if (tail == 0.0) continue;   // already tridiagonal in this column
+        double x0 = a(k + 1, k);
+        double alpha = x0 >= 0.0 ? -xn : xn;
+        for (int i = k + 1; i < n; ++i) v[i] = a(i, k);
+        v[k + 1] -= alpha;
+        double vv = 0.0;
+        for (int i = k + 1; i < n; ++i) vv += v[i] * v[i];
+        double beta = 2.0 / vv;
+        // A22 <- H A22 H with H = I - beta v v^T:  p = beta A22 v, K = beta/2 v.p,
+        // u = p - K v, A22 -= v u^T + u v^T
+        double vp = 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            double s = 0.0;
+            for (int j = k + 1; j < n; ++j) s += a(i, j) * v[j];
+            pv[i] = beta * s;
+            vp += v[i] * pv[i];
+        }
+        double K = 0.5 * beta * vp;
+        for (int i = k + 1; i < n; ++i) pv[i] -= K * v[i];
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j < n; ++j) a(i, j) -= v[i] * pv[j] + pv[i] * v[j];
+        a(k + 1, k) = alpha;
+        a(k, k + 1) = alpha;
+        for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+        // Q <- Q H: Q[r][:] -= beta (Q[r][:] . v) v^T
+        std::fill(sr.begin(), sr.end(), 0.0);
+        for (int j = k + 1; j < n; ++j) {
+            const double vj = v[j];
+            const double *col = &Qt[(size_t)j * n];
+            for (int r = 0; r < n; ++r) sr[r] += col[r] * vj;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            const double bvj = beta * v[j];
+            double *col = &Qt[(size_t)j * n];
+            for (int r = 0; r < n; ++r) col[r] -= sr[r] * bvj;
+        }
+    }
+    std::vector<double> d(n), e(n, 0.0);
+    for (int i = 0; i < n; ++i) d[i] = a(i, i);
+    for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
+    const double eps = 2.220446049250313e-16;
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0; iter < 200; ++iter) {
+            int m = l;
+            for (; m + 1 < n; ++m) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= eps * dd) break;
+            }
+            if (m == l) break;
+            // Wilkinson shift from the leading 2x2 block of the unreduced part
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hyp(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = m - 1;
+            bool deflated = false;
+            for (; i >= l; --i) {
+                double f = s * e[i], bb = c * e[i];
+                r = hyp(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {   // exact split: undo the pending shift and restart
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    deflated = true;
+                    break;
                 }
-                for (int k = 0; k < n; ++k) {   // A <- J^T A (rows p, q)
-                    double apk = a(p, k), aqk = a(q, k);
-                    a(p, k) = c * apk - s * aqk;
-                    a(q, k) = s * apk + c * aqk;
-                }
-                for (int k = 0; k < n; ++k) {   // Z <- Z J
-                    double zkp = Z[(size_t)p * n + k], zkq = Z[(size_t)q * n + k];
-                    Z[(size_t)p * n + k] = c * zkp - s * zkq;
-                    Z[(size_t)q * n + k] = s * zkp + c * zkq;
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * bb;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - bb;
+                double *ci = &Qt[(size_t)i * n], *ci1 = &Qt[(size_t)(i + 1) * n];
+                for (int k2 = 0; k2 < n; ++k2) {   // columns i, i+1 of the basis
+                    double zi = ci[k2], zi1 = ci1[k2];
+                    ci1[k2] = s * zi + c * zi1;
+                    ci[k2] = c * zi - s * zi1;
                 }
             }
+            if (deflated) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
     }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return a(i, i) < a(j, j); });
-    std::vector<double> Zs((size_t)n * n);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
     w.resize(n);
+    Z.assign((size_t)n * n, 0.0);
     for (int c = 0; c < n; ++c) {
-        w[c] = a(idx[c], idx[c]);
-        std::copy(Z.begin() + (size_t)idx[c] * n, Z.begin() + (size_t)idx[c] * n + n,
-                  Zs.begin() + (size_t)c * n);
+        w[c] = d[idx[c]] * amax;
+        std::copy(Qt.begin() + (size_t)idx[c] * n, Qt.begin() + (size_t)idx[c] * n + n,
+                  Z.begin() + (size_t)c * n);
     }
-    Z.swap(Zs);
 }
 
 namespace {
@@ -245,7 +323,7 @@ int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, in
         }
         // Rayleigh-Ritz on the projected matrix
         std::vector<double> A = T;
-        jacobi_eigh(m, A, theta, Y);
+        sym_eigh(m, A, theta, Y);
         nconv = 0;
         for (int i = 0; i < m; ++i) res[i] = std::fabs(beta_last * Y[(size_t)i * m + (m - 1)]);
         while (nconv < nev && res[nconv] <= tol) ++nconv;

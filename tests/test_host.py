@@ -215,3 +215,54 @@ def test_product_never_imports_the_oracle():
             if f.endswith((".py", ".cpp", ".hip", ".h", ".jl")):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "import oracle" not in src and "liboracle" not in src and "orc_" not in src, f
+
+
+def _eigh_lib(A):
+    import ctypes
+    from momentumed import _lib
+    L = _lib.load()
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    w = np.empty(n)
+    Z = np.empty((n, n))
+    _lib.check(L.edm_projected_eigh(n, A.ctypes.data, w.ctypes.data, Z.ctypes.data))
+    return w, Z            # Z[i] = eigenvector i
+
+
+@pytest.mark.parametrize("kind", ["random", "arrow", "tridiag", "degenerate", "diag", "tiny"])
+def test_projected_eigh_matches_lapack(kind):
+    """The host Rayleigh-Ritz eigensolver against LAPACK (numpy.linalg.eigh): eigenvalues to
+    1e-13 of ||A||, orthonormal vectors, residuals A z = w z to 1e-13 of ||A||."""
+    rng = np.random.default_rng({"random": 1, "arrow": 2, "tridiag": 3, "degenerate": 4,
+                                 "diag": 5, "tiny": 6}[kind])
+    mats = []
+    for n in (1, 2, 3, 7, 30, 40):
+        if kind == "random":
+            B = rng.normal(size=(n, n)); A = B + B.T
+        elif kind == "arrow":          # kept Ritz values + coupling row + tridiagonal tail
+            A = np.zeros((n, n)); k = (3 * n) // 5
+            A[np.arange(n), np.arange(n)] = rng.normal(size=n)
+            for c in range(min(k, n - 1)):
+                A[c, k] = A[k, c] = rng.normal() * 1e-3
+            for i in range(k, n - 1):
+                A[i, i + 1] = A[i + 1, i] = abs(rng.normal())
+        elif kind == "tridiag":
+            A = np.diag(rng.normal(size=n)) + np.diag(np.abs(rng.normal(size=n - 1)), 1)
+            A = A + np.triu(A, 1).T
+        elif kind == "degenerate":     # repeated eigenvalues
+            Qm, _ = np.linalg.qr(rng.normal(size=(n, n)))
+            ev = np.repeat(rng.normal(size=(n + 2) // 3), 3)[:n]
+            A = (Qm * ev) @ Qm.T; A = 0.5 * (A + A.T)
+        elif kind == "diag":
+            A = np.diag(rng.permutation(n).astype(float))
+        else:
+            A = np.diag(rng.normal(size=n) * 1e-200)
+        mats.append(A)
+    for A in mats:
+        w, Z = _eigh_lib(A)
+        ref = np.linalg.eigvalsh(A)
+        scale = max(np.abs(A).max(), 1e-300)
+        assert np.all(np.diff(w) >= 0)
+        np.testing.assert_allclose(w, ref, rtol=0, atol=1e-13 * scale * A.shape[0])
+        np.testing.assert_allclose(Z @ Z.T, np.eye(A.shape[0]), atol=1e-13)
+        assert np.abs(A @ Z.T - Z.T * w).max() <= 1e-13 * scale * A.shape[0]
