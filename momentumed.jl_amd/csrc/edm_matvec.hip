@@ -18,6 +18,8 @@
 // row and writes y_i once: no atomics.
 #include "edm_internal.h"
 
+#include <climits>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -376,14 +378,135 @@ __device__ __forceinline__ const RankSearch &rank_row(const RankSearch &r, uint6
 __device__ __forceinline__ int64_t rank_issue(const RankSearch &r, uint64_t t) { return r(t); }
 __device__ __forceinline__ int64_t rank_finish(const RankSearch &, int64_t j) { return j; }
 
+// Flip the sign of a double when neg is 1 (exact: toggles the sign bit).
+__device__ __forceinline__ double flip_sign(double v, uint32_t neg)
+{
+    return __hiloint2double(__double2hiint(v) ^ (int)(neg << 31), __double2loint(v));
+}
+
+// Lookup traits: a complete sector's rank tables find every partner of a sector-conserving
+// operator; the binary search may miss (partner outside a partial basis).
+template <class Rank> struct RankMayMiss { static constexpr bool value = true; };
+template <> struct RankMayMiss<RankLin> { static constexpr bool value = false; };
+
+// Row lookup for the one-candidate path: 32-bit halves (nlo <= 32 and the high half of a
+// state below 2^32 for every basis with rank tables), partner index as a 32-bit value.
+struct LinRow32 {
+    const uint32_t *jhi, *jlo;
+    const uint16_t *jlo16;
+    int nlo;
+    uint32_t lomask, hs, jhs;
+};
+__device__ __forceinline__ LinRow32 rank_row32(const RankLin &r, uint64_t s)
+{
+    uint32_t hs = (uint32_t)(s >> r.nlo);
+    return LinRow32{r.jhi, r.jlo, r.jlo16, r.nlo, (uint32_t)r.lomask, hs, r.jhi[hs]};
+}
+__device__ __forceinline__ int64_t rank_find(const LinRow32 &r, uint64_t t)
+{
+    uint32_t h = (uint32_t)(t >> r.nlo);
+    uint32_t hv = h == r.hs ? r.jhs : r.jhi[h];
+    uint32_t li = (uint32_t)t & r.lomask;
+    uint32_t lv = r.jlo16 ? (uint32_t)r.jlo16[li] : r.jlo[li];
+    return (int64_t)(hv + lv);
+}
+__device__ __forceinline__ const RankSearch &rank_row32(const RankSearch &r, uint64_t) { return r; }
+__device__ __forceinline__ int64_t rank_find(const RankSearch &r, uint64_t t) { return r(t); }
+
+// One row, one candidate at a time: pair buckets (key >= 64) hold only two-body entries and
+// single-orbital buckets only one-body entries, so each bucket kind has its own loop; the
+// per-bucket part of the fermion sign is hoisted; the sign is folded into the amplitude.
+template <class Rank, bool HighFirst>
+__device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned char *base,
+                                           const CompactDev &t, const Rank &rank,
+                                           const double2 *__restrict__ x, int64_t jmin,
+                                           int64_t jmax, int with_diag, bool windowed)
+{
+    const uint16_t *s_keyptr = (const uint16_t *)base;
+    const double2 *s_diag = (const double2 *)(base + t.off_diag);
+    const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
+    double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
+    const auto rrow = rank_row32(rank, s);
+
+    auto take = [&](int e, uint64_t target, uint32_t neg) {
+        int64_t j = rank_find(rrow, target);
+        if (RankMayMiss<Rank>::value && j < 0) return;
+        if (windowed && (j < jmin || j >= jmax)) return;
+        double2 a = t.amp[e];
+        double2 xv = x[j];
+        double ax = flip_sign(a.x, neg), ay = flip_sign(a.y, neg);
+        yr += ax * xv.x - ay * xv.y;
+        yi += ax * xv.y + ay * xv.x;
+    };
+    auto enter = [&](int key, int &e0, int &e1) {
+        if (with_diag) {
+            double2 dg = s_diag[key];
+            dr += dg.x;
+            di += dg.y;
+        }
+        e0 = s_keyptr[key];
+        e1 = s_keyptr[key + 1];
+    };
+    uint64_t rem = s;
+    while (rem) {
+        int b = HighFirst ? 63 - __builtin_clzll(rem) : __builtin_ctzll(rem);
+        rem ^= 1ULL << b;
+        const uint64_t ob = 1ULL << b;
+        if (b < t.nkeys) {   // one-body entries: in = {c}
+            int e0, e1;
+            enter(b, e0, e1);
+            const uint64_t mid = s & ~ob;
+            const uint32_t pout = __popcll(mid & pm_one(b)) & 1u;
+            for (int e = e0; e < e1; ++e) {
+                const int c = s_code[e] & 0xFF;
+                const uint64_t inm = 1ULL << c;
+                if (mid & inm) continue;
+                const uint32_t neg = (pout + (uint32_t)__popcll(mid & pm_one(c))) & 1u;
+                take(e, mid | inm, neg);
+            }
+        }
+        uint64_t low = s & (ob - 1ULL);
+        const int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            int a = HighFirst ? 63 - __builtin_clzll(low) : __builtin_ctzll(low);
+            low ^= 1ULL << a;
+            const int key = kb + a;
+            if (key >= t.nkeys) continue;
+            int e0, e1;
+            enter(key, e0, e1);
+            const uint64_t mid = s & ~(ob | (1ULL << a));
+            const uint32_t pout = __popcll(mid & pm_pair(a, b)) & 1u;
+            for (int e = e0; e < e1; ++e) {   // two-body entries: in = {c, d}, c < d
+                const unsigned code = s_code[e];
+                const int c = code & 0xFF, d = code >> 8;
+                const uint64_t bd = 1ULL << d;
+                const uint64_t inm = bd | (1ULL << c);
+                if (mid & inm) continue;
+                const uint32_t neg = (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u;
+                take(e, mid | inm, neg);
+            }
+        }
+    }
+    if (with_diag) {
+        double2 xi = x[i];
+        yr += dr * xi.x - di * xi.y;
+        yi += dr * xi.y + di * xi.x;
+    }
+    return make_double2(yr, yi);
+}
+
 // One row of the LDS-staged matvec: y_i (with the diagonal when with_diag) over partners in
-// [jmin, jmax). The term image must already be in LDS at `base`.
+// [jmin, jmax) (jmin = 0, jmax = INT64_MAX: no window). The term image must already be in
+// LDS at `base`.
 template <class Rank, bool HighFirst, int U>
 __device__ __forceinline__ double2 ldsu_row(int64_t i, uint64_t s, const unsigned char *base,
                                             const CompactDev &t, const Rank &rank,
                                             const double2 *__restrict__ x, int64_t jmin,
                                             int64_t jmax, int with_diag)
 {
+    if constexpr (U == 1)
+        return row_one<Rank, HighFirst>(i, s, base, t, rank, x, jmin, jmax, with_diag,
+                                        jmin > 0 || jmax < (int64_t)INT64_MAX);
     const uint16_t *s_keyptr = (const uint16_t *)base;
     const double2 *s_diag = (const double2 *)(base + t.off_diag);
     const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
@@ -523,8 +646,8 @@ k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ bl
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
     double2 v = P.use_lin
-        ? ldsu_row<RankLin, true, 1>(i, P.states[i], base, P.t, P.lin, P.x, 0, P.D, 1)
-        : ldsu_row<RankSearch, true, 1>(i, P.states[i], base, P.t, P.search, P.x, 0, P.D, 1);
+        ? ldsu_row<RankLin, true, 1>(i, P.states[i], base, P.t, P.lin, P.x, 0, INT64_MAX, 1)
+        : ldsu_row<RankSearch, true, 1>(i, P.states[i], base, P.t, P.search, P.x, 0, INT64_MAX, 1);
     P.y[i] = v;
 }
 
@@ -653,6 +776,7 @@ static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct
     int passes = matvec_passes();
     for (int p = 0; p < passes; ++p) {
         int64_t jmin = b->D * p / passes, jmax = b->D * (p + 1) / passes;
+        if (passes == 1) jmax = INT64_MAX;   // no window
         hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, U>), dim3(nb), dim3(kLdsBlock), ct.lds_bytes,
                            ctx->stream, b->D, b->d_states, cd, rank, x, y, jmin, jmax, p > 0 ? 1 : 0,
                            p == 0 ? 1 : 0);
