@@ -39,6 +39,7 @@ typedef struct edm_ctx edm_ctx;
 typedef struct edm_basis edm_basis;
 typedef struct edm_op edm_op;
 typedef struct edm_batch edm_batch;
+typedef struct edm_csr edm_csr;
 
 /* Description of a momentum-conserved system and particle numbers
  * (EDPara, MomentumED/src/preparation/init_parameter.jl:62-146, and the N_each_component
@@ -163,6 +164,27 @@ int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, 
 int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
                  const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+
+/* ----------------------------------------------------------------- stored matrix (:sparse) */
+/* SparseHmltMatrix + create_gpu_matrix (MomentumED/src/method/sparse_matrix.jl:65-191;
+ * MomentumEDCUDAExt.jl:147-152): the operator's matrix in the basis, built on the device and
+ * kept there as CSR (int64 row offsets, int32 column indices ascending within each row,
+ * complex128 values; entries of equal (row, column) summed, as the reference's CSC build
+ * does). Row i holds <i|H|j>. One- and two-body number-conserving operators; D < 2^31 and at
+ * most 2^31 contributions. */
+int edm_csr_build(edm_ctx *ctx, const edm_op *op, const edm_basis *b, edm_csr **out);
+int edm_csr_size(const edm_csr *m, int64_t *D, int64_t *nnz);
+/* Host copies: rowptr[D+1], cols[nnz], vals[2*nnz]. */
+int edm_csr_download(const edm_csr *m, int64_t *rowptr, int32_t *cols, double *vals);
+/* y = A x on device vectors (the CuSparseMatrixCSR product inside krylov_matrix_solve),
+ * stream-ordered. */
+int edm_csr_matvec(edm_ctx *ctx, const edm_csr *m, const void *x_dev, void *y_dev);
+/* krylov_matrix_solve(H_gpu, N; ishermitian=true, ...) (method/gpu_interface.jl:92-94 for
+ * matrices; MomentumED.jl:316-333): arguments and results as edm_eigsolve. */
+int edm_csr_eigsolve(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylovdim, double tol,
+                     int32_t maxiter, uint64_t seed, const double *vec0, double *vals,
+                     double *vecs, edm_solve_info *info);
+int edm_csr_destroy(edm_csr *m);
 
 /* ----------------------------------------------------------------- batched / served solves */
 /* The reference has no batch object: a sweep over sectors and parameter points is a host loop

@@ -482,6 +482,82 @@ int edm_fill_random(edm_ctx *ctx, void *v_dev, int64_t D, uint64_t seed, int32_t
     });
 }
 
+int edm_csr_build(edm_ctx *ctx, const edm_op *op, const edm_basis *b, edm_csr **out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && out, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL,
+                    "operator and basis must belong to the context");
+        *out = nullptr;
+        *out = csr_build(ctx, op, b);
+        return EDM_OK;
+    });
+}
+
+int edm_csr_size(const edm_csr *m, int64_t *D, int64_t *nnz)
+{
+    return guarded(m ? m->ctx : nullptr, [&]() -> int {
+        EDM_REQUIRE(m, EDM_EINVAL, "NULL matrix");
+        if (D) *D = m->D;
+        if (nnz) *nnz = m->nnz;
+        return EDM_OK;
+    });
+}
+
+int edm_csr_download(const edm_csr *m, int64_t *rowptr, int32_t *cols, double *vals)
+{
+    return guarded(m ? m->ctx : nullptr, [&]() -> int {
+        EDM_REQUIRE(m && rowptr && cols && vals, EDM_EINVAL, "NULL argument");
+        hipStream_t st = m->ctx->stream;
+        EDM_HIP_CHECK(hipMemcpyAsync(rowptr, m->d_rowptr, (m->D + 1) * sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, st));
+        if (m->nnz > 0) {
+            EDM_HIP_CHECK(hipMemcpyAsync(cols, m->d_cols, m->nnz * sizeof(int32_t),
+                                         hipMemcpyDeviceToHost, st));
+            EDM_HIP_CHECK(hipMemcpyAsync(vals, m->d_vals, m->nnz * sizeof(double2),
+                                         hipMemcpyDeviceToHost, st));
+        }
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+        return EDM_OK;
+    });
+}
+
+int edm_csr_matvec(edm_ctx *ctx, const edm_csr *m, const void *x_dev, void *y_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && m && (m->D == 0 || (x_dev && y_dev)), EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(m->ctx == ctx, EDM_EINVAL, "matrix belongs to another context");
+        EDM_REQUIRE(x_dev != y_dev || m->D == 0, EDM_EINVAL, "x and y must not alias");
+        csr_matvec(ctx, m, (const double2 *)x_dev, (double2 *)y_dev);
+        return EDM_OK;
+    });
+}
+
+int edm_csr_eigsolve(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylovdim, double tol,
+                     int32_t maxiter, uint64_t seed, const double *vec0, double *vals,
+                     double *vecs, edm_solve_info *info)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && m && vals, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(m->ctx == ctx, EDM_EINVAL, "matrix belongs to another context");
+        int rc = run_eigsolve_mv(
+            ctx, m->D, [&](const double2 *x, double2 *y) { csr_matvec(ctx, m, x, y); }, nev,
+            krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+        if (rc == EDM_ENOCONV) ctx->last_error = "Krylov method fails. Cannot find N eigenvectors.";
+        return rc;
+    });
+}
+
+int edm_csr_destroy(edm_csr *m)
+{
+    if (!m) return EDM_OK;
+    return guarded(m->ctx, [&]() -> int {
+        (void)hipStreamSynchronize(m->ctx->stream);
+        csr_destroy(m);
+        return EDM_OK;
+    });
+}
+
 int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z)
 {
     return guarded(nullptr, [&]() -> int {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -208,6 +209,24 @@ bool verify_rank_tables(edm_basis *b);
 int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, int krylovdim,
                  double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
                  double *vecs, edm_solve_info *info);
+// the same Lanczos iteration on any stream-ordered y = A x (edm_lanczos.cpp)
+int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const double2 *, double2 *)> &mv,
+                    int nev, int krylovdim, double tol, int maxiter, uint64_t seed,
+                    const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+// stored matrix (edm_csr.hip): row keys (column << 32 | term << 1 | sign) of every
+// contribution, written at rowptr[r] (edm_matvec.hip)
+void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const int64_t *d_rowptr,
+                     uint64_t *d_keys);
+struct edm_csr {
+    edm_ctx *ctx = nullptr;
+    int64_t D = 0, nnz = 0;
+    int64_t *d_rowptr = nullptr;   // D + 1
+    int32_t *d_cols = nullptr;     // nnz, ascending within a row
+    double2 *d_vals = nullptr;     // nnz
+};
+edm_csr *csr_build(edm_ctx *ctx, const edm_op *op, const edm_basis *b);
+void csr_destroy(edm_csr *m);
+void csr_matvec(edm_ctx *ctx, const edm_csr *m, const double2 *x, double2 *y);
 const char *edm_global_error();
 // eigensolver for a small real symmetric matrix (edm_lanczos.cpp)
 void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z);

@@ -248,8 +248,17 @@ int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, in
                  double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
                  double *vecs, edm_solve_info *info)
 {
+    return run_eigsolve_mv(
+        ctx, b->D,
+        [&](const double2 *x, double2 *y) { launch_matvec(ctx, op, b, x, y, ApplyMode::Forward); },
+        nev, krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+}
+
+int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const double2 *, double2 *)> &mv,
+                    int nev, int krylovdim, double tol, int maxiter, uint64_t seed,
+                    const double *vec0, double *vals, double *vecs, edm_solve_info *info)
+{
     auto t0 = std::chrono::steady_clock::now();
-    const int64_t D = b->D;
     EDM_REQUIRE(D > 0, EDM_EINVAL, "empty Hilbert subspace");
     EDM_REQUIRE(nev >= 1, EDM_EINVAL, "N should be at least 1.");
     EDM_REQUIRE(nev <= D, EDM_EINVAL, "more eigenpairs requested than the subspace dimension");
@@ -292,7 +301,7 @@ int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, in
 
     for (iters = 1; iters <= maxiter; ++iters) {
         for (int j = k; j < m; ++j) {
-            launch_matvec(ctx, op, b, ws.V[j], ws.V[j + 1], ApplyMode::Forward);
+            mv(ws.V[j], ws.V[j + 1]);
             ++matvecs;
             cgs2(ws, j + 1, ws.V[j + 1], h);
             t(j, j) = h[j].x;

@@ -204,6 +204,26 @@ k_conn_fill(int64_t row0, int64_t nrows, const uint64_t *__restrict__ states, Op
         });
 }
 
+// Stored-matrix build: every contribution of row r as one sortable key
+// (column << 32 | term << 1 | sign), at rowptr[r]..
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_csr_keys(int64_t D, const uint64_t *__restrict__ states, OpDev op, Rank rank,
+           const int64_t *__restrict__ rowptr, uint64_t *__restrict__ keys)
+{
+    int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= D) return;
+    int64_t p = rowptr[r];
+    visit_row(
+        states[r], op, rank,
+        [&](uint32_t e, int64_t j, int neg) {
+            keys[p++] = ((uint64_t)j << 32) | ((uint64_t)(uint32_t)op.tterm[e] << 1) | (uint64_t)neg;
+        },
+        [&](uint32_t e) {
+            keys[p++] = ((uint64_t)r << 32) | ((uint64_t)(uint32_t)op.tterm[e] << 1);
+        });
+}
+
 template <class Rank>
 __global__ void __launch_bounds__(kRowBlock)
 k_conn_totals(int64_t D, const uint64_t *__restrict__ states, OpDev op, Rank rank,
@@ -904,6 +924,22 @@ void launch_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int
                                ctx->stream, row0, nrows, b->d_states, od, make_search(b),
                                d_rowptr, d_cols, d_terms, d_negs);
     }
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const int64_t *d_rowptr,
+                     uint64_t *d_keys)
+{
+    EDM_REQUIRE(op->fast, EDM_EINVAL,
+                "the stored matrix supports one- and two-body number-conserving operators");
+    if (b->D == 0) return;
+    OpDev od = opdev(op, op->fwd);
+    if (use_lin(op, b))
+        hipLaunchKernelGGL(k_csr_keys<RankLin>, dim3(nblk(b->D)), dim3(kRowBlock), 0, ctx->stream,
+                           b->D, b->d_states, od, make_lin(b), d_rowptr, d_keys);
+    else
+        hipLaunchKernelGGL(k_csr_keys<RankSearch>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                           ctx->stream, b->D, b->d_states, od, make_search(b), d_rowptr, d_keys);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

@@ -11,7 +11,8 @@ from .core import (HilbertSubspace, MBOperator, MBS64, MBS64Vector, Scatter, get
                    index_fit, isdiagonal, isupper, make_mask64, occ_list, parity_mask,
                    sort_merge_scatlist)
 from .device import (DEFAULT_GPU, GPU_AVAILABLE, activate_gpu_device, create_gpu_linearmap,
-                     krylov_map_solve, release_gpu_memory)
+                     create_gpu_matrix, krylov_map_solve, krylov_matrix_solve,
+                     release_gpu_memory)
 from .preparation import (EDPara, ED_momentum_subspaces, ED_scatterlist_onebody,
                           ED_scatterlist_twobody, ED_sortedScatterList_onebody,
                           ED_sortedScatterList_twobody, MBS_totalmomentum, momentum_residue)
@@ -25,5 +26,6 @@ __all__ = [
     "get_body", "index_fit", "isdiagonal", "isupper", "make_mask64", "occ_list", "parity_mask",
     "sort_merge_scatlist", "EDsolve", "GPU_AVAILABLE", "DEFAULT_GPU", "activate_gpu_device",
     "create_gpu_linearmap", "krylov_map_solve", "release_gpu_memory", "BatchSolver",
+    "create_gpu_matrix", "krylov_matrix_solve",
     "EDsolve_batch",
 ]

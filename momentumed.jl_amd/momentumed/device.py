@@ -19,7 +19,7 @@ from ._lib import EdmSolveInfo, EdmSystem, check
 __all__ = [
     "GPU_AVAILABLE", "DEFAULT_GPU", "activate_gpu_device", "release_gpu_memory", "Context",
     "get_context", "DeviceBasis", "DeviceOperator", "ROCmLinearMap", "create_gpu_linearmap",
-    "krylov_map_solve", "system_struct",
+    "krylov_map_solve", "system_struct", "DeviceCSR", "create_gpu_matrix", "krylov_matrix_solve",
 ]
 
 # gpu_interface.jl:25-30,39
@@ -332,6 +332,108 @@ def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True
     rc = ctx.L.edm_eigsolve(ctx.h, H.op.h, H.basis.h, N_eigen, krylovdim, float(tol), maxiter,
                             seed, v0.ctypes.data if v0 is not None else None, vals.ctypes.data,
                             vecs.ctypes.data if vecs is not None else None, ctypes.byref(info))
+    if rc not in (_lib.EDM_OK, _lib.EDM_ENOCONV):
+        check(rc, ctx.h)
+    out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
+                "converged": info.converged, "krylovdim": info.krylovdim,
+                "max_residual": info.max_residual, "seconds": info.seconds}
+    return vals, (list(vecs) if vecs is not None else None), out_info
+
+
+# ------------------------------------------------------------------------- stored matrix
+class DeviceCSR:
+    """The Hamiltonian of a subspace stored on the device as CSR (the CuSparseMatrixCSR
+    counterpart, MomentumEDCUDAExt.jl:147-152), built on the device from the term tables."""
+
+    def __init__(self, op: DeviceOperator, basis: DeviceBasis, space=None):
+        ctx = op.ctx
+        h = ctypes.c_void_p()
+        check(ctx.L.edm_csr_build(ctx.h, op.h, basis.h, ctypes.byref(h)), ctx.h)
+        self.ctx = ctx
+        self.h = h
+        self.space = space
+        D, nnz = ctypes.c_int64(), ctypes.c_int64()
+        check(ctx.L.edm_csr_size(h, ctypes.byref(D), ctypes.byref(nnz)), ctx.h)
+        self.D, self.nnz = D.value, nnz.value
+        self._fin = weakref.finalize(self, ctx.L.edm_csr_destroy, h)
+
+    def size(self):
+        return (self.D, self.D)
+
+    def download(self):
+        """(rowptr int64[D+1], cols int32[nnz], vals complex128[nnz]) on the host."""
+        rp = np.empty(self.D + 1, dtype=np.int64)
+        cols = np.empty(max(self.nnz, 1), dtype=np.int32)
+        vals = np.empty(max(self.nnz, 1), dtype=np.complex128)
+        check(self.ctx.L.edm_csr_download(self.h, rp.ctypes.data, cols.ctypes.data,
+                                          vals.ctypes.data), self.ctx.h)
+        return rp, cols[: self.nnz], vals[: self.nnz]
+
+    def matvec_device(self, y_ptr: int, x_ptr: int):
+        check(self.ctx.L.edm_csr_matvec(self.ctx.h, self.h, x_ptr, y_ptr), self.ctx.h)
+
+    def __call__(self, y, x=None):
+        """y = A x for torch complex128 device tensors or host numpy arrays."""
+        import torch
+        if x is None:
+            x, y = y, None
+        host = isinstance(x, np.ndarray)
+        xd = torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex128)).to(
+            f"cuda:{self.ctx.device}") if host else x
+        if len(xd) != self.D:
+            raise ValueError("Dimension of Hamiltonian matrix mismatches vector length.")
+        yd = torch.empty_like(xd)
+        torch.cuda.current_stream(xd.device).synchronize()
+        self.matvec_device(yd.data_ptr(), xd.data_ptr())
+        self.ctx.synchronize()
+        if host:
+            out = yd.cpu().numpy()
+            if y is not None:
+                y[:] = out
+                return y
+            return out
+        if y is not None:
+            y.copy_(yd)
+            return y
+        return yd
+
+    def close(self):
+        self._fin()
+
+
+def create_gpu_matrix(op, space, device: str = "rocm", ctx: Context | None = None) -> DeviceCSR:
+    """create_gpu_matrix(SparseHmltMatrix(subspace, scats), Val(device)) in one step
+    (sparse_matrix.jl:65-191 + MomentumEDCUDAExt.jl:147-152): the matrix is built on the
+    device from the operator's term list and the subspace's device basis."""
+    if device != "rocm":
+        raise ValueError(f"Unknown GPU {device}.")
+    ctx = ctx or get_context(0)
+    basis = device_basis_of(space, ctx)
+    dop = DeviceOperator.from_mboperator(ctx, op, space.bits)
+    return DeviceCSR(dop, basis, space)
+
+
+def krylov_matrix_solve(H: DeviceCSR, N_eigen: int, *, ishermitian: bool = True, vec0=None,
+                        krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100,
+                        seed: int = 0x5EED, want_vectors: bool = True, **_ignored):
+    """krylov_matrix_solve (MomentumED.jl:316-333) on the device matrix; the same Lanczos
+    iteration and defaults as krylov_map_solve. Returns (vals, vecs, info)."""
+    if not ishermitian:
+        raise ValueError("the device eigensolver implements the Hermitian (:SR) case")
+    D = H.D
+    N_eigen = min(N_eigen, D)
+    vals = np.empty(N_eigen, dtype=np.float64)
+    vecs = np.empty((N_eigen, D), dtype=np.complex128) if want_vectors else None
+    v0 = None
+    if vec0 is not None:
+        v0 = np.ascontiguousarray(np.asarray(vec0, dtype=np.complex128).reshape(-1))
+        if v0.size != D:
+            raise ValueError("Dimension of Hamiltonian matrix mismatches vector length.")
+    info = EdmSolveInfo()
+    ctx = H.ctx
+    rc = ctx.L.edm_csr_eigsolve(ctx.h, H.h, N_eigen, krylovdim, float(tol), maxiter, seed,
+                                v0.ctypes.data if v0 is not None else None, vals.ctypes.data,
+                                vecs.ctypes.data if vecs is not None else None, ctypes.byref(info))
     if rc not in (_lib.EDM_OK, _lib.EDM_ENOCONV):
         check(rc, ctx.h)
     out_info = {"iterations": info.iterations, "matvecs": info.matvecs,

@@ -14,7 +14,7 @@ __all__ = ["EDsolve"]
 _DEVICES = ("cpu", "gpu", "rocm", "cuda", "oneapi", "metal", "multi_cuda")
 
 
-def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method: str = "map",
+def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method: str = "sparse",
             device: str = "gpu", showtime: bool = False, ishermitian: bool = True,
             element_type=float, index_type=np.int32, min_sparse_dim: int = 100,
             max_dense_dim: int = 200, map_warning_dim: int = 20000, method_info: bool = True,
@@ -22,10 +22,15 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
     """Lowest N eigenpairs of `hamiltonian` in `subspace`.
 
     Same call shape and return value as the reference: ``energies, vectors = EDsolve(subspace,
-    H; N, method, device, ...)`` with energies ascending (float64[N]) and vectors a list of
-    MBS64Vector (MomentumED.jl:375-480). This package implements the matrix-free map on the
-    device (method="map", device="gpu"/"rocm"); KrylovKit keywords krylovdim, tol, maxiter,
-    vec0 are honoured, plus `seed` for the start vector."""
+    H; N, method, device, ...)`` or ``EDsolve(subspace, scat_list1, scat_list2, ...; ...)``, with
+    energies ascending (float64[N]) and vectors a list of MBS64Vector (MomentumED.jl:217-480).
+    Both Krylov methods run on the device: method="sparse" (the default, as in the reference)
+    builds the stored matrix on the device and solves with it (MomentumED.jl:259-333);
+    method="map" applies the matrix-free map (MomentumED.jl:375-480). KrylovKit keywords
+    krylovdim, tol, maxiter, vec0 are honoured, plus `seed` for the start vector. The dense
+    method (host LAPACK) is not part of this package; the reference's automatic switch from
+    :sparse to :dense below min_sparse_dim is therefore not taken: small subspaces stay on
+    the stored-matrix path, which gives the same eigenpairs."""
     if N < 1:
         raise AssertionError("N should be at least 1.")
     if method not in ("sparse", "dense", "map"):
@@ -33,17 +38,20 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
     if device not in _DEVICES:
         raise AssertionError(f"Unknown device: {device}.")
     if not isinstance(hamiltonian, MBOperator):
-        # EDsolve(subspace, scat_lists...) form (MomentumED.jl:217-231)
+        # EDsolve(subspace, scat_lists...) form (MomentumED.jl:217-231): upper-Hermitian terms
         if method == "map":
             raise ValueError("Linear map methods are only supported when input Hamitonian is "
                              "MBOperator instead of Vector{Scatter}.")
         lists = [hamiltonian, *more_scats]
         if not all(isinstance(s, Scatter) for lst in lists for s in lst):
             raise TypeError("expected MBOperator or lists of Scatter")
-    if method != "map":
-        raise NotImplementedError(
-            f"method={method!r}: this package implements the matrix-free device map "
-            "(method='map'); stored-matrix methods are not part of it")
+        if not ishermitian:
+            raise AssertionError("Current Hamiltonian matrix construction assumes it being "
+                                 "Hermitian.")
+        hamiltonian = MBOperator(*lists, upper_hermitian=True)
+    if method == "dense":
+        raise NotImplementedError("method='dense' (host LAPACK) is not part of this package; "
+                                  "use method='sparse' or 'map' on the device")
     if ishermitian and not isupper(hamiltonian):
         raise AssertionError("Use upper_hermitian form of Hamiltonian operator when "
                              "ishermitian = true.")
@@ -52,11 +60,11 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
             raise AssertionError(f"Hilbert space dimension {len(subspace)} exceeds limit of "
                                  f"index_type={index_type}.")
     dim = len(subspace)
-    if dim < map_warning_dim and method_info:
+    if method == "map" and dim < map_warning_dim and method_info:
         warnings.warn(f"Linear map may be slow for dim={dim}. Consider using :sparse method.")
     if device == "cpu":
-        raise NotImplementedError("device='cpu': the CPU map is the reference's; this package "
-                                  "runs the map on the GPU (device='gpu' or 'rocm')")
+        raise NotImplementedError("device='cpu': the CPU methods are the reference's; this "
+                                  "package runs on the GPU (device='gpu' or 'rocm')")
     if device == "gpu":
         if dev.DEFAULT_GPU["value"] == "nogpu":
             dev.activate_gpu_device("rocm")
@@ -69,14 +77,21 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
     dev.throw_gpu_unavailable("rocm")
 
     t0 = time.perf_counter()
-    H_gpu = dev.create_gpu_linearmap(hamiltonian, subspace, "rocm")
+    if method == "sparse":
+        H_gpu = dev.create_gpu_matrix(hamiltonian, subspace, "rocm")
+        what = f"device matrix, {H_gpu.nnz} stored entries"
+    else:
+        H_gpu = dev.create_gpu_linearmap(hamiltonian, subspace, "rocm")
+        what = "device linear map"
     t1 = time.perf_counter()
-    vals, vecs, info = dev.krylov_map_solve(H_gpu, N, ishermitian=ishermitian,
-                                            **krylovkit_kwargs)
+    solve = dev.krylov_matrix_solve if method == "sparse" else dev.krylov_map_solve
+    vals, vecs, info = solve(H_gpu, N, ishermitian=ishermitian, **krylovkit_kwargs)
     t2 = time.perf_counter()
     if showtime:
-        print(f"  {t1 - t0:.6f} seconds (device linear map)")
+        print(f"  {t1 - t0:.6f} seconds ({what})")
         print(f"  {t2 - t1:.6f} seconds (device Krylov solve, {info['matvecs']} matvecs)")
+    if method == "sparse":
+        H_gpu.close()   # release_gpu_memory after the solve (MomentumED.jl:329-331)
     if len(vals) < N:
         raise RuntimeError(f"Krylov method fails. Cannot find {N} eigenvectors.")
     if info["converged"] < min(N, dim):

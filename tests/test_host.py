@@ -173,14 +173,16 @@ def test_edsolve_argument_errors():
     with pytest.raises(AssertionError):
         me.EDsolve(sp, op, N=0)
     with pytest.raises(NotImplementedError):
-        me.EDsolve(sp, op, N=1, method="sparse")
+        me.EDsolve(sp, op, N=1, method="dense")
     with pytest.raises(NotImplementedError):
         me.EDsolve(sp, op, N=1, device="cpu", method_info=False)
+    with pytest.raises(NotImplementedError):
+        me.EDsolve(sp, op, N=1, method="sparse", device="cpu")
     non_upper = MBOperator([Scatter(1.0, 1, 2, bits=2)], upper_hermitian=False)
     with pytest.raises(AssertionError):
         me.EDsolve(sp, non_upper, N=1, method_info=False)
     with pytest.raises(ValueError):
-        me.EDsolve(sp, [Scatter(1.0, 1, 1, bits=2)], N=1)
+        me.EDsolve(sp, [Scatter(1.0, 1, 1, bits=2)], N=1, method="map")
 
 
 # ------------------------------------------------ the C-ABI library
