@@ -165,6 +165,25 @@ int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev
                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
                  const double *vec0, double *vals, double *vecs, edm_solve_info *info);
 
+/* ----------------------------------------------------------------- eigenvector consumers */
+/* y = O x between two bases: y_i = sum_j <bra_i|O|ket_j> x_j (y has bra's length, x ket's),
+ * the product inside ED_bracket(bra, op, ket) (EDCore/src/multiplication.jl:153-307) when the
+ * two vectors live in different subspaces (e.g. an operator that moves momentum). With
+ * bra == ket it is edm_matvec. One- and two-body number-conserving terms. */
+int edm_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
+                       const edm_basis *ket, const void *x_dev, void *y_dev);
+/* PES_1rdm (MomentumED/src/analysis/particle_reduced_density_matrix.jl:4-33): for each of
+ * npairs (i, j) (0-based orbitals, pairs[2p], pairs[2p+1]) out[2p..2p+1] = <psi|c_i^+ c_j|psi>
+ * for the device vector psi in basis b. On a complete sector, pairs that do not conserve it
+ * are exactly 0 and are not computed. */
+int edm_rdm1(edm_ctx *ctx, const edm_basis *b, const void *psi_dev, int32_t npairs,
+             const int32_t *pairs, double *out);
+/* ED_step_inner_prod (MomentumED/src/analysis/manybody_quantum_geometry.jl:43-91): for g
+ * vectors each (device pointers vf_dev[a], vi_dev[b] in basis b) and orbital inner products
+ * u[nbits] (host, complex), out[a*g+b] = sum_x conj(f_a[x]) i_b[x] prod_{k occupied in x} u[k]. */
+int edm_step_inner_prod(edm_ctx *ctx, const edm_basis *b, int32_t g, const void *const *vf_dev,
+                        const void *const *vi_dev, const double *orbital_ip, double *out);
+
 /* ----------------------------------------------------------------- stored matrix (:sparse) */
 /* SparseHmltMatrix + create_gpu_matrix (MomentumED/src/method/sparse_matrix.jl:65-191;
  * MomentumEDCUDAExt.jl:147-152): the operator's matrix in the basis, built on the device and

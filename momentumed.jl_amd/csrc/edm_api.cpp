@@ -558,6 +558,125 @@ int edm_csr_destroy(edm_csr *m)
     });
 }
 
+int edm_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
+                       const edm_basis *ket, const void *x_dev, void *y_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && bra && ket, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && bra->ctx == ctx && ket->ctx == ctx, EDM_EINVAL,
+                    "handles from another context");
+        EDM_REQUIRE((bra->D == 0 || y_dev) && (ket->D == 0 || x_dev), EDM_EINVAL, "NULL vector");
+        EDM_REQUIRE(x_dev != y_dev || bra->D == 0, EDM_EINVAL, "x and y must not alias");
+        launch_matvec_between(ctx, op, bra, ket, (const double2 *)x_dev, (double2 *)y_dev);
+        return EDM_OK;
+    });
+}
+
+int edm_rdm1(edm_ctx *ctx, const edm_basis *b, const void *psi_dev, int32_t npairs,
+             const int32_t *pairs, double *out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && b && out && (npairs == 0 || pairs), EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(b->ctx == ctx, EDM_EINVAL, "basis from another context");
+        EDM_REQUIRE(b->D == 0 || psi_dev, EDM_EINVAL, "NULL vector");
+        EDM_REQUIRE(npairs >= 0 && npairs <= 65535, EDM_EINVAL, "npairs must be 0..65535");
+        std::fill(out, out + 2 * (size_t)npairs, 0.0);
+        if (b->D == 0 || npairs == 0) return EDM_OK;
+        // a complete sector's rank tables hold every target of a sector-conserving c_i^+ c_j;
+        // every target of any other pair lies outside the sector (entry exactly 0)
+        const bool lin = b->rank_mode == 1 && b->has_spec;
+        std::vector<int2> use;
+        std::vector<int> where;
+        for (int p = 0; p < npairs; ++p) {
+            int i = pairs[2 * p], j = pairs[2 * p + 1];
+            EDM_REQUIRE(i >= 0 && j >= 0 && i < b->nbits && j < b->nbits, EDM_EINVAL,
+                        "orbital index outside the basis bits");
+            if (lin && i != j) {
+                const SystemSpec &sp = b->spec;
+                bool keep = sp.comp_of_bit(i) == sp.comp_of_bit(j);
+                for (int d = 0; keep && d < sp.dim; ++d) {
+                    int64_t dk = sp.orb_k(i, d) - sp.orb_k(j, d);
+                    keep = sp.gk[d] != 0 ? dk % sp.gk[d] == 0 : dk == 0;
+                }
+                if (!keep) continue;
+            }
+            use.push_back(make_int2(i, j));
+            where.push_back(p);
+        }
+        if (use.empty()) return EDM_OK;
+        const int n = (int)use.size();
+        size_t pb = n * sizeof(int2), ob = n * sizeof(double2);
+        int2 *d_pairs = (int2 *)ctx->alloc(pb);
+        double2 *d_out = (double2 *)ctx->alloc(ob);
+        std::vector<double2> h(n);
+        try {
+            EDM_HIP_CHECK(hipMemcpyAsync(d_pairs, use.data(), pb, hipMemcpyHostToDevice, ctx->stream));
+            launch_rdm1(ctx, b, (const double2 *)psi_dev, d_pairs, n, lin, d_out);
+            EDM_HIP_CHECK(hipMemcpyAsync(h.data(), d_out, ob, hipMemcpyDeviceToHost, ctx->stream));
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->free(d_pairs, pb);
+            ctx->free(d_out, ob);
+            throw;
+        }
+        ctx->free(d_pairs, pb);
+        ctx->free(d_out, ob);
+        for (int q = 0; q < n; ++q) {
+            out[2 * where[q]] = h[q].x;
+            out[2 * where[q] + 1] = h[q].y;
+        }
+        return EDM_OK;
+    });
+}
+
+int edm_step_inner_prod(edm_ctx *ctx, const edm_basis *b, int32_t g, const void *const *vf_dev,
+                        const void *const *vi_dev, const double *orbital_ip, double *out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && b && vf_dev && vi_dev && orbital_ip && out, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(b->ctx == ctx, EDM_EINVAL, "basis from another context");
+        EDM_REQUIRE(g >= 1 && g <= 64, EDM_EINVAL, "g must be 1..64");
+        std::fill(out, out + 2 * (size_t)g * g, 0.0);
+        if (b->D == 0) return EDM_OK;
+        size_t ub = std::max(b->nbits, 1) * sizeof(double2), tb = 2 * (size_t)g * sizeof(void *);
+        size_t ob = (size_t)g * g * sizeof(double2);
+        double2 *d_u = (double2 *)ctx->alloc(ub);
+        const double2 **d_tab = (const double2 **)ctx->alloc(tb);
+        double2 *d_out = (double2 *)ctx->alloc(ob);
+        std::vector<const void *> tab(2 * (size_t)g);
+        for (int a = 0; a < g; ++a) {
+            EDM_REQUIRE(vf_dev[a] && vi_dev[a], EDM_EINVAL, "NULL vector");
+            tab[a] = vf_dev[a];
+            tab[g + a] = vi_dev[a];
+        }
+        std::vector<double2> h((size_t)g * g);
+        try {
+            if (b->nbits > 0)
+                EDM_HIP_CHECK(hipMemcpyAsync(d_u, orbital_ip, b->nbits * sizeof(double2),
+                                             hipMemcpyHostToDevice, ctx->stream));
+            EDM_HIP_CHECK(hipMemcpyAsync(d_tab, tab.data(), tb, hipMemcpyHostToDevice, ctx->stream));
+            launch_step_gram(ctx, b, d_u, d_tab, d_tab + g, g, d_out);
+            EDM_HIP_CHECK(hipMemcpyAsync(h.data(), d_out, ob, hipMemcpyDeviceToHost, ctx->stream));
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->free(d_u, ub);
+            ctx->free(d_tab, tb);
+            ctx->free(d_out, ob);
+            throw;
+        }
+        ctx->free(d_u, ub);
+        ctx->free(d_tab, tb);
+        ctx->free(d_out, ob);
+        for (size_t q = 0; q < h.size(); ++q) {
+            out[2 * q] = h[q].x;
+            out[2 * q + 1] = h[q].y;
+        }
+        return EDM_OK;
+    });
+}
+
 int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z)
 {
     return guarded(nullptr, [&]() -> int {

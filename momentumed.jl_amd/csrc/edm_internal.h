@@ -217,6 +217,13 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
 // contribution, written at rowptr[r] (edm_matvec.hip)
 void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const int64_t *d_rowptr,
                      uint64_t *d_keys);
+// eigenvector consumers (edm_matvec.hip)
+void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
+                           const edm_basis *ket, const double2 *x, double2 *y);
+void launch_rdm1(edm_ctx *ctx, const edm_basis *b, const double2 *psi, const int2 *d_pairs,
+                 int npairs, bool lin, double2 *d_out);
+void launch_step_gram(edm_ctx *ctx, const edm_basis *b, const double2 *d_u,
+                      const double2 *const *d_vf, const double2 *const *d_vi, int g, double2 *d_out);
 struct edm_csr {
     edm_ctx *ctx = nullptr;
     int64_t D = 0, nnz = 0;

@@ -671,6 +671,137 @@ k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ bl
     P.y[i] = v;
 }
 
+// ---------------------------------------------------------------- eigenvector consumers
+// <bra basis | O | ket basis>: rows are bra states, partners are looked up in the ket basis
+// (binary search: the bases may be different sectors); a diagonal term contributes where the
+// row's own state is in the ket basis. y has the bra's length, x the ket's.
+template <class Rank>
+__global__ void __launch_bounds__(kRowBlock)
+k_matvec_rect(int64_t Dbra, const uint64_t *__restrict__ bra, OpDev op, Rank ket,
+              const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Dbra) return;
+    const uint64_t s = bra[i];
+    const int64_t self = ket(s);
+    double yr = 0.0, yi = 0.0;
+    visit_row(
+        s, op, ket,
+        [&](uint32_t e, int64_t j, int neg) {
+            double2 a = op.tamp[e];
+            double2 xv = x[j];
+            double pr = a.x * xv.x - a.y * xv.y;
+            double pi = a.x * xv.y + a.y * xv.x;
+            if (neg) { yr -= pr; yi -= pi; } else { yr += pr; yi += pi; }
+        },
+        [&](uint32_t e) {
+            if (self < 0) return;
+            double2 a = op.tamp[e];
+            double2 xv = x[self];
+            yr += a.x * xv.x - a.y * xv.y;
+            yi += a.x * xv.y + a.y * xv.x;
+        });
+    y[i] = make_double2(yr, yi);
+}
+
+constexpr int kRedBlock = 256;
+
+// block-wide sum of (a, b) in a fixed order; valid in thread 0
+__device__ __forceinline__ double2 block_sum_fixed(double a, double b, double2 *sh)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        b += __shfl_xor(b, off, 64);
+    }
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = make_double2(a, b);
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { r.x += sh[w].x; r.y += sh[w].y; }
+    return r;
+}
+
+// One-body reduced density matrix entries rho_ij = <psi| c_i^dagger c_j |psi> (orbitals
+// 0-based), one (pair, chunk of rows) per workgroup; partial[pair * nchunks + chunk].
+// Sign: (-1)^(occupied orbitals above i + above j in the state with j emptied), the rule of
+// PES_1rdm (particle_reduced_density_matrix.jl:4-33).
+template <class Rank>
+__global__ void __launch_bounds__(kRedBlock)
+k_rdm1(int64_t D, const uint64_t *__restrict__ states, Rank rank, const double2 *__restrict__ psi,
+       const int2 *__restrict__ pairs, int64_t rows_per_chunk, double2 *__restrict__ partial)
+{
+    __shared__ double2 sh[kRedBlock / 64];
+    const int2 pr = pairs[blockIdx.y];
+    const int oi = pr.x, oj = pr.y;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = r0 + rows_per_chunk < D ? r0 + rows_per_chunk : D;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t x = r0 + threadIdx.x; x < r1; x += kRedBlock) {
+        const uint64_t s = states[x];
+        if (!((s >> oj) & 1ULL)) continue;
+        const uint64_t mid = s & ~(1ULL << oj);
+        if (oi != oj && ((mid >> oi) & 1ULL)) continue;
+        int64_t xo = x;
+        if (oi != oj) {
+            xo = rank(mid | (1ULL << oi));
+            if (xo < 0) continue;
+        }
+        const int neg = __popcll(mid & (pm_one(oi) ^ pm_one(oj))) & 1;
+        const double2 a = psi[xo], b = psi[x];
+        double pr_ = a.x * b.x + a.y * b.y;    // conj(a) * b
+        double pi_ = a.x * b.y - a.y * b.x;
+        if (neg) { ar -= pr_; ai -= pi_; } else { ar += pr_; ai += pi_; }
+    }
+    double2 t = block_sum_fixed(ar, ai, sh);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// sum_x conj(f_a[x]) i_b[x] prod_{occupied k of state x, ascending} u[k] for pair (a, b) =
+// blockIdx.y (a * g + b), rows in chunks (ED_step_inner_prod,
+// manybody_quantum_geometry.jl:43-91).
+__global__ void __launch_bounds__(kRedBlock)
+k_step_gram(int64_t D, const uint64_t *__restrict__ states, const double2 *__restrict__ u,
+            const double2 *const *__restrict__ vf, const double2 *const *__restrict__ vi, int g,
+            int64_t rows_per_chunk, double2 *__restrict__ partial)
+{
+    __shared__ double2 sh[kRedBlock / 64];
+    const int a = blockIdx.y / g, bb = blockIdx.y % g;
+    const double2 *f = vf[a], *iv = vi[bb];
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = r0 + rows_per_chunk < D ? r0 + rows_per_chunk : D;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t x = r0 + threadIdx.x; x < r1; x += kRedBlock) {
+        double wr = 1.0, wi = 0.0;
+        for (uint64_t s = states[x]; s; s &= s - 1) {
+            double2 uk = u[__builtin_ctzll(s)];
+            double tr = wr * uk.x - wi * uk.y;
+            wi = wr * uk.y + wi * uk.x;
+            wr = tr;
+        }
+        const double2 fv = f[x], iv_ = iv[x];
+        double cr = fv.x * iv_.x + fv.y * iv_.y;   // conj(f) * i
+        double ci = fv.x * iv_.y - fv.y * iv_.x;
+        ar += cr * wr - ci * wi;
+        ai += cr * wi + ci * wr;
+    }
+    double2 t = block_sum_fixed(ar, ai, sh);
+    if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+}
+
+// out[p] = sum over chunks of partial[p * nchunks + c], in chunk order
+__global__ void __launch_bounds__(kRedBlock)
+k_sum_chunks(const double2 *__restrict__ partial, int64_t nchunks, double2 *__restrict__ out)
+{
+    __shared__ double2 sh[kRedBlock / 64];
+    const double2 *p = partial + (int64_t)blockIdx.x * nchunks;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t c = threadIdx.x; c < nchunks; c += kRedBlock) { ar += p[c].x; ai += p[c].y; }
+    double2 t = block_sum_fixed(ar, ai, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
 inline unsigned nblk(int64_t n) { return (unsigned)((n + kRowBlock - 1) / kRowBlock); }
 
 OpDev opdev(const edm_op *op, const OpTable &t)
@@ -940,6 +1071,67 @@ void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const i
     else
         hipLaunchKernelGGL(k_csr_keys<RankSearch>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
                            ctx->stream, b->D, b->d_states, od, make_search(b), d_rowptr, d_keys);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
+                           const edm_basis *ket, const double2 *x, double2 *y)
+{
+    EDM_REQUIRE(op->nbits == bra->nbits && op->nbits == ket->nbits, EDM_EINVAL,
+                "operator and bases have different bits");
+    if (bra == ket) {
+        launch_matvec(ctx, op, bra, x, y, ApplyMode::Forward);
+        return;
+    }
+    EDM_REQUIRE(op->fast, EDM_EINVAL,
+                "matrix elements between two bases support one- and two-body number-conserving "
+                "operators");
+    if (bra->D == 0) return;
+    hipLaunchKernelGGL(k_matvec_rect<RankSearch>, dim3(nblk(bra->D)), dim3(kRowBlock), 0,
+                       ctx->stream, bra->D, bra->d_states, opdev(op, op->fwd), make_search(ket),
+                       x, y);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+static int64_t chunk_rows(int64_t D, int64_t nitems)
+{
+    // about 2048 workgroups in total, at least one row block per chunk
+    int64_t want = std::max<int64_t>(1, 2048 / std::max<int64_t>(nitems, 1));
+    int64_t rows = (D + want - 1) / want;
+    return std::max<int64_t>(rows, kRedBlock);
+}
+
+void launch_rdm1(edm_ctx *ctx, const edm_basis *b, const double2 *psi, const int2 *d_pairs,
+                 int npairs, bool lin, double2 *d_out)
+{
+    if (npairs == 0) return;
+    const int64_t D = b->D;
+    const int64_t rows = chunk_rows(D, npairs);
+    const int64_t nch = std::max<int64_t>(1, (D + rows - 1) / rows);
+    double2 *part = (double2 *)ctx->partials((size_t)nch * npairs * sizeof(double2));
+    dim3 grid((unsigned)nch, (unsigned)npairs);
+    if (lin)
+        hipLaunchKernelGGL(k_rdm1<RankLin>, grid, dim3(kRedBlock), 0, ctx->stream, D, b->d_states,
+                           make_lin(b), psi, d_pairs, rows, part);
+    else
+        hipLaunchKernelGGL(k_rdm1<RankSearch>, grid, dim3(kRedBlock), 0, ctx->stream, D,
+                           b->d_states, make_search(b), psi, d_pairs, rows, part);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)npairs), dim3(kRedBlock), 0, ctx->stream, part,
+                       nch, d_out);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_step_gram(edm_ctx *ctx, const edm_basis *b, const double2 *d_u,
+                      const double2 *const *d_vf, const double2 *const *d_vi, int g, double2 *d_out)
+{
+    const int64_t D = b->D;
+    const int64_t rows = chunk_rows(D, (int64_t)g * g);
+    const int64_t nch = std::max<int64_t>(1, (D + rows - 1) / rows);
+    double2 *part = (double2 *)ctx->partials((size_t)nch * g * g * sizeof(double2));
+    hipLaunchKernelGGL(k_step_gram, dim3((unsigned)nch, (unsigned)(g * g)), dim3(kRedBlock), 0,
+                       ctx->stream, D, b->d_states, d_u, d_vf, d_vi, g, rows, part);
+    hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)(g * g)), dim3(kRedBlock), 0, ctx->stream,
+                       part, nch, d_out);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

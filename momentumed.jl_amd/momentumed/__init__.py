@@ -18,6 +18,8 @@ from .preparation import (EDPara, ED_momentum_subspaces, ED_scatterlist_onebody,
                           ED_sortedScatterList_twobody, MBS_totalmomentum, momentum_residue)
 from .solve import EDsolve
 from .batch import BatchSolver, EDsolve_batch
+from .analysis import (ED_bracket, ED_bracket_threaded, ED_connection_gaugefixing,
+                       ED_step_inner_prod, PES_1rdm)
 
 __all__ = [
     "EDPara", "ED_momentum_subspaces", "ED_scatterlist_onebody", "ED_scatterlist_twobody",

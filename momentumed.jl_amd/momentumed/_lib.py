@@ -73,6 +73,9 @@ _SIGS = [
     ("edm_batch_count", c_i32, [c_vp, P(c_i32), P(c_i32)]),
     ("edm_batch_set_quantum", c_i32, [c_vp, c_i64]),
     ("edm_batch_problem_bytes", c_i32, [c_i64, c_i32, c_i32, P(c_u64)]),
+    ("edm_matvec_between", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("edm_rdm1", c_i32, [c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    ("edm_step_inner_prod", c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
     ("edm_csr_build", c_i32, [c_vp, c_vp, c_vp, P(c_vp)]),
     ("edm_csr_size", c_i32, [c_vp, P(c_i64), P(c_i64)]),
     ("edm_csr_download", c_i32, [c_vp, c_vp, c_vp, c_vp]),

@@ -1,0 +1,130 @@
+"""Eigenvector consumers on the device against the oracle and closed forms.
+
+Bars: brackets within 1e-12 of max scale of the oracle's dense matrices (between two
+different sectors too); <psi|H|psi> of an eigenvector = its energy within 1e-10; the one-body
+density matrix of a basis state = its occupations exactly, Hermitian with trace N for any
+normalised state, entry by entry equal to the oracle's <psi|c_i^+ c_j|psi> within 1e-13;
+step inner products equal vdot for unit orbital overlaps and the phase of the summed angles
+for unimodular ones.
+"""
+import numpy as np
+import pytest
+
+import momentumed as me
+from momentumed import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_full(O, op):
+    amp, out, inn = op.arrays()
+    return O.full_list(O.TermList(amp, out, inn), op.upper_hermitian)
+
+
+def _rand(n, seed):
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=n) + 1j * rng.normal(size=n)
+    return v / np.linalg.norm(v)
+
+
+def test_bracket_same_space(gpu_ctx, oracle_mod):
+    w = W.small()
+    op = w.operator()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][1]
+    Hd = oracle_mod.dense_matrix(sp.list, _oracle_full(oracle_mod, op))
+    a, b = _rand(sp.length, 1), _rand(sp.length, 2)
+    got = me.ED_bracket(me.MBS64Vector(a, sp), op, me.MBS64Vector(b, sp))
+    ref = np.vdot(a, Hd @ b)
+    assert abs(got - ref) <= 1e-12 * np.abs(Hd).max() * sp.length
+    E, V = me.EDsolve(sp, op, N=3, method="map", method_info=False)
+    for e, v in zip(E, V):
+        assert abs(me.ED_bracket(v, op, v) - e) <= 1e-10 * max(1.0, abs(e))
+
+
+def test_bracket_between_sectors(gpu_ctx, oracle_mod):
+    """A one-body operator that moves momentum, between two different sectors: against the
+    oracle's dense matrix on the union of the two bases."""
+    w = W.small()
+    subs, ks = me.ED_momentum_subspaces(w.para, 5)
+    # c_a^+ c_b with k_a - k_b = (1, 0): maps sector K to K + (1, 0)
+    terms = []
+    for b in range(1, 16):
+        kb = ((b - 1) % 3, (b - 1) // 3)
+        ka = ((kb[0] + 1) % 3, kb[1])
+        a = ka[0] + 3 * ka[1] + 1
+        terms.append(me.Scatter(complex(0.3 * b, -0.1 * b), a, b, bits=15))
+    op = me.MBOperator(terms, upper_hermitian=False)
+    ket_sp = subs[ks.index((0, 0))]
+    bra_sp = subs[ks.index((1, 0))]
+    union = np.union1d(bra_sp.list, ket_sp.list).astype(np.uint64)
+    Hd = oracle_mod.dense_matrix(union, _oracle_full(oracle_mod, op))
+    ri = np.searchsorted(union, bra_sp.list)
+    ci = np.searchsorted(union, ket_sp.list)
+    block = Hd[np.ix_(ri, ci)]
+    assert np.abs(block).max() > 0
+    a, b = _rand(bra_sp.length, 3), _rand(ket_sp.length, 4)
+    got = me.ED_bracket(me.MBS64Vector(a, bra_sp), op, me.MBS64Vector(b, ket_sp))
+    ref = np.vdot(a, block @ b)
+    assert abs(got - ref) <= 1e-12 * np.abs(block).max() * union.size
+    # the wrong target sector gives exactly zero
+    other = subs[ks.index((2, 0))]
+    assert me.ED_bracket(me.MBS64Vector(_rand(other.length, 5), other), op,
+                         me.MBS64Vector(b, ket_sp)) == 0
+
+
+def test_rdm1_closed_forms_and_oracle(gpu_ctx, oracle_mod):
+    w = W.small()
+    op = w.operator()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][0]
+    bits = sp.bits
+    # a basis state: occupations on the diagonal, nothing else
+    e = np.zeros(sp.length, dtype=complex)
+    e[17] = 1.0
+    rho = me.PES_1rdm(me.MBS64Vector(e, sp))
+    occ = [(int(sp.list[17]) >> k) & 1 for k in range(bits)]
+    np.testing.assert_array_equal(rho, np.diag(np.array(occ, dtype=complex)))
+    # an eigenvector: Hermitian, trace N, block-diagonal in momentum, equal to the oracle
+    E, V = me.EDsolve(sp, op, N=1, method="map", method_info=False)
+    psi = V[0].vec
+    rho = me.PES_1rdm(V[0])
+    np.testing.assert_allclose(rho, rho.conj().T, atol=1e-14)
+    assert abs(np.trace(rho) - 5) < 1e-12
+    for i in range(bits):
+        for j in range(bits):
+            one = oracle_mod.TermList(np.array([1.0 + 0j]), np.array([1 << i], dtype=np.uint64),
+                                      np.array([1 << j], dtype=np.uint64))
+            full = oracle_mod.full_list(one, False)
+            ref = np.vdot(psi, oracle_mod.matvec(sp.list, full, psi))
+            assert abs(rho[i, j] - ref) <= 1e-13, (i, j)
+
+
+def test_rdm1_two_component_sector(gpu_ctx, oracle_mod):
+    para = W.batched_para(1.2)
+    op = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    sp = me.ED_momentum_subspaces(para, (2, 3))[0][3]
+    E, V = me.EDsolve(sp, op, N=1, method="map", method_info=False)
+    psi = V[0].vec
+    rho = me.PES_1rdm(V[0])
+    for (i, j) in [(0, 0), (5, 5), (13, 13), (2, 14), (14, 2), (1, 1 + 12), (23, 11)]:
+        one = oracle_mod.TermList(np.array([1.0 + 0j]), np.array([1 << i], dtype=np.uint64),
+                                  np.array([1 << j], dtype=np.uint64))
+        ref = np.vdot(psi, oracle_mod.matvec(sp.list, oracle_mod.full_list(one, False), psi))
+        assert abs(rho[i, j] - ref) <= 1e-13, (i, j)
+    assert abs(np.trace(rho) - 5) < 1e-12
+
+
+def test_step_inner_prod(gpu_ctx):
+    w = W.small()
+    op = w.operator()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][2]
+    E, V = me.EDsolve(sp, op, N=3, method="map", method_info=False)
+    ones = np.ones(sp.bits, dtype=complex)
+    got = me.ED_step_inner_prod(V[0], V[1], ones)
+    assert abs(got - np.vdot(V[0].vec, V[1].vec)) < 1e-14
+    theta = np.linspace(0.1, 1.7, sp.bits)
+    u = np.exp(1j * theta)
+    m = me.ED_step_inner_prod(V, V, u)
+    phase = np.array([np.exp(1j * sum(theta[k] for k in range(sp.bits) if (int(s) >> k) & 1))
+                      for s in sp.list])
+    ref = np.array([[np.vdot(a.vec, b.vec * phase) for b in V] for a in V])
+    np.testing.assert_allclose(m, ref, atol=1e-13)
