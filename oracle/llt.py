@@ -8,8 +8,9 @@ level_form_factor(0, 0, ql) (:157-173) is exp(-|ql|^2/4) exactly (laguerrel(0, 0
 no third-party special functions are needed.
 
 Used to reproduce numbers the reference's notebooks print: the 1734 upper two-body terms of the
-4x6, Ne=8 triangular Coulomb system ("4. cuda support.ipynb") and the lowest energies of the
-3x5, Ne=5 sector (0,0) ("1.FQH_Laughlin.ipynb").
+4x6, Ne=8 triangular Coulomb system ("4. cuda support.ipynb"), the lowest energies of the
+3x5, Ne=5 sector (0,0) ("1.FQH_Laughlin.ipynb"), and the bilayer Halperin energies, term count
+and many-body Chern numbers ("2.BilayerFQH_Halperin.ipynb"; orbital_overlap, :352-393).
 """
 from __future__ import annotations
 
@@ -167,3 +168,33 @@ class LandauInteraction:
                 sign = _ita(g1 + gs1[0], g2 + gs1[1]) * _ita(g1 + gs2[0], g2 + gs2[1])
                 total += sign * V * F * phase
         return total
+
+
+def orbital_overlap(k_list, Gk, V: LandauInteraction, shift_f, shift_i):
+    """Landau level torus.jl:352-393: <u_{k+shift_f} | u_{k+shift_i}> for every orbital
+    (k fastest, then component), for the level-0 components of V. shift_f / shift_i: one
+    (s1, s2) for all components or one per component. The form factor is the reference's
+    one-argument level_form_factor (:174-181), exp(-1/4) L_n(|ql|^2 / 2) as written there
+    (a real constant for n = 0)."""
+    nc = len(V.layer)
+    nk = len(k_list[0])
+    if not isinstance(shift_f, list):
+        shift_f = [shift_f] * nc
+    if not isinstance(shift_i, list):
+        shift_i = [shift_i] * nc
+    G1, G2 = V.lattice.G1, V.lattice.G2
+    out = []
+    for c in range(nc):
+        sf = tuple(Fraction(v) for v in shift_f[c])
+        si = tuple(Fraction(v) for v in shift_i[c])
+        q = ((sf[0] - si[0]) / Gk[0], (sf[1] - si[1]) / Gk[1])
+        ql = (float(q[0]) * G1 + float(q[1]) * G2) * L_MAG
+        F = math.exp(-0.25) * _laguerre0(0, 0.5 * abs(ql) ** 2)
+        for k in range(nk):
+            kf = ((k_list[0][k] + sf[0]) / Gk[0], (k_list[1][k] + sf[1]) / Gk[1])
+            ki = ((k_list[0][k] + si[0]) / Gk[0], (k_list[1][k] + si[1]) / Gk[1])
+            ang = kf[0] * ki[1] - kf[1] * ki[0]
+            s = (ki[0] + kf[0], ki[1] + kf[1])
+            ang += s[0] * q[1] - s[1] * q[0]
+            out.append(F * cmath.exp(1j * math.pi * float(ang)))
+    return out

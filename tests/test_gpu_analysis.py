@@ -128,3 +128,61 @@ def test_step_inner_prod(gpu_ctx):
                       for s in sp.list])
     ref = np.array([[np.vdot(a.vec, b.vec * phase) for b in V] for a in V])
     np.testing.assert_allclose(m, ref, atol=1e-13)
+
+
+def _bilayer_para(g, tunneling=False):
+    import llt
+    V = llt.LandauInteraction(llt.ReciprocalLattice.named(g["lattice"]),
+                              *[tuple(c) for c in g["components"]])
+    V.D_l, V.d_l, V.mix = g["D_l"], g["d_l"], g["mix"]
+    V.V_intra, V.V_inter = list(g["V_intra"]), list(g["V_inter"])
+    k = np.array(g["k_list"], dtype=np.int64)
+    if not tunneling:
+        return me.EDPara(k_list=k, Gk=tuple(g["Gk"]), Nc_mix=1, Nc_conserve=2, H_two=V), V
+
+    def tunnel(kk, cf, ci, t_W0=0.0):      # the notebook's tunneling(k, cf, ci; t_W0)
+        return t_W0 if cf != ci else 0.0
+    return me.EDPara(k_list=k, Gk=tuple(g["Gk"]), Nc_mix=2, Nc_conserve=1, H_one=tunnel, H_two=V,
+                     one_momentum_coordinate=False), V
+
+
+@pytest.mark.parametrize("tunneling", [False, True])
+def test_bilayer_halperin_energies_and_chern_number(gpu_ctx, tunneling):
+    """End to end against 2.BilayerFQH_Halperin.ipynb: the ten lowest energies of sector (0,0)
+    (device stored-matrix EDsolve) and the many-body Chern number of the two-fold ground
+    space from a kshift plaquette (the three shifted Hamiltonians solved in one batch, the
+    step overlaps on the device)."""
+    import json
+    import os
+    from fractions import Fraction
+    import llt
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "reference_printed_values.json")))["bilayer_halperin"]
+    print(f"[bilayer] tunneling={tunneling}", flush=True)
+    para, Vint = _bilayer_para(g, tunneling)
+    N = 6 if tunneling else (3, 3)
+    subs, ks = me.ED_momentum_subspaces(para, N)
+    sp = subs[ks.index(tuple(g["sector"]))]
+    one = [me.ED_scatterlist_onebody(para, t_W0=g["t_W0"])] if tunneling else []
+    H = me.MBOperator(*one, me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    E, V = me.EDsolve(sp, H, N=10, method="sparse", method_info=False)
+    ref = g["energies_per_Nk_tunneling" if tunneling else "energies_per_Nk_no_tunneling"]
+    np.testing.assert_allclose(E / para.Nk, ref, rtol=1e-10, atol=1e-13)
+    # kshift plaquette
+    n = g["N_kshift"]
+    f = Fraction(1, n)
+    path = [(Fraction(0), Fraction(0)), (f, Fraction(0)), (f, f), (Fraction(0), f),
+            (Fraction(0), Fraction(0))]
+    ops = [me.MBOperator(*one, me.ED_scatterlist_twobody(para, kshift=path[i]),
+                         upper_hermitian=True) for i in (1, 2, 3)]
+    out = me.EDsolve_batch([(sp, o) for o in ops], N=6)
+    vn = slice(g["vn_range"][0] - 1, g["vn_range"][1])
+    psi = [V[vn]] + [vecs[vn] for _, vecs in out] + [V[vn]]
+    W = [me.ED_step_inner_prod(psi[i + 1], psi[i],
+                               llt.orbital_overlap(g["k_list"], g["Gk"], Vint, path[i + 1],
+                                                   path[i]))
+         for i in range(4)]
+    P = W[0] @ W[1] @ W[2] @ W[3]
+    chern = np.angle(np.linalg.det(P)) / (2 * np.pi) * n * n / len(psi[0])
+    ref_c = g["chern_tunneling" if tunneling else "chern_no_tunneling"]
+    assert abs(chern - ref_c) < 1e-8, (chern, ref_c)

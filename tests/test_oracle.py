@@ -200,6 +200,47 @@ def test_term_count_reference_notebook(oracle_mod):
     assert len(op.scats) == g["upper_twobody_terms"]
 
 
+def _bilayer_para(g, tunneling=False):
+    import llt
+    V = llt.LandauInteraction(llt.ReciprocalLattice.named(g["lattice"]),
+                              *[tuple(c) for c in g["components"]])
+    V.D_l, V.d_l, V.mix = g["D_l"], g["d_l"], g["mix"]
+    V.V_intra, V.V_inter = list(g["V_intra"]), list(g["V_inter"])
+    k = np.array(g["k_list"], dtype=np.int64)
+    if not tunneling:
+        return EDPara(k_list=k, Gk=tuple(g["Gk"]), Nc_mix=1, Nc_conserve=2, H_two=V)
+
+    def tunnel(kk, cf, ci, t_W0=0.0):      # the notebook's tunneling(k, cf, ci; t_W0)
+        return t_W0 if cf != ci else 0.0
+    return EDPara(k_list=k, Gk=tuple(g["Gk"]), Nc_mix=2, Nc_conserve=1, H_one=tunnel, H_two=V,
+                  one_momentum_coordinate=False)
+
+
+def test_bilayer_term_count_reference_notebook(oracle_mod):
+    g = GOLDEN["term_counts"][1]
+    para = _bilayer_para(GOLDEN["bilayer_halperin"])
+    op = MBOperator(ED_scatterlist_twobody(para), upper_hermitian=True)
+    assert len(op.scats) == g["upper_twobody_terms"]
+
+
+def test_orbital_overlap_closed_forms():
+    """The restated orbital overlap: equal shifts give exp(-1/4) on every orbital (q = 0, no
+    phase); the phases of a shift and its reverse are conjugate; |overlap| = exp(-1/4)."""
+    import llt
+    from fractions import Fraction
+    g = GOLDEN["bilayer_halperin"]
+    V = llt.LandauInteraction(llt.ReciprocalLattice.named(g["lattice"]),
+                              *[tuple(c) for c in g["components"]])
+    kl, Gk = g["k_list"], g["Gk"]
+    same = llt.orbital_overlap(kl, Gk, V, (Fraction(1, 10), 0), (Fraction(1, 10), 0))
+    np.testing.assert_allclose(same, np.exp(-0.25), rtol=0, atol=1e-15)
+    a = np.array(llt.orbital_overlap(kl, Gk, V, (Fraction(1, 10), Fraction(1, 10)), (0, 0)))
+    b = np.array(llt.orbital_overlap(kl, Gk, V, (0, 0), (Fraction(1, 10), Fraction(1, 10))))
+    np.testing.assert_allclose(a, np.conj(b), atol=1e-15)
+    np.testing.assert_allclose(np.abs(a), np.exp(-0.25), atol=1e-15)
+    assert len(a) == 24
+
+
 def test_laughlin_energies_reference_notebook(oracle_mod):
     g = GOLDEN["energies_per_Nk"][0]
     para = _llt_para(g["mesh"])
