@@ -34,7 +34,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def _kernel_name(rank_mode: int) -> str:
+def _kernel_name(rank_mode: int, block: int = 512) -> str:
     """Name of the matvec kernel libedm.so launches for the bucketed path (EDM_MATVEC)."""
     rank = "RankLin" if rank_mode == 1 else "RankSearch"
     v = os.environ.get("EDM_MATVEC", "u1")
@@ -43,7 +43,7 @@ def _kernel_name(rank_mode: int) -> str:
     if v in ("hi", "lo"):
         return f"k_matvec_lds<{rank}, {str(v == 'hi').lower()}>"
     u = {"u2": 2, "u4": 4}.get(v, 1)
-    return f"k_matvec_ldsu<{rank}, true, {u}>"
+    return f"k_matvec_ldsu<{rank}, true, {u}, {block if u == 1 else 512}>"
 
 
 def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
@@ -186,6 +186,8 @@ def main():
     ap.add_argument("--solve", action="store_true", help="also time the lowest-nev eigensolve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default 512-row workgroups (skip the autotuner)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-ratios", type=int, default=64)
@@ -241,6 +243,12 @@ def main():
     x /= torch.linalg.vector_norm(x)
     torch.cuda.synchronize()
     lm = dev.ROCmLinearMap(dop, basis)
+    if args.no_tune:
+        block, tune_ms = 512, None
+    else:
+        block, tune_ms = lm.tune()
+        log(f"[bench] tuner: {block}-row workgroups ({tune_ms} ms per matvec)")
+        block = block or 512
 
     stream = torch.cuda.ExternalStream(ctx.stream, device="cuda:0")
     for _ in range(args.warmup):
@@ -273,7 +281,7 @@ def main():
     achieved = B_mv / (ms_per * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": _kernel_name(basis.rank_mode),
+                "kernel": _kernel_name(basis.rank_mode, block),
                 "bytes_model": "D*40 + nnz_off*16 + S_full*40", "algorithmic_bytes": B_mv}
     tr_path = os.path.join(ROOT, "profiles", f"matvec_{args.workload}_traffic.json")
     if os.path.exists(tr_path):
@@ -288,7 +296,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "mesh": [int(g) for g in w.para.Gk], "N": list(w.N_each),
                    "sector": list(w.sector), "D": D, "S_full": S, "nnz_offdiag": nnz_off.value,
-                   "parallelism": "single-gpu"},
+                   "parallelism": "single-gpu", "matvec_workgroup": block,
+                   "tuner_ms": tune_ms},
         "roofline": roofline,
         "hbm_gbps_algorithmic": round(achieved, 1),
         "setup": {"terms_s": round(t_terms, 3), "enumerate_s": round(t_enum, 3),

@@ -153,6 +153,15 @@ int edm_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t
 int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t *nnz,
                             int64_t *nnz_offdiag);
 
+/* Autotuner (no reference counterpart; the reference has one GPU kernel configuration): times
+ * the default matvec kernel with 256-, 512- and 1024-row workgroups on this operator and
+ * basis (three launches each, on scratch vectors) and uses the fastest for every later
+ * edm_matvec / edm_eigsolve on the pair. The per-row arithmetic is the same in all three, so
+ * y does not change. *block = chosen size (0 when the pair runs another kernel); ms3 (may be
+ * NULL) = mean milliseconds per launch at 256, 512, 1024. */
+int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *block,
+                    double *ms3);
+
 /* ----------------------------------------------------------------- eigensolve (a9-a10) */
 /* krylov_map_solve(H::AbstractGPULinearMap, N; ishermitian=true, vec0, krylovdim, tol,
  * maxiter) (gpu_interface.jl:92-94; MomentumEDCUDAExt.jl:159-179 -> KrylovKit.eigsolve(...,

@@ -139,6 +139,8 @@ struct edm_op {
     edm_ctx *ctx = nullptr;
     // cached op_conserves() results keyed by basis id
     mutable std::vector<std::pair<uint64_t, bool>> conserve_cache;
+    // matvec workgroup size per basis id chosen by matvec_tune (default 512)
+    mutable std::vector<std::pair<uint64_t, int>> tuned_block;
     int nbits = 0;
     bool upper = false;
     int64_t S_full = 0;
@@ -217,6 +219,9 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
 // contribution, written at rowptr[r] (edm_matvec.hip)
 void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const int64_t *d_rowptr,
                      uint64_t *d_keys);
+// time the default matvec kernel at 256/512/1024-row workgroups on this (op, basis), keep
+// the fastest for later launches; ms3 gets the three timings; returns the chosen size or 0
+int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms3);
 // eigenvector consumers (edm_matvec.hip)
 void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
                            const edm_basis *ket, const double2 *x, double2 *y);

@@ -618,8 +618,8 @@ __device__ __forceinline__ void stage_image(const CompactDev &t, uint4 *lds)
 // window: only partners j in [jmin, jmax) are gathered, and the result is added to y when
 // `accumulate` is set. A matvec split into passes over windows of x keeps each pass's gather
 // footprint small.
-template <class Rank, bool HighFirst, int U>
-__global__ void __launch_bounds__(kLdsBlock)
+template <class Rank, bool HighFirst, int U, int BLOCK = kLdsBlock>
+__global__ void __launch_bounds__(BLOCK)
 k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
               const double2 *__restrict__ x, double2 *__restrict__ y, int64_t jmin, int64_t jmax,
               int accumulate, int with_diag)
@@ -919,12 +919,25 @@ static int matvec_passes()
 
 template <class Rank, int U>
 static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
-                        const double2 *x, double2 *y)
+                        const double2 *x, double2 *y, int block = kLdsBlock)
 {
     CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
                   (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
-    unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
     int passes = matvec_passes();
+    if (U == 1 && passes == 1 && block != kLdsBlock) {
+        // workgroup size chosen by edm_matvec_tune: the same per-row code, so the same y
+        unsigned nb = (unsigned)((b->D + block - 1) / block);
+        if (block == 256)
+            hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, 1, 256>), dim3(nb), dim3(256),
+                               ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, rank, x, y, 0,
+                               INT64_MAX, 0, 1);
+        else
+            hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, 1, 1024>), dim3(nb), dim3(1024),
+                               ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, rank, x, y, 0,
+                               INT64_MAX, 0, 1);
+        return;
+    }
+    unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
     for (int p = 0; p < passes; ++p) {
         int64_t jmin = b->D * p / passes, jmax = b->D * (p + 1) / passes;
         if (passes == 1) jmax = INT64_MAX;   // no window
@@ -936,12 +949,12 @@ static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct
 
 template <class Rank>
 static void launch_compact(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
-                           const double2 *x, double2 *y, int variant)
+                           const double2 *x, double2 *y, int variant, int block = kLdsBlock)
 {
     if (variant >= 11) {
         if (variant == 12) launch_ldsu<Rank, 2>(ctx, b, ct, rank, x, y);
         else if (variant == 14) launch_ldsu<Rank, 4>(ctx, b, ct, rank, x, y);
-        else launch_ldsu<Rank, 1>(ctx, b, ct, rank, x, y);
+        else launch_ldsu<Rank, 1>(ctx, b, ct, rank, x, y, block);
         return;
     }
     CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
@@ -965,10 +978,14 @@ void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const dou
         const CompactTable &ct = (mode == ApplyMode::Adjoint && !op->upper) ? op->cadj : op->cfwd;
         int variant = matvec_variant();
         if (ct.ok && variant != 0) {
+            int block = kLdsBlock;
+            if (mode == ApplyMode::Forward)
+                for (auto &kv : op->tuned_block)
+                    if (kv.first == b->id) block = kv.second;
             if (use_lin(op, b))
-                launch_compact(ctx, b, ct, make_lin(b), x, y, variant);
+                launch_compact(ctx, b, ct, make_lin(b), x, y, variant, block);
             else
-                launch_compact(ctx, b, ct, make_search(b), x, y, variant);
+                launch_compact(ctx, b, ct, make_search(b), x, y, variant, block);
             EDM_HIP_CHECK(hipGetLastError());
             return;
         }
@@ -1072,6 +1089,62 @@ void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const i
         hipLaunchKernelGGL(k_csr_keys<RankSearch>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
                            ctx->stream, b->D, b->d_states, od, make_search(b), d_rowptr, d_keys);
     EDM_HIP_CHECK(hipGetLastError());
+}
+
+int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms3)
+{
+    for (int k = 0; k < 3; ++k) ms3[k] = -1.0;
+    if (!op->fast || !op->cfwd.ok || matvec_variant() != 11 || matvec_passes() != 1 || b->D == 0)
+        return 0;
+    size_t vb = (size_t)b->D * sizeof(double2);
+    double2 *x = (double2 *)ctx->alloc(vb);
+    double2 *y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = kLdsBlock;
+    try {
+        y = (double2 *)ctx->alloc(vb);
+        blas_fill_random(ctx, x, b->D, 0x7E57ULL, 1);
+        EDM_HIP_CHECK(hipEventCreate(&e0));
+        EDM_HIP_CHECK(hipEventCreate(&e1));
+        const int blocks[3] = {256, 512, 1024};
+        double bestms = 1e300;
+        for (int k = 0; k < 3; ++k) {
+            auto run = [&]() {
+                if (use_lin(op, b))
+                    launch_compact(ctx, b, op->cfwd, make_lin(b), x, y, 11, blocks[k]);
+                else
+                    launch_compact(ctx, b, op->cfwd, make_search(b), x, y, 11, blocks[k]);
+            };
+            run();   // warm-up
+            EDM_HIP_CHECK(hipEventRecord(e0, ctx->stream));
+            for (int r = 0; r < 3; ++r) run();
+            EDM_HIP_CHECK(hipEventRecord(e1, ctx->stream));
+            EDM_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            EDM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            ms3[k] = ms / 3.0;
+            if (ms3[k] < bestms) {
+                bestms = ms3[k];
+                best = blocks[k];
+            }
+        }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->free(x, vb);
+        if (y) ctx->free(y, vb);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->free(x, vb);
+    ctx->free(y, vb);
+    bool found = false;
+    for (auto &kv : op->tuned_block)
+        if (kv.first == b->id) { kv.second = best; found = true; }
+    if (!found) op->tuned_block.emplace_back(b->id, best);
+    return best;
 }
 
 void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,

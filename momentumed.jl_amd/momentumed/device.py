@@ -244,6 +244,15 @@ class ROCmLinearMap:
         if len(x) != D or len(y) != D:
             raise ValueError("Dimension of Hamiltonian linear map mismatches vector length.")
 
+    def tune(self):
+        """Pick the matvec workgroup size for this operator and basis by timing (edm.h
+        edm_matvec_tune); returns (chosen rows per workgroup or 0, {size: ms per matvec})."""
+        blk = ctypes.c_int32()
+        ms = (ctypes.c_double * 3)()
+        check(self.ctx.L.edm_matvec_tune(self.ctx.h, self.op.h, self.basis.h, ctypes.byref(blk),
+                                         ms), self.ctx.h)
+        return blk.value, {256: ms[0], 512: ms[1], 1024: ms[2]}
+
     def matvec_device(self, y_ptr: int, x_ptr: int):
         """Stream-ordered y = H x on raw device pointers (no synchronisation)."""
         L = self.ctx.L

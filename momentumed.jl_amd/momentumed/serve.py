@@ -140,7 +140,7 @@ class SolveServer:
     def __init__(self, hbm_budget_bytes: int = 200 * 10 ** 9, max_queue: int = 256,
                  krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100, seed: int = 0x5EED,
                  mode: str = "batched", prepare_ahead: int = 64, device: int = 0,
-                 quantum_rows: int = 1 << 21):
+                 quantum_rows: int = 1 << 15):
         if mode not in ("batched", "plain"):
             raise ValueError("mode must be 'batched' or 'plain'")
         self.mode = mode

@@ -203,6 +203,29 @@ def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizz
     assert np.abs(_gpu_matvec(gpu_ctx, dop, b, x) - yo).max() <= Y_TOL * np.abs(yo).max()
 
 
+def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
+    """The autotuner only changes the workgroup size: y is bitwise the same before and after,
+    for whichever size it picks, and every size gives the same y."""
+    w = W.medium()
+    op = w.operator()
+    dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, w.para.bits)
+    b = dev.DeviceBasis.enumerate(gpu_ctx, _system(w.para, w.N_each), 0)
+    H = dev.ROCmLinearMap(dop, b)
+    x = _rand(b.D, 77)
+    y0 = _gpu_matvec(gpu_ctx, dop, b, x)
+    blk, ms = H.tune()
+    print(f"[tune] medium: chose {blk}, ms {ms}", flush=True)
+    assert blk in (256, 512, 1024) and all(v > 0 for v in ms.values())
+    np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b, x), y0)
+    part = dev.DeviceBasis.upload(gpu_ctx, b.download()[::5].copy(), w.para.bits)
+    Hp = dev.ROCmLinearMap(dop, part)
+    xp = _rand(part.D, 78)
+    yp0 = _gpu_matvec(gpu_ctx, dop, part, xp)
+    blk2, _ = Hp.tune()
+    assert blk2 in (256, 512, 1024)
+    np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, part, xp), yp0)
+
+
 @pytest.mark.parametrize("passes", ["2", "3"])
 def test_matvec_windowed_passes(gpu_ctx, oracle_mod, monkeypatch, passes):
     """The matvec split into passes over windows of x (accumulating into y)."""
