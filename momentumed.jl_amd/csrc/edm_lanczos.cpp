@@ -171,9 +171,9 @@ struct Workspace {
             for (int i = 0; i < ntmp; ++i) tmp.push_back((double2 *)ctx->alloc(vbytes));
             d_vptrs = (double2 **)ctx->alloc((m + 1) * sizeof(double2 *));
             d_tmpptrs = (double2 **)ctx->alloc(std::max(ntmp, 1) * sizeof(double2 *));
-            d_h = (double2 *)ctx->alloc((m + 1) * sizeof(double2));
-            d_h2 = (double2 *)ctx->alloc((m + 1) * sizeof(double2));
-            d_beta = (double *)ctx->alloc(sizeof(double) * 2);
+            d_h = (double2 *)ctx->alloc((2 * (m + 1) + 1) * sizeof(double2));
+            d_h2 = d_h + (m + 1);
+            d_beta = (double *)(d_h2 + (m + 1));
             d_Y = (double *)ctx->alloc((size_t)m * m * sizeof(double));
         } catch (...) {
             release();
@@ -198,9 +198,7 @@ struct Workspace {
         tmp.clear();
         ctx->free(d_vptrs, (m + 1) * sizeof(double2 *));
         ctx->free(d_tmpptrs, sizeof(double2 *));
-        ctx->free(d_h, (m + 1) * sizeof(double2));
-        ctx->free(d_h2, (m + 1) * sizeof(double2));
-        ctx->free(d_beta, sizeof(double) * 2);
+        ctx->free(d_h, (2 * (m + 1) + 1) * sizeof(double2));
         ctx->free(d_Y, (size_t)m * m * sizeof(double));
         d_vptrs = d_tmpptrs = nullptr;
         d_h = d_h2 = nullptr;
@@ -230,6 +228,28 @@ void cgs2(Workspace &ws, int n, double2 *target, std::vector<double2> &h_out)
         h_out[i].x += h2[i].x;
         h_out[i].y += h2[i].y;
     }
+}
+
+// CGS2 of w against V[0..n), then beta = ||w|| and w /= beta, with one device-to-host copy
+// and one synchronisation; h_out = first-pass + second-pass coefficients.
+void cgs2_norm_scale(Workspace &ws, int n, double2 *w, std::vector<double2> &h_out, double *beta)
+{
+    edm_ctx *ctx = ws.ctx;
+    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.d_h);
+    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h, w, ws.D);
+    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.d_h2);
+    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h2, w, ws.D);
+    blas_norm(ctx, w, ws.D, ws.d_beta);
+    blas_scale_inv(ctx, w, w, ws.d_beta, ws.D);
+    // h, h2 and beta are adjacent on the device (d_h2 = d_h + m + 1, d_beta follows)
+    std::vector<double2> buf(2 * (size_t)(ws.m + 1) + 1);
+    EDM_HIP_CHECK(hipMemcpyAsync(buf.data(), ws.d_h, buf.size() * sizeof(double2),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    h_out.resize(n);
+    const double2 *h2 = buf.data() + (ws.m + 1);
+    for (int i = 0; i < n; ++i) h_out[i] = make_double2(buf[i].x + h2[i].x, buf[i].y + h2[i].y);
+    *beta = *(const double *)(buf.data() + 2 * (ws.m + 1));
 }
 
 double norm_host(Workspace &ws, const double2 *v)
@@ -301,14 +321,14 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
 
     for (iters = 1; iters <= maxiter; ++iters) {
         for (int j = k; j < m; ++j) {
+            // one host round trip per step: matvec, both Gram-Schmidt passes, the norm and
+            // the scaling are queued, then h, h2 and beta come back together (a breakdown
+            // overwrites the scaled vector below)
             mv(ws.V[j], ws.V[j + 1]);
             ++matvecs;
-            cgs2(ws, j + 1, ws.V[j + 1], h);
+            double beta = 0.0;
+            cgs2_norm_scale(ws, j + 1, ws.V[j + 1], h, &beta);
             t(j, j) = h[j].x;
-            if (j > k) {
-                // beta_{j-1} already stored; nothing else to record
-            }
-            double beta = norm_host(ws, ws.V[j + 1]);
             hnorm = std::max(hnorm, std::fabs(h[j].x) + beta + (j > 0 ? std::fabs(t(j, j - 1)) : 0.0));
             bool breakdown = !(beta > 1e-14 * std::max(hnorm, 1e-300));
             if (breakdown) {
@@ -322,8 +342,8 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
                     double nb = norm_host(ws, ws.V[j + 1]);
                     if (nb > 1e-8) break;
                 }
+                blas_scale_inv(ctx, ws.V[j + 1], ws.V[j + 1], ws.d_beta, D);
             }
-            blas_scale_inv(ctx, ws.V[j + 1], ws.V[j + 1], ws.d_beta, D);
             if (j + 1 < m) {
                 t(j, j + 1) = beta;
                 t(j + 1, j) = beta;
