@@ -268,3 +268,61 @@ def test_projected_eigh_matches_lapack(kind):
         np.testing.assert_allclose(w, ref, rtol=0, atol=1e-13 * scale * A.shape[0])
         np.testing.assert_allclose(Z @ Z.T, np.eye(A.shape[0]), atol=1e-13)
         assert np.abs(A @ Z.T - Z.T * w).max() <= 1e-13 * scale * A.shape[0]
+
+
+# ------------------------------------------------ batched / served workloads (host side)
+def test_served_stream_rules():
+    from fractions import Fraction
+    from momentumed import serve as S
+    reqs = S.served_stream(rate=50.0, duration=20.0, seed=7)
+    assert reqs == S.served_stream(rate=50.0, duration=20.0, seed=7)    # seeded
+    assert 800 < len(reqs) < 1200                                       # Poisson(1000)
+    assert all(a.arrival < b.arrival for a, b in zip(reqs, reqs[1:]))
+    seen = set()
+    for r in reqs:
+        nk = r.mesh[0] * r.mesh[1]
+        assert r.mesh in S.MESHES and r.filling in (Fraction(1, 3), Fraction(2, 5))
+        assert r.N == nk * r.filling                 # redraw rule: N is always an integer
+        assert 0 <= r.sector[0] < r.mesh[0] and 0 <= r.sector[1] < r.mesh[1]
+        assert 1.0 <= r.screening < 4.0
+        seen.add(r.label)
+    # 4x5@1/3, 4x6@2/5 and 6x6@2/5 never appear; the other seven pairs do
+    assert seen == {"3x5@1/3", "3x5@2/5", "4x5@2/5", "4x6@1/3", "5x6@1/3", "5x6@2/5", "6x6@1/3"}
+
+
+def test_batched_workload_dimensions(oracle_mod):
+    """The batched sweep's sectors span the intended 1.2k-20k range (oracle enumeration)."""
+    from momentumed import workloads as W
+    r = W.batched_ratios()
+    assert len(r) == 64 and r.min() >= 0.5 and r.max() <= 2.0
+    np.testing.assert_array_equal(r, W.batched_ratios())
+    para = W.batched_para(1.0)
+    dims = []
+    for f in W.BATCH_FILLINGS:
+        lists, ks = oracle_mod.enumerate_sectors(para.k_list, para.Gk, 1, 2, list(f))
+        assert len(lists) == 12
+        dims += [len(l) for l in lists]
+    assert min(dims) == 1210 and max(dims) == 20439
+
+
+def test_problem_bytes_accounts_for_the_krylov_workspace():
+    from momentumed.batch import problem_bytes
+    for D, nev, kd in ((1000, 4, 30), (20439, 4, 30), (5, 2, 30), (34770492, 6, 40)):
+        m = min(max(kd, nev + 2), D)
+        b = problem_bytes(D, nev, kd)
+        assert b >= (2 * m + 1) * D * 16
+        assert b <= (2 * m + 1) * (D + 64) * 16 + 4 * (m + 1) * ((D + 1023) // 1024 + 2) * 16 + 65536
+    assert problem_bytes(2000, 4, 30) > problem_bytes(1000, 4, 30)
+
+
+def test_gauge_fixing_closed_forms():
+    from momentumed import ED_connection_gaugefixing
+    sp = HilbertSubspace([3, 5, 6], bits=3)
+    v = me.MBS64Vector(np.array([1j, 2.0, -1.0]), sp)
+    ED_connection_gaugefixing(v)
+    np.testing.assert_allclose(v.vec, [1.0, -2j, 1j], atol=1e-15)
+    a = me.MBS64Vector(np.array([1j, 0.0, 0.0]), sp)
+    b = me.MBS64Vector(np.array([0.0, 1.0, 0.0]), sp)
+    ED_connection_gaugefixing([a, b])                # det [[1j, 0], [0, 1]] = 1j
+    np.testing.assert_allclose(a.vec, [1.0, 0, 0], atol=1e-15)
+    np.testing.assert_allclose(b.vec, [0, 1.0, 0], atol=1e-15)
