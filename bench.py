@@ -78,6 +78,54 @@ def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
     }, err
 
 
+def block_section(ctx, dop, basis, x, y, stream, args, nnz_off, S, ms_single):
+    """y_v = H x_v for 4 vectors per edm_matvec_block call (the block-Lanczos matvec: one
+    candidate walk and one partner lookup serve all 4 vectors; x interleaved so a partner's 4
+    entries share a cache line). Timed with HIP events on the library stream like the single
+    matvec; algorithmic bytes D*(8 + 4*32) + 4*16*nnz_off + 40*S per call."""
+    import torch
+    from momentumed._lib import check
+    nv, D = 4, basis.D
+    X = [x] + [torch.empty_like(x) for _ in range(nv - 1)]
+    for v in range(1, nv):
+        check(ctx.L.edm_fill_random(ctx.h, X[v].data_ptr(), D, 12345 + v, 1), ctx.h)
+    Y = [y] + [torch.empty_like(x) for _ in range(nv - 1)]
+    work = torch.empty(nv * D, dtype=torch.complex128, device=x.device)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    xp = (ctypes.c_void_p * nv)(*[t.data_ptr() for t in X])
+    yp = (ctypes.c_void_p * nv)(*[t.data_ptr() for t in Y])
+
+    def call():
+        check(ctx.L.edm_matvec_block(ctx.h, dop.h, basis.h, nv, xp, yp, work.data_ptr()), ctx.h)
+    for _ in range(max(1, args.warmup)):
+        call()
+    ctx.synchronize()
+    steps = max(3, args.steps // 2)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        call()
+    ev1.record(stream)
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    B = D * (8 + nv * 32) + nv * 16 * nnz_off + 40 * S
+    ach = B / (ms * 1e-3) / 1e9
+    log(f"[bench] block matvec x{nv}: {ms:.3f} ms per call, {ms / nv:.3f} ms per vector")
+    del X[1:], Y[1:], work
+    torch.cuda.empty_cache()
+    return {"vectors_per_call": nv, "calls": steps, "ms_per_call": round(ms, 4),
+            "matvecs_per_s": round(nv * 1e3 / ms, 3),
+            "speedup_per_vector_vs_single": round(ms_single * nv / ms, 3),
+            "kernel": f"k_matvec_block<{'RankLin' if basis.rank_mode else 'RankSearch'},4,512> "
+                      "+ k_interleave<4>",
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                         "bytes_model": "D*(8+4*32) + 4*16*nnz_off + 40*S_full",
+                         "algorithmic_bytes": B}}
+
+
 def batched_section(args):
     """Sector-solves/s of the batched sweep against the plain loop of single solves.
 
@@ -188,6 +236,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default 512-row workgroups (skip the autotuner)")
+    ap.add_argument("--no-block", action="store_true",
+                    help="skip the block-matvec line (4 vectors per launch)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-ratios", type=int, default=64)
@@ -304,6 +354,10 @@ def main():
                   "op_upload_s": round(t_op, 3), "rank_mode": basis.rank_mode},
         "wall_s_timed": round(wall, 4),
     }
+
+    if not args.no_block:
+        out["block4"] = block_section(ctx, dop, basis, x, y, stream, args, nnz_off.value, S,
+                                      ms_per)
 
     if args.solve:
         log(f"[bench] eigensolve nev={w.nev} krylovdim={w.krylovdim} tol={w.tol}")

@@ -133,6 +133,14 @@ int edm_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const void *x
 /* The adjoint map (AdjointLinearMap, linear_map.jl:120-139): y = H^dagger x. */
 int edm_matvec_adjoint(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const void *x_dev,
                        void *y_dev);
+/* Block form of the map: y_v = H x_v for v < nv (the map applied to nv vectors, as a block
+ * Krylov method needs it; the reference applies its map one vector at a time,
+ * gpu_interface.jl:214-220). x_dev[v], y_dev[v] are device pointers to D complex values;
+ * work_dev is caller-owned device scratch of min(nv, 8) * D complex values (it receives an
+ * interleaved copy of the x_v so that one partner's nv entries are contiguous). Column v
+ * equals edm_matvec(x_dev[v]) bit for bit. Stream-ordered. */
+int edm_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nv,
+                     const void *const *x_dev, void *const *y_dev, void *work_dev);
 /* Host-buffer convenience: copies x in, runs edm_matvec, copies y out, synchronises. */
 int edm_matvec_host(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double *x,
                     double *y);
@@ -173,6 +181,20 @@ int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t 
 int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
                  const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+
+/* The same eigenpairs by thick-restart block Lanczos with block size `block` (1..8): each
+ * step applies the map to `block` vectors at once (edm_matvec_block) and orthonormalises
+ * them against the whole basis (CGS2, column by column); the projected matrix is complex
+ * Hermitian. krylovdim is the basis size (rounded up to a multiple of block). Fewer host
+ * round trips per matvec, one shared candidate walk per block, and robust on degenerate
+ * multiplets up to `block` levels; there is no counterpart in the reference (KrylovKit's
+ * Lanczos is single-vector). block = 1, or a subspace with D < krylovdim + block, runs
+ * edm_eigsolve's iteration. Arguments and returns as edm_eigsolve (vec0 seeds the first
+ * block column; the others are seeded random vectors). */
+int edm_eigsolve_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
+                       int32_t block, int32_t krylovdim, double tol, int32_t maxiter,
+                       uint64_t seed, const double *vec0, double *vals, double *vecs,
+                       edm_solve_info *info);
 
 /* ----------------------------------------------------------------- eigenvector consumers */
 /* y = O x between two bases: y_i = sum_j <bra_i|O|ket_j> x_j (y has bra's length, x ket's),
@@ -263,6 +285,10 @@ int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t 
  * does the same with LAPACK (the Lanczos factorization's eigen-decomposition inside
  * eigsolve). No device work. */
 int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z);
+/* The same for a complex Hermitian n x n matrix (the projected matrix of block Lanczos,
+ * edm_eigsolve_block): A and Z hold n*n interleaved (re, im) pairs, A row-major, Z with
+ * eigenvector i in Z[2*i*n .. 2*i*n + 2*n). */
+int edm_projected_heigh(int32_t n, const double *A, double *w, double *Z);
 
 /* ----------------------------------------------------------------- device vector helpers */
 /* Fill D complex values at a device pointer: mode 0 = uniform [0,1) in re and im

@@ -1,6 +1,8 @@
 // edm_api.cpp — extern "C" entry points of libedm.so (declared in include/edm.h).
 #include "edm_internal.h"
 
+#include <complex>
+
 #include <algorithm>
 #include <cstring>
 
@@ -338,6 +340,26 @@ int edm_matvec_adjoint(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const
     });
 }
 
+int edm_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nv,
+                     const void *const *x_dev, void *const *y_dev, void *work_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && x_dev && y_dev && work_dev, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(nv >= 1 && nv <= 64, EDM_EINVAL, "nv must be in 1..64");
+        for (int v = 0; v < nv; ++v) {
+            EDM_REQUIRE(x_dev[v] && y_dev[v], EDM_EINVAL, "NULL vector pointer");
+            for (int u = 0; u < nv; ++u)
+                EDM_REQUIRE(x_dev[u] != y_dev[v], EDM_EINVAL, "x and y must not alias");
+            EDM_REQUIRE(x_dev[v] != work_dev && y_dev[v] != work_dev, EDM_EINVAL,
+                        "work must not alias x or y");
+        }
+        launch_matvec_block(ctx, op, b, nv, (const double2 *const *)x_dev, (double2 *)work_dev,
+                            (double2 *const *)y_dev);
+        return EDM_OK;
+    });
+}
+
 int edm_matvec_host(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double *x, double *y)
 {
     return guarded(ctx, [&]() -> int {
@@ -465,6 +487,26 @@ int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev
         EDM_REQUIRE(op->nbits == b->nbits, EDM_EDIM,
                     "Dimension of Hamiltonian linear map mismatches vector length.");
         int rc = run_eigsolve(ctx, op, b, nev, krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+        if (rc == EDM_ENOCONV) {
+            ctx->last_error = "Krylov method fails. Cannot find " + std::to_string(nev) +
+                              " eigenvectors within maxiter.";
+        }
+        return rc;
+    });
+}
+
+int edm_eigsolve_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
+                       int32_t block, int32_t krylovdim, double tol, int32_t maxiter,
+                       uint64_t seed, const double *vec0, double *vals, double *vecs,
+                       edm_solve_info *info)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && vals, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(op->nbits == b->nbits, EDM_EDIM,
+                    "Dimension of Hamiltonian linear map mismatches vector length.");
+        int rc = run_eigsolve_block_op(ctx, op, b, block, nev, krylovdim, tol, maxiter, seed, vec0,
+                                       vals, vecs, info);
         if (rc == EDM_ENOCONV) {
             ctx->last_error = "Krylov method fails. Cannot find " + std::to_string(nev) +
                               " eigenvectors within maxiter.";
@@ -699,6 +741,23 @@ int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z)
         sym_eigh(n, a, ww, zz);
         std::copy(ww.begin(), ww.end(), w);
         std::copy(zz.begin(), zz.end(), Z);
+        return EDM_OK;
+    });
+}
+
+int edm_projected_heigh(int32_t n, const double *A, double *w, double *Z)
+{
+    return guarded(nullptr, [&]() -> int {
+        EDM_REQUIRE(n >= 1 && A && w && Z, EDM_EINVAL, "bad argument");
+        std::vector<std::complex<double>> a((size_t)n * n), zz;
+        for (size_t q = 0; q < a.size(); ++q) a[q] = std::complex<double>(A[2 * q], A[2 * q + 1]);
+        std::vector<double> ww;
+        herm_eigh(n, a, ww, zz);
+        std::copy(ww.begin(), ww.end(), w);
+        for (size_t q = 0; q < zz.size(); ++q) {
+            Z[2 * q] = zz[q].real();
+            Z[2 * q + 1] = zz[q].imag();
+        }
         return EDM_OK;
     });
 }

@@ -160,6 +160,34 @@ k_rotate(const double2 *const *__restrict__ vptrs, int m, const double *__restri
     for (int c = 0; c < kc; ++c) outptrs[c0 + c][r] = make_double2(ar[c], ai[c]);
 }
 
+// the same with complex Y (block Lanczos: the projected matrix is complex Hermitian)
+__global__ void __launch_bounds__(kBlk)
+k_rotate_c(const double2 *const *__restrict__ vptrs, int m, const double2 *__restrict__ Y,
+           int ldy, int c0, int kc, double2 *const *__restrict__ outptrs, int64_t D)
+{
+    extern __shared__ double2 yc[];   // m * kRotChunk
+    for (int t = threadIdx.x; t < m * kRotChunk; t += kBlk) {
+        int i = t / kRotChunk, c = t % kRotChunk;
+        yc[t] = c < kc ? Y[i + (int64_t)ldy * (c0 + c)] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    int64_t r = (int64_t)blockIdx.x * kBlk + threadIdx.x;
+    if (r >= D) return;
+    double ar[kRotChunk], ai[kRotChunk];
+#pragma unroll
+    for (int c = 0; c < kRotChunk; ++c) { ar[c] = 0.0; ai[c] = 0.0; }
+    for (int i = 0; i < m; ++i) {
+        double2 v = vptrs[i][r];
+#pragma unroll
+        for (int c = 0; c < kRotChunk; ++c) {
+            double2 y = yc[i * kRotChunk + c];
+            ar[c] += v.x * y.x - v.y * y.y;
+            ai[c] += v.x * y.y + v.y * y.x;
+        }
+    }
+    for (int c = 0; c < kc; ++c) outptrs[c0 + c][r] = make_double2(ar[c], ai[c]);
+}
+
 // SplitMix64 (Steele, Lea & Flood, OOPSLA 2014) as a counter-based generator:
 // value(seed, n) = mix(seed + (n + 1) * golden).
 __device__ __forceinline__ uint64_t splitmix(uint64_t seed, uint64_t n)
@@ -432,6 +460,20 @@ void blas_rotate(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const doubl
     for (int c0 = 0; c0 < k; c0 += kRotChunk) {
         int kc = std::min(kRotChunk, k - c0);
         hipLaunchKernelGGL(k_rotate, dim3(rows_blocks(D)), dim3(kBlk), sh, ctx->stream, d_vptrs,
+                           m, d_Y, ldy, c0, kc, d_outptrs, D);
+    }
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_rotate_c(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const double2 *d_Y, int ldy,
+                   int k, double2 *const *d_outptrs, int64_t D)
+{
+    if (D == 0 || k == 0) return;
+    size_t sh = (size_t)m * kRotChunk * sizeof(double2);
+    EDM_REQUIRE(sh <= 64 * 1024, EDM_EINVAL, "Krylov dimension too large for the rotation kernel");
+    for (int c0 = 0; c0 < k; c0 += kRotChunk) {
+        int kc = std::min(kRotChunk, k - c0);
+        hipLaunchKernelGGL(k_rotate_c, dim3(rows_blocks(D)), dim3(kBlk), sh, ctx->stream, d_vptrs,
                            m, d_Y, ldy, c0, kc, d_outptrs, D);
     }
     EDM_HIP_CHECK(hipGetLastError());

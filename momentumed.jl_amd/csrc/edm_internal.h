@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <complex>
 #include <functional>
 #include <string>
 #include <vector>
@@ -198,6 +199,12 @@ bool build_rank_tables(edm_basis *b);
 enum class ApplyMode { Forward, Adjoint };
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
                    double2 *y, ApplyMode mode);
+// y_v = H x_v for v < nv. When the operator has an LDS term image, x_0..x_{nv-1} are first
+// interleaved into xi (nv * D complex, caller-provided) and each group of up to 8 vectors is
+// one launch of the block kernel; otherwise nv single matvecs (xi unused). Column v equals
+// launch_matvec(x_v) bit for bit either way.
+void launch_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nv,
+                         const double2 *const *x, double2 *xi, double2 *const *y);
 void launch_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t row0,
                          int64_t nrows, int64_t *d_counts, const int64_t *d_rowptr,
                          int64_t *d_cols, int32_t *d_terms, int8_t *d_negs);
@@ -215,6 +222,14 @@ int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, in
 int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const double2 *, double2 *)> &mv,
                     int nev, int krylovdim, double tol, int maxiter, uint64_t seed,
                     const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+// block Lanczos (edm_lanczos.cpp): mv(nv, x, y) queues y_v = A x_v for v < nv
+using BlockMatvec = std::function<void(int, const double2 *const *, double2 *const *)>;
+int run_eigsolve_block(edm_ctx *ctx, int64_t D, int bs, const BlockMatvec &mv, int nev,
+                       int krylovdim, double tol, int maxiter, uint64_t seed, const double *vec0,
+                       double *vals, double *vecs, edm_solve_info *info);
+int run_eigsolve_block_op(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int bs, int nev,
+                          int krylovdim, double tol, int maxiter, uint64_t seed,
+                          const double *vec0, double *vals, double *vecs, edm_solve_info *info);
 // stored matrix (edm_csr.hip): row keys (column << 32 | term << 1 | sign) of every
 // contribution, written at rowptr[r] (edm_matvec.hip)
 void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const int64_t *d_rowptr,
@@ -242,6 +257,9 @@ void csr_matvec(edm_ctx *ctx, const edm_csr *m, const double2 *x, double2 *y);
 const char *edm_global_error();
 // eigensolver for a small real symmetric matrix (edm_lanczos.cpp)
 void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z);
+// the same for a complex Hermitian matrix (row-major A; Z column-major)
+void herm_eigh(int n, std::vector<std::complex<double>> &A, std::vector<double> &w,
+               std::vector<std::complex<double>> &Z);
 // batched matvec over many problems (edm_matvec.hip). batch_mv_desc writes the problem's
 // descriptor (batch_mv_desc_size() bytes) for y = H x and returns false when the operator has
 // no LDS-staged table (the caller then runs launch_matvec for it); blocks are 512 rows.
@@ -290,3 +308,6 @@ void bblas_rotate(edm_ctx *ctx, const BatchRotDesc *d_jobs, const int *d_blk2job
 // out_c = sum_i V_i Y[i + ldy*c] for c < k (Y real, on device)
 void blas_rotate(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const double *d_Y, int ldy,
                  int k, double2 *const *d_outptrs, int64_t D);
+// out_c = sum_i V_i Y[i + ldy*c] with complex Y (column-major), c < k
+void blas_rotate_c(edm_ctx *ctx, const double2 *const *d_vptrs, int m, const double2 *d_Y, int ldy,
+                   int k, double2 *const *d_outptrs, int64_t D);

@@ -19,28 +19,94 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <numeric>
+
+namespace {
+
+// Implicit QL iteration with Wilkinson shifts on the real symmetric tridiagonal matrix
+// (d, e) (e[i] couples i and i+1), each plane rotation applied to the columns of the
+// accumulated basis Qt (column c at Qt[c*n .. c*n+n), real or complex entries) (Bowdler,
+// Martin, Reinsch & Wilkinson, Numer. Math. 11, 1968). On return d holds the eigenvalues
+// (unsorted) and column c of Qt the eigenvector of d[c].
+template <class T>
+void tridiag_ql(int n, std::vector<double> &d, std::vector<double> &e, std::vector<T> &Qt)
+{
+    auto hyp = [](double x, double y) { return std::sqrt(x * x + y * y); };
+    const double eps = 2.220446049250313e-16;
+    for (int l = 0; l < n; ++l) {
+        for (int iter = 0; iter < 200; ++iter) {
+            int m = l;
+            for (; m + 1 < n; ++m) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= eps * dd) break;
+            }
+            if (m == l) break;
+            // Wilkinson shift from the leading 2x2 block of the unreduced part
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hyp(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double s = 1.0, c = 1.0, p = 0.0;
+            int i = m - 1;
+            bool deflated = false;
+            for (; i >= l; --i) {
+                double f = s * e[i], bb = c * e[i];
+                r = hyp(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {   // exact split: undo the pending shift and restart
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    deflated = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * bb;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - bb;
+                T *ci = &Qt[(size_t)i * n], *ci1 = &Qt[(size_t)(i + 1) * n];
+                for (int k2 = 0; k2 < n; ++k2) {   // columns i, i+1 of the basis
+                    T zi = ci[k2], zi1 = ci1[k2];
+                    ci1[k2] = s * zi + c * zi1;
+                    ci[k2] = c * zi - s * zi1;
+                }
+            }
+            if (deflated) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+}
+
+// exponent e with 2^e >= max |x|: scaling by 2^-e is exact and keeps sqrt(f^2 + g^2)
+// away from overflow and underflow
+int scale_exponent(double amax)
+{
+    int ex = 0;
+    if (amax > 0.0) (void)std::frexp(amax, &ex);
+    return ex;
+}
+
+}  // namespace
 
 // Eigen-decomposition of a real symmetric n x n matrix A (row-major, destroyed): w[i]
 // ascending and Z[:, i] (column-major, ldz = n) the matching orthonormal eigenvectors.
 // Householder reduction to tridiagonal form with the reflections accumulated (Golub & Van
-// Loan, Matrix Computations, 4th ed., Alg. 8.3.1), then the implicit QL iteration with
-// Wilkinson shifts on the tridiagonal matrix, each plane rotation applied to the accumulated
-// basis (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11, 1968). O(n^3) with small
-// constants: 0.03 ms at n = 30, the Rayleigh-Ritz step of every Lanczos restart.
+// Loan, Matrix Computations, 4th ed., Alg. 8.3.1), then the implicit QL iteration
+// (tridiag_ql). O(n^3) with small constants: 0.03 ms at n = 30, the Rayleigh-Ritz step of
+// every Lanczos restart.
 void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z)
 {
     auto a = [&](int i, int j) -> double & { return A[(size_t)i * n + j]; };
-    // work on A / 2^e with 2^e >= max|A_ij| (an exact scaling) so that plain
-    // sqrt(f^2 + g^2) neither overflows nor underflows
     double amax = 0.0;
     for (double x : A) amax = std::max(amax, std::fabs(x));
-    int ex = 0;
-    if (amax > 0.0) (void)std::frexp(amax, &ex);
+    const int ex = scale_exponent(amax);
     amax = std::ldexp(1.0, ex);
     const double inv = std::ldexp(1.0, -ex);
     for (double &x : A) x *= inv;
-    auto hyp = [](double x, double y) { return std::sqrt(x * x + y * y); };
     // the accumulated orthogonal basis, stored by columns: qt(c, r) = Q[r][c]
     std::vector<double> Qt((size_t)n * n, 0.0);
     auto qt = [&](int c, int r) -> double & { return Qt[(size_t)c * n + r]; };
@@ -93,57 +159,111 @@ void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector
     std::vector<double> d(n), e(n, 0.0);
     for (int i = 0; i < n; ++i) d[i] = a(i, i);
     for (int i = 0; i + 1 < n; ++i) e[i] = a(i + 1, i);
-    const double eps = 2.220446049250313e-16;
-    for (int l = 0; l < n; ++l) {
-        for (int iter = 0; iter < 200; ++iter) {
-            int m = l;
-            for (; m + 1 < n; ++m) {
-                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
-                if (std::fabs(e[m]) <= eps * dd) break;
-            }
-            if (m == l) break;
-            // Wilkinson shift from the leading 2x2 block of the unreduced part
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hyp(g, 1.0);
-            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
-            double s = 1.0, c = 1.0, p = 0.0;
-            int i = m - 1;
-            bool deflated = false;
-            for (; i >= l; --i) {
-                double f = s * e[i], bb = c * e[i];
-                r = hyp(f, g);
-                e[i + 1] = r;
-                if (r == 0.0) {   // exact split: undo the pending shift and restart
-                    d[i + 1] -= p;
-                    e[m] = 0.0;
-                    deflated = true;
-                    break;
-                }
-                s = f / r;
-                c = g / r;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2.0 * c * bb;
-                p = s * r;
-                d[i + 1] = g + p;
-                g = c * r - bb;
-                double *ci = &Qt[(size_t)i * n], *ci1 = &Qt[(size_t)(i + 1) * n];
-                for (int k2 = 0; k2 < n; ++k2) {   // columns i, i+1 of the basis
-                    double zi = ci[k2], zi1 = ci1[k2];
-                    ci1[k2] = s * zi + c * zi1;
-                    ci[k2] = c * zi - s * zi1;
-                }
-            }
-            if (deflated) continue;
-            d[l] -= p;
-            e[l] = g;
-            e[m] = 0.0;
-        }
-    }
+    tridiag_ql(n, d, e, Qt);
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
     w.resize(n);
     Z.assign((size_t)n * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        w[c] = d[idx[c]] * amax;
+        std::copy(Qt.begin() + (size_t)idx[c] * n, Qt.begin() + (size_t)idx[c] * n + n,
+                  Z.begin() + (size_t)c * n);
+    }
+}
+
+// Eigen-decomposition of a complex Hermitian n x n matrix A (row-major, destroyed): w[i]
+// ascending and Z[:, i] (column-major) orthonormal eigenvectors. Householder reduction with
+// complex reflectors H = I - beta v v^H (beta real) to a Hermitian tridiagonal matrix, whose
+// off-diagonal phases are then moved into the basis by a diagonal unitary (Golub & Van Loan,
+// 4th ed., Sec. 8.3.1 and Problem 8.3.7), leaving a real symmetric tridiagonal matrix for
+// the implicit QL iteration.
+void herm_eigh(int n, std::vector<std::complex<double>> &A, std::vector<double> &w,
+               std::vector<std::complex<double>> &Z)
+{
+    using cd = std::complex<double>;
+    auto a = [&](int i, int j) -> cd & { return A[(size_t)i * n + j]; };
+    double amax = 0.0;
+    for (const cd &x : A) amax = std::max(amax, std::max(std::fabs(x.real()), std::fabs(x.imag())));
+    const int ex = scale_exponent(amax);
+    amax = std::ldexp(1.0, ex);
+    const double inv = std::ldexp(1.0, -ex);
+    for (cd &x : A) x *= inv;
+    for (int i = 0; i < n; ++i) a(i, i) = cd(a(i, i).real(), 0.0);
+    std::vector<cd> Qt((size_t)n * n, cd(0.0, 0.0));
+    auto qt = [&](int c, int r) -> cd & { return Qt[(size_t)c * n + r]; };
+    for (int i = 0; i < n; ++i) qt(i, i) = 1.0;
+    std::vector<cd> v(n), pv(n), sr(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        // reflect x = A[k+1:, k] onto alpha e1, alpha = -(x0 / |x0|) |x|
+        double xn = 0.0, tail = 0.0;
+        for (int i = k + 1; i < n; ++i) xn += std::norm(a(i, k));
+        for (int i = k + 2; i < n; ++i) tail += std::norm(a(i, k));
+        xn = std::sqrt(xn);
+        if (tail == 0.0) continue;
+        const cd x0 = a(k + 1, k);
+        const double ax0 = std::abs(x0);
+        const cd ph = ax0 > 0.0 ? x0 / ax0 : cd(1.0, 0.0);
+        const cd alpha = -ph * xn;
+        for (int i = k + 1; i < n; ++i) v[i] = a(i, k);
+        v[k + 1] -= alpha;
+        double vv = 0.0;
+        for (int i = k + 1; i < n; ++i) vv += std::norm(v[i]);
+        const double beta = 2.0 / vv;
+        // A22 <- H A22 H:  p = beta A22 v, K = beta/2 v^H p (real), u = p - K v,
+        // A22 -= v u^H + u v^H
+        cd vp = 0.0;
+        for (int i = k + 1; i < n; ++i) {
+            cd s = 0.0;
+            for (int j = k + 1; j < n; ++j) s += a(i, j) * v[j];
+            pv[i] = beta * s;
+            vp += std::conj(v[i]) * pv[i];
+        }
+        const double K = 0.5 * beta * vp.real();
+        for (int i = k + 1; i < n; ++i) pv[i] -= K * v[i];
+        for (int i = k + 1; i < n; ++i)
+            for (int j = k + 1; j < n; ++j)
+                a(i, j) -= v[i] * std::conj(pv[j]) + pv[i] * std::conj(v[j]);
+        for (int i = k + 1; i < n; ++i) a(i, i) = cd(a(i, i).real(), 0.0);
+        a(k + 1, k) = alpha;
+        a(k, k + 1) = std::conj(alpha);
+        for (int i = k + 2; i < n; ++i) a(i, k) = a(k, i) = 0.0;
+        // Q <- Q H: Q[r][j] -= beta (sum_l Q[r][l] v_l) conj(v_j)
+        std::fill(sr.begin(), sr.end(), cd(0.0, 0.0));
+        for (int j = k + 1; j < n; ++j) {
+            const cd vj = v[j];
+            const cd *col = &Qt[(size_t)j * n];
+            for (int r = 0; r < n; ++r) sr[r] += col[r] * vj;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            const cd bvj = beta * std::conj(v[j]);
+            cd *col = &Qt[(size_t)j * n];
+            for (int r = 0; r < n; ++r) col[r] -= sr[r] * bvj;
+        }
+    }
+    // T = P T' P^H with P = diag(ph), T' real: ph[0] = 1, ph[i+1] = ph[i] e_i / |e_i|;
+    // the eigenvectors of A are Q P z
+    std::vector<double> d(n), e(n, 0.0);
+    cd phase = 1.0;
+    for (int i = 0; i < n; ++i) {
+        d[i] = a(i, i).real();
+        if (i > 0) {
+            cd* col = &Qt[(size_t)i * n];
+            for (int r = 0; r < n; ++r) col[r] *= phase;
+        }
+        if (i + 1 < n) {
+            const cd ei = a(i + 1, i);
+            const double ae = std::abs(ei);
+            e[i] = ae;
+            if (ae > 0.0) phase *= ei / ae;
+        }
+    }
+    tridiag_ql(n, d, e, Qt);
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
+    w.resize(n);
+    Z.assign((size_t)n * n, cd(0.0, 0.0));
     for (int c = 0; c < n; ++c) {
         w[c] = d[idx[c]] * amax;
         std::copy(Qt.begin() + (size_t)idx[c] * n, Qt.begin() + (size_t)idx[c] * n + n,
@@ -402,4 +522,344 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
         info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
     return nconv >= nev ? EDM_OK : EDM_ENOCONV;
+}
+
+// ---------------------------------------------------------------- block Lanczos
+// Thick-restart block Lanczos with block size bs (Golub & Underwood 1977; thick restart as in
+// Wu & Simon 2000, with the restart coupling a bs x keep block instead of a row). A step
+// applies H to the current block of bs basis vectors at once (one launch of the block matvec:
+// candidate generation, signs and partner lookups are shared by the bs vectors) and
+// orthonormalises the bs results column by column with CGS2 against every earlier basis
+// vector, so the projected matrix T = V^H H V is block tridiagonal plus a block arrow after a
+// restart, and complex Hermitian (herm_eigh). Residual of Ritz pair i: ||Bn y_i[last block]||
+// with Bn the triangular coupling of the next block.
+namespace {
+
+struct BlockWorkspace {
+    edm_ctx *ctx;
+    int64_t D;
+    int M, bs, ld;                // basis capacity (multiple of bs), block size, stride of d_c
+    std::vector<double2 *> V, tmp;
+    double2 **d_vptrs = nullptr, **d_tmpptrs = nullptr;
+    double2 *d_c = nullptr;       // per column slot: h (M+bs) | h2 (M+bs) | beta
+    double2 *d_Y = nullptr;
+    size_t vbytes = 0;
+
+    BlockWorkspace(edm_ctx *c, int64_t D_, int M_, int bs_) : ctx(c), D(D_), M(M_), bs(bs_)
+    {
+        ld = 2 * (M + bs) + 1;
+        vbytes = (size_t)std::max<int64_t>(D, 1) * sizeof(double2);
+        try {
+            for (int i = 0; i < M + bs; ++i) V.push_back((double2 *)ctx->alloc(vbytes));
+            for (int i = 0; i < M; ++i) tmp.push_back((double2 *)ctx->alloc(vbytes));
+            d_vptrs = (double2 **)ctx->alloc((M + bs) * sizeof(double2 *));
+            d_tmpptrs = (double2 **)ctx->alloc(M * sizeof(double2 *));
+            d_c = (double2 *)ctx->alloc((size_t)bs * ld * sizeof(double2));
+            d_Y = (double2 *)ctx->alloc((size_t)M * M * sizeof(double2));
+        } catch (...) {
+            release();
+            throw;
+        }
+        upload_ptrs();
+    }
+    void upload_ptrs()
+    {
+        EDM_HIP_CHECK(hipMemcpyAsync(d_vptrs, V.data(), V.size() * sizeof(double2 *),
+                                     hipMemcpyHostToDevice, ctx->stream));
+        EDM_HIP_CHECK(hipMemcpyAsync(d_tmpptrs, tmp.data(), tmp.size() * sizeof(double2 *),
+                                     hipMemcpyHostToDevice, ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    double2 *h(int c) { return d_c + (size_t)c * ld; }
+    double2 *h2(int c) { return d_c + (size_t)c * ld + (M + bs); }
+    double *beta(int c) { return (double *)(d_c + (size_t)c * ld + 2 * (M + bs)); }
+    void release()
+    {
+        for (auto p : V) ctx->free(p, vbytes);
+        for (auto p : tmp) ctx->free(p, vbytes);
+        V.clear();
+        tmp.clear();
+        ctx->free(d_vptrs, (M + bs) * sizeof(double2 *));
+        ctx->free(d_tmpptrs, M * sizeof(double2 *));
+        ctx->free(d_c, (size_t)bs * ld * sizeof(double2));
+        ctx->free(d_Y, (size_t)M * M * sizeof(double2));
+        d_vptrs = d_tmpptrs = nullptr;
+        d_c = d_Y = nullptr;
+    }
+    ~BlockWorkspace() { release(); }
+};
+
+// queue CGS2 of V[n] against V[0..n), its norm and its scaling into column slot c
+void queue_column(BlockWorkspace &ws, int c, int n)
+{
+    edm_ctx *ctx = ws.ctx;
+    double2 *w = ws.V[n];
+    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.h(c));
+    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.h(c), w, ws.D);
+    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.h2(c));
+    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.h2(c), w, ws.D);
+    blas_norm(ctx, w, ws.D, ws.beta(c));
+}
+
+// read back slots 0..nc: hsum[c][i] = h + h2 for i < n0 + c, beta[c]
+void read_columns(BlockWorkspace &ws, int nc, int n0, std::vector<std::vector<double2>> &hsum,
+                  std::vector<double> &beta)
+{
+    std::vector<double2> buf((size_t)nc * ws.ld);
+    EDM_HIP_CHECK(hipMemcpyAsync(buf.data(), ws.d_c, buf.size() * sizeof(double2),
+                                 hipMemcpyDeviceToHost, ws.ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ws.ctx->stream));
+    hsum.assign(nc, {});
+    beta.assign(nc, 0.0);
+    for (int c = 0; c < nc; ++c) {
+        const double2 *h = buf.data() + (size_t)c * ws.ld, *h2 = h + (ws.M + ws.bs);
+        int n = n0 + c;
+        hsum[c].resize(n);
+        for (int i = 0; i < n; ++i) hsum[c][i] = make_double2(h[i].x + h2[i].x, h[i].y + h2[i].y);
+        beta[c] = *(const double *)(h2 + (ws.M + ws.bs));
+    }
+}
+
+}  // namespace
+
+int run_eigsolve_block(edm_ctx *ctx, int64_t D, int bs, const BlockMatvec &mv, int nev,
+                       int krylovdim, double tol, int maxiter, uint64_t seed, const double *vec0,
+                       double *vals, double *vecs, edm_solve_info *info)
+{
+    using cd = std::complex<double>;
+    auto t0 = std::chrono::steady_clock::now();
+    EDM_REQUIRE(D > 0, EDM_EINVAL, "empty Hilbert subspace");
+    EDM_REQUIRE(nev >= 1, EDM_EINVAL, "N should be at least 1.");
+    EDM_REQUIRE(nev <= D, EDM_EINVAL, "more eigenpairs requested than the subspace dimension");
+    EDM_REQUIRE(tol > 0, EDM_EINVAL, "tol must be positive");
+    EDM_REQUIRE(maxiter >= 1, EDM_EINVAL, "maxiter must be at least 1");
+    EDM_REQUIRE(bs >= 1 && bs <= 8, EDM_EINVAL, "block size must be in 1..8");
+    int M = std::max(krylovdim, nev + 2 * bs);
+    M = (M + bs - 1) / bs * bs;
+    if (bs == 1 || (int64_t)(M + bs) > D) {
+        // one vector at a time (also when the space is too small for M + bs basis vectors)
+        return run_eigsolve_mv(
+            ctx, D,
+            [&](const double2 *x, double2 *y) {
+                const double2 *xs[1] = {x};
+                double2 *ys[1] = {y};
+                mv(1, xs, ys);
+            },
+            nev, krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+    }
+    EDM_REQUIRE(M + bs <= 256, EDM_EINVAL, "krylovdim + block size must be at most 256");
+    BlockWorkspace ws(ctx, D, M, bs);
+    {
+        int64_t nb = (D + 1023) / 1024;
+        ctx->partials((size_t)(nb + 1) * (M + bs) * sizeof(double2));
+    }
+    hipStream_t st = ctx->stream;
+    int matvecs = 0;
+    uint64_t rng_stream = 0;
+    double hnorm = 0.0;
+
+    // orthonormalise V[n] against V[0..n) with a host check after every pass: a column that
+    // lost more than 8 digits to cancellation gets a further CGS2, one that vanishes against
+    // ||H|| is replaced by a random direction (coupling zero). hsum gets the coefficients.
+    auto careful_column = [&](int n, std::vector<double2> &hsum) -> double {
+        hsum.assign(n, make_double2(0.0, 0.0));
+        bool replaced = false;
+        for (int pass = 0; pass < 4; ++pass) {
+            queue_column(ws, 0, n);
+            std::vector<std::vector<double2>> hh;
+            std::vector<double> bb;
+            read_columns(ws, 1, n, hh, bb);
+            double hn = 0.0;
+            for (int i = 0; i < n; ++i) {
+                if (!replaced) {
+                    hsum[i].x += hh[0][i].x;
+                    hsum[i].y += hh[0][i].y;
+                }
+                hn += hh[0][i].x * hh[0][i].x + hh[0][i].y * hh[0][i].y;
+            }
+            double beta = bb[0];
+            double wn = std::sqrt(hn + beta * beta);
+            if (!replaced && !(beta > 1e-14 * std::max(hnorm, 1e-300))) {
+                blas_fill_random(ctx, ws.V[n], D, seed ^ (0x5A5A5A5AULL + ++rng_stream), 0);
+                replaced = true;
+                continue;
+            }
+            if (beta > 1e-8 * wn) {
+                blas_scale_inv(ctx, ws.V[n], ws.V[n], ws.beta(0), D);
+                return replaced ? 0.0 : beta;
+            }
+        }
+        EDM_REQUIRE(false, EDM_ENOCONV, "block Lanczos could not orthonormalise a basis vector");
+        return 0.0;
+    };
+
+    // start block: vec0 (or random) and bs - 1 random vectors, orthonormalised
+    if (vec0)
+        EDM_HIP_CHECK(hipMemcpyAsync(ws.V[0], vec0, D * sizeof(double2), hipMemcpyHostToDevice, st));
+    else
+        blas_fill_random(ctx, ws.V[0], D, seed, 0);
+    for (int c = 1; c < bs; ++c) blas_fill_random(ctx, ws.V[c], D, seed ^ (0x9E37ULL * c), 0);
+    for (int c = 0; c < bs; ++c) {
+        std::vector<double2> hs;
+        hnorm = 1.0;   // start vectors: cancellation relative to their own norm
+        double b0 = careful_column(c, hs);
+        EDM_REQUIRE(b0 > 0 || c > 0, EDM_EINVAL, "start vector is zero or not finite");
+    }
+    hnorm = 0.0;
+
+    std::vector<cd> T((size_t)M * M, cd(0.0, 0.0));
+    auto t = [&](int i, int j) -> cd & { return T[(size_t)i * M + j]; };
+    std::vector<cd> Bn((size_t)bs * bs, cd(0.0, 0.0));   // coupling of the block after V[M-bs..M)
+    std::vector<double> theta, res(M, 0.0);
+    std::vector<cd> Y;
+    int k = 0, iters = 0, nconv = 0;
+    std::vector<const double2 *> xs(bs);
+    std::vector<double2 *> ys(bs);
+
+    for (iters = 1; iters <= maxiter; ++iters) {
+        for (int j = k; j < M; j += bs) {
+            const int p0 = j + bs;
+            for (int c = 0; c < bs; ++c) {
+                xs[c] = ws.V[j + c];
+                ys[c] = ws.V[p0 + c];
+            }
+            mv(bs, xs.data(), ys.data());
+            matvecs += bs;
+            // fast path: every column's CGS2 + norm + scaling queued, one read-back
+            for (int c = 0; c < bs; ++c) {
+                queue_column(ws, c, p0 + c);
+                blas_scale_inv(ctx, ws.V[p0 + c], ws.V[p0 + c], ws.beta(c), D);
+            }
+            std::vector<std::vector<double2>> hs;
+            std::vector<double> beta;
+            read_columns(ws, bs, p0, hs, beta);
+            double est = 0.0;
+            for (int c = 0; c < bs; ++c) {
+                double col = beta[c];
+                for (int r = 0; r < bs; ++r) col += std::hypot(hs[c][j + r].x, hs[c][j + r].y);
+                est = std::max(est, col);
+            }
+            hnorm = std::max(hnorm, est);
+            bool clean = true;
+            for (int c = 0; c < bs && clean; ++c) {
+                double hn = 0.0;
+                for (const double2 &v : hs[c]) hn += v.x * v.x + v.y * v.y;
+                double wn = std::sqrt(hn + beta[c] * beta[c]);
+                clean = beta[c] > 1e-14 * std::max(hnorm, 1e-300) && beta[c] > 1e-8 * wn;
+            }
+            if (!clean) {
+                // slow path: recompute H X and orthonormalise column by column
+                mv(bs, xs.data(), ys.data());
+                matvecs += bs;
+                for (int c = 0; c < bs; ++c) beta[c] = careful_column(p0 + c, hs[c]);
+            }
+            // diagonal block (Hermitian part) and the triangular coupling to the new block
+            for (int r = 0; r < bs; ++r)
+                for (int c = 0; c < bs; ++c) {
+                    cd a(hs[c][j + r].x, hs[c][j + r].y), b(hs[r][j + c].x, -hs[r][j + c].y);
+                    t(j + r, j + c) = r == c ? cd(a.real(), 0.0) : 0.5 * (a + b);
+                }
+            for (int c = 0; c < bs; ++c)
+                for (int r = 0; r < bs; ++r) {
+                    cd bv = r < c ? cd(hs[c][p0 + r].x, hs[c][p0 + r].y)
+                                  : (r == c ? cd(beta[c], 0.0) : cd(0.0, 0.0));
+                    if (p0 < M) {
+                        t(p0 + r, j + c) = bv;
+                        t(j + c, p0 + r) = std::conj(bv);
+                    } else {
+                        Bn[(size_t)r * bs + c] = bv;
+                    }
+                }
+        }
+        // Rayleigh-Ritz
+        std::vector<cd> A = T;
+        herm_eigh(M, A, theta, Y);
+        nconv = 0;
+        for (int i = 0; i < M; ++i) {
+            double s2 = 0.0;
+            for (int r = 0; r < bs; ++r) {
+                cd v = 0.0;
+                for (int s = 0; s < bs; ++s) v += Bn[(size_t)r * bs + s] * Y[(size_t)i * M + M - bs + s];
+                s2 += std::norm(v);
+            }
+            res[i] = std::sqrt(s2);
+        }
+        while (nconv < nev && res[nconv] <= tol) ++nconv;
+        if (nconv >= nev || iters == maxiter) break;
+        // thick restart: keep about the lowest (3M + 2 nconv) / 5 Ritz vectors (at least
+        // nev), rounded down so that M - keep is a whole number q of blocks: rounding up
+        // instead can leave one block step per cycle, which converges far more slowly
+        int keep = std::max((3 * M + 2 * nconv) / 5, nev);
+        int q = std::max(1, (M - keep + bs - 1) / bs);
+        while (q > 1 && M - q * bs < nev) --q;
+        keep = M - q * bs;
+        EDM_HIP_CHECK(hipMemcpyAsync(ws.d_Y, Y.data(), (size_t)M * keep * sizeof(double2),
+                                     hipMemcpyHostToDevice, st));
+        blas_rotate_c(ctx, (const double2 *const *)ws.d_vptrs, M, ws.d_Y, M, keep,
+                      (double2 *const *)ws.d_tmpptrs, D);
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int c = 0; c < keep; ++c) std::swap(ws.V[c], ws.tmp[c]);
+        for (int r = 0; r < bs; ++r) std::swap(ws.V[keep + r], ws.V[M + r]);
+        ws.upload_ptrs();
+        std::fill(T.begin(), T.end(), cd(0.0, 0.0));
+        for (int c = 0; c < keep; ++c) {
+            t(c, c) = theta[c];
+            for (int r = 0; r < bs; ++r) {
+                cd s = 0.0;
+                for (int u = 0; u < bs; ++u) s += Bn[(size_t)r * bs + u] * Y[(size_t)c * M + M - bs + u];
+                t(keep + r, c) = s;
+                t(c, keep + r) = std::conj(s);
+            }
+        }
+        k = keep;
+    }
+    if (iters > maxiter) iters = maxiter;
+
+    for (int i = 0; i < nev; ++i) vals[i] = theta[i];
+    double maxres = 0.0;
+    for (int i = 0; i < nev; ++i) maxres = std::max(maxres, res[i]);
+    if (vecs) {
+        EDM_HIP_CHECK(hipMemcpyAsync(ws.d_Y, Y.data(), (size_t)M * nev * sizeof(double2),
+                                     hipMemcpyHostToDevice, st));
+        blas_rotate_c(ctx, (const double2 *const *)ws.d_vptrs, M, ws.d_Y, M, nev,
+                      (double2 *const *)ws.d_tmpptrs, D);
+        for (int i = 0; i < nev; ++i)
+            EDM_HIP_CHECK(hipMemcpyAsync(vecs + 2 * (size_t)i * D, ws.tmp[i], D * sizeof(double2),
+                                         hipMemcpyDeviceToHost, st));
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    if (info) {
+        info->iterations = iters;
+        info->matvecs = matvecs;
+        info->converged = nconv;
+        info->krylovdim = M;
+        info->max_residual = maxres;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return nconv >= nev ? EDM_OK : EDM_ENOCONV;
+}
+
+int run_eigsolve_block_op(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int bs, int nev,
+                          int krylovdim, double tol, int maxiter, uint64_t seed,
+                          const double *vec0, double *vals, double *vecs, edm_solve_info *info)
+{
+    const int64_t D = b->D;
+    const size_t xb = (size_t)std::min(bs, 8) * (size_t)std::max<int64_t>(D, 1) * sizeof(double2);
+    double2 *xi = (double2 *)ctx->alloc(xb);
+    int rc;
+    try {
+        rc = run_eigsolve_block(
+            ctx, D, bs,
+            [&](int nv, const double2 *const *x, double2 *const *y) {
+                launch_matvec_block(ctx, op, b, nv, x, xi, y);
+            },
+            nev, krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->free(xi, xb);
+        throw;
+    }
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->free(xi, xb);
+    return rc;
 }

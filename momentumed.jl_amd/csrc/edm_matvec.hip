@@ -404,6 +404,17 @@ __device__ __forceinline__ double flip_sign(double v, uint32_t neg)
     return __hiloint2double(__double2hiint(v) ^ (int)(neg << 31), __double2loint(v));
 }
 
+// y += (ar + i ai) * x as four explicit fused multiply-adds (real part: +ar*x.re, then
+// -ai*x.im; imaginary part: +ar*x.im, then +ai*x.re). Spelled out so that every kernel that
+// uses it rounds identically, whatever the compiler would otherwise contract.
+__device__ __forceinline__ void cmac(double &yr, double &yi, double ar, double ai, double2 x)
+{
+    yr = __builtin_fma(ar, x.x, yr);
+    yr = __builtin_fma(-ai, x.y, yr);
+    yi = __builtin_fma(ar, x.y, yi);
+    yi = __builtin_fma(ai, x.x, yi);
+}
+
 // Lookup traits: a complete sector's rank tables find every partner of a sector-conserving
 // operator; the binary search may miss (partner outside a partial basis).
 template <class Rank> struct RankMayMiss { static constexpr bool value = true; };
@@ -454,9 +465,7 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
         if (windowed && (j < jmin || j >= jmax)) return;
         double2 a = t.amp[e];
         double2 xv = x[j];
-        double ax = flip_sign(a.x, neg), ay = flip_sign(a.y, neg);
-        yr += ax * xv.x - ay * xv.y;
-        yi += ax * xv.y + ay * xv.x;
+        cmac(yr, yi, flip_sign(a.x, neg), flip_sign(a.y, neg), xv);
     };
     auto enter = [&](int key, int &e0, int &e1) {
         if (with_diag) {
@@ -507,11 +516,7 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
             }
         }
     }
-    if (with_diag) {
-        double2 xi = x[i];
-        yr += dr * xi.x - di * xi.y;
-        yi += dr * xi.y + di * xi.x;
-    }
+    if (with_diag) cmac(yr, yi, dr, di, x[i]);
     return make_double2(yr, yi);
 }
 
@@ -637,6 +642,122 @@ k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
         v.y += o.y;
     }
     y[i] = v;
+}
+
+// ---------------------------------------------------------------- block matvec
+// Y = H X for NV vectors at once. The candidate generation, sign and partner lookup of a row
+// are done once and serve all NV vectors; the gathers read x_0..x_{NV-1} of partner j from one
+// interleaved copy Xi[j*NV + v] (NV*16 contiguous bytes), so one partner costs one cache
+// line for all vectors. Per vector the arithmetic (cmac) and the summation order are those
+// of row_one, so column v of Y equals the single-vector matvec of x_v bit for bit.
+template <int NV> struct VecPtrs { double2 *p[NV]; };
+template <int NV> struct CVecPtrs { const double2 *p[NV]; };
+
+template <class Rank, int NV>
+__device__ __forceinline__ void row_block(uint64_t s, const unsigned char *base,
+                                          const CompactDev &t, const Rank &rank,
+                                          const double2 *__restrict__ xi, double2 *acc,
+                                          double2 *dsum)
+{
+    const uint16_t *s_keyptr = (const uint16_t *)base;
+    const double2 *s_diag = (const double2 *)(base + t.off_diag);
+    const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
+    double dr = 0.0, di = 0.0;
+    const auto rrow = rank_row32(rank, s);
+
+    auto take = [&](int e, uint64_t target, uint32_t neg) {
+        int64_t j = rank_find(rrow, target);
+        if (RankMayMiss<Rank>::value && j < 0) return;
+        double2 a = t.amp[e];
+        const double2 *xj = xi + j * NV;
+        double2 xv[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xv[v] = xj[v];
+        double ax = flip_sign(a.x, neg), ay = flip_sign(a.y, neg);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) cmac(acc[v].x, acc[v].y, ax, ay, xv[v]);
+    };
+    auto enter = [&](int key, int &e0, int &e1) {
+        double2 dg = s_diag[key];
+        dr += dg.x;
+        di += dg.y;
+        e0 = s_keyptr[key];
+        e1 = s_keyptr[key + 1];
+    };
+    uint64_t rem = s;
+    while (rem) {
+        int b = 63 - __builtin_clzll(rem);
+        rem ^= 1ULL << b;
+        const uint64_t ob = 1ULL << b;
+        if (b < t.nkeys) {
+            int e0, e1;
+            enter(b, e0, e1);
+            const uint64_t mid = s & ~ob;
+            const uint32_t pout = __popcll(mid & pm_one(b)) & 1u;
+            for (int e = e0; e < e1; ++e) {
+                const int c = s_code[e] & 0xFF;
+                const uint64_t inm = 1ULL << c;
+                if (mid & inm) continue;
+                const uint32_t neg = (pout + (uint32_t)__popcll(mid & pm_one(c))) & 1u;
+                take(e, mid | inm, neg);
+            }
+        }
+        uint64_t low = s & (ob - 1ULL);
+        const int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            int a = 63 - __builtin_clzll(low);
+            low ^= 1ULL << a;
+            const int key = kb + a;
+            if (key >= t.nkeys) continue;
+            int e0, e1;
+            enter(key, e0, e1);
+            const uint64_t mid = s & ~(ob | (1ULL << a));
+            const uint32_t pout = __popcll(mid & pm_pair(a, b)) & 1u;
+            for (int e = e0; e < e1; ++e) {
+                const unsigned code = s_code[e];
+                const int c = code & 0xFF, d = code >> 8;
+                const uint64_t bd = 1ULL << d;
+                const uint64_t inm = bd | (1ULL << c);
+                if (mid & inm) continue;
+                const uint32_t neg = (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u;
+                take(e, mid | inm, neg);
+            }
+        }
+    }
+    *dsum = make_double2(dr, di);
+}
+
+template <class Rank, int NV, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_matvec_block(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
+               const double2 *__restrict__ xi, VecPtrs<NV> y)
+{
+    extern __shared__ uint4 lds_u4[];
+    stage_image(t, lds_u4);
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= D) return;
+    double2 acc[NV], dg;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = make_double2(0.0, 0.0);
+    row_block<Rank, NV>(states[i], (const unsigned char *)lds_u4, t, rank, xi, acc, &dg);
+    const double2 *xr = xi + i * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        cmac(acc[v].x, acc[v].y, dg.x, dg.y, xr[v]);
+        y.p[v][i] = acc[v];
+    }
+}
+
+// Xi[i*NV + v] = x_v[i]
+template <int NV>
+__global__ void __launch_bounds__(kRowBlock)
+k_interleave(int64_t D, CVecPtrs<NV> x, double2 *__restrict__ xi)
+{
+    int64_t i = (int64_t)blockIdx.x * kRowBlock + threadIdx.x;
+    if (i >= D) return;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) xi[i * NV + v] = x.p[v][i];
 }
 
 // ---------------------------------------------------------------- batched matvec
@@ -1008,6 +1129,57 @@ void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const dou
                                op->d_out, op->d_in, op->d_par, make_search(b), x, y, adj);
     }
     EDM_HIP_CHECK(hipGetLastError());
+}
+
+template <class Rank, int NV>
+static void launch_block_nv(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                            const double2 *const *x, double2 *xi, double2 *const *y)
+{
+    CVecPtrs<NV> xp;
+    VecPtrs<NV> yp;
+    for (int v = 0; v < NV; ++v) {
+        xp.p[v] = x[v];
+        yp.p[v] = y[v];
+    }
+    hipLaunchKernelGGL(k_interleave<NV>, dim3(nblk(b->D)), dim3(kRowBlock), 0, ctx->stream, b->D,
+                       xp, xi);
+    CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
+                  (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
+    hipLaunchKernelGGL((k_matvec_block<Rank, NV, kLdsBlock>), dim3(nb), dim3(kLdsBlock),
+                       ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, rank, xi, yp);
+}
+
+template <class Rank>
+static void launch_block_rank(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                              int nv, const double2 *const *x, double2 *xi, double2 *const *y)
+{
+    // groups of 8, then 4, 2 and 1; xi is reused by consecutive groups (stream-ordered)
+    int v = 0;
+    while (v < nv) {
+        int n = nv - v >= 8 ? 8 : nv - v >= 4 ? 4 : nv - v >= 2 ? 2 : 1;
+        if (n == 8) launch_block_nv<Rank, 8>(ctx, b, ct, rank, x + v, xi, y + v);
+        else if (n == 4) launch_block_nv<Rank, 4>(ctx, b, ct, rank, x + v, xi, y + v);
+        else if (n == 2) launch_block_nv<Rank, 2>(ctx, b, ct, rank, x + v, xi, y + v);
+        else launch_block_nv<Rank, 1>(ctx, b, ct, rank, x + v, xi, y + v);
+        EDM_HIP_CHECK(hipGetLastError());
+        v += n;
+    }
+}
+
+void launch_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nv,
+                         const double2 *const *x, double2 *xi, double2 *const *y)
+{
+    EDM_REQUIRE(op->nbits == b->nbits, EDM_EINVAL, "operator and basis have different bits");
+    if (b->D == 0 || nv <= 0) return;
+    if (!(op->fast && op->cfwd.ok && matvec_variant() == 11)) {
+        for (int v = 0; v < nv; ++v) launch_matvec(ctx, op, b, x[v], y[v], ApplyMode::Forward);
+        return;
+    }
+    if (use_lin(op, b))
+        launch_block_rank(ctx, b, op->cfwd, make_lin(b), nv, x, xi, y);
+    else
+        launch_block_rank(ctx, b, op->cfwd, make_search(b), nv, x, xi, y);
 }
 
 size_t batch_mv_desc_size() { return sizeof(BatchMvDesc); }

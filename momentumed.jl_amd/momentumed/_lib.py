@@ -55,12 +55,15 @@ _SIGS = [
     ("edm_op_destroy", c_i32, [c_vp]),
     ("edm_matvec", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("edm_matvec_adjoint", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("edm_matvec_block", c_i32, [c_vp, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     ("edm_matvec_host", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("edm_connectivity_counts", c_i32, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     ("edm_connectivity", c_i32, [c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp]),
     ("edm_connectivity_totals", c_i32, [c_vp, c_vp, c_vp, P(c_i64), P(c_i64)]),
     ("edm_eigsolve", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_vp,
                              c_vp, P(EdmSolveInfo)]),
+    ("edm_eigsolve_block", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_i32, c_u64,
+                                   c_vp, c_vp, c_vp, P(EdmSolveInfo)]),
     ("edm_batch_create", c_i32, [c_vp, P(c_vp)]),
     ("edm_batch_destroy", c_i32, [c_vp]),
     ("edm_batch_add", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_i32,
@@ -85,6 +88,7 @@ _SIGS = [
                                  P(EdmSolveInfo)]),
     ("edm_csr_destroy", c_i32, [c_vp]),
     ("edm_projected_eigh", c_i32, [c_i32, c_vp, c_vp, c_vp]),
+    ("edm_projected_heigh", c_i32, [c_i32, c_vp, c_vp, c_vp]),
     ("edm_fill_random", c_i32, [c_vp, c_vp, c_i64, c_u64, c_i32]),
 ]
 

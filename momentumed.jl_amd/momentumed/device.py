@@ -319,10 +319,13 @@ def device_basis_of(space, ctx: Context) -> DeviceBasis:
 
 def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True, vec0=None,
                      krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100,
-                     seed: int = 0x5EED, want_vectors: bool = True, **_ignored):
+                     seed: int = 0x5EED, want_vectors: bool = True, blocksize: int = 1,
+                     **_ignored):
     """gpu_interface.jl:92-94 / MomentumEDCUDAExt.jl:159-179 with KrylovKit's defaults
     (krylovdim 30, tol 1e-12, maxiter 100). Returns (vals, vecs, info); vecs is a list of
-    host complex128 arrays (or None)."""
+    host complex128 arrays (or None). blocksize > 1 runs block Lanczos (edm_eigsolve_block:
+    `blocksize` vectors per step through the block matvec; krylovdim is then the basis size,
+    rounded up to a multiple of the block)."""
     if not ishermitian:
         raise ValueError("the device eigensolver implements the Hermitian (:SR) case")
     if H.is_adjoint:
@@ -338,9 +341,15 @@ def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True
             raise ValueError("Dimension of Hamiltonian linear map mismatches vector length.")
     info = EdmSolveInfo()
     ctx = H.ctx
-    rc = ctx.L.edm_eigsolve(ctx.h, H.op.h, H.basis.h, N_eigen, krylovdim, float(tol), maxiter,
-                            seed, v0.ctypes.data if v0 is not None else None, vals.ctypes.data,
-                            vecs.ctypes.data if vecs is not None else None, ctypes.byref(info))
+    v0p = v0.ctypes.data if v0 is not None else None
+    vp = vecs.ctypes.data if vecs is not None else None
+    if blocksize == 1:
+        rc = ctx.L.edm_eigsolve(ctx.h, H.op.h, H.basis.h, N_eigen, krylovdim, float(tol),
+                                maxiter, seed, v0p, vals.ctypes.data, vp, ctypes.byref(info))
+    else:
+        rc = ctx.L.edm_eigsolve_block(ctx.h, H.op.h, H.basis.h, N_eigen, int(blocksize),
+                                      krylovdim, float(tol), maxiter, seed, v0p,
+                                      vals.ctypes.data, vp, ctypes.byref(info))
     if rc not in (_lib.EDM_OK, _lib.EDM_ENOCONV):
         check(rc, ctx.h)
     out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
@@ -424,9 +433,13 @@ def create_gpu_matrix(op, space, device: str = "rocm", ctx: Context | None = Non
 
 def krylov_matrix_solve(H: DeviceCSR, N_eigen: int, *, ishermitian: bool = True, vec0=None,
                         krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100,
-                        seed: int = 0x5EED, want_vectors: bool = True, **_ignored):
+                        seed: int = 0x5EED, want_vectors: bool = True, blocksize: int = 1,
+                        **_ignored):
     """krylov_matrix_solve (MomentumED.jl:316-333) on the device matrix; the same Lanczos
     iteration and defaults as krylov_map_solve. Returns (vals, vecs, info)."""
+    if blocksize != 1:
+        raise NotImplementedError("block Lanczos is provided for the matrix-free map "
+                                  "(method=\"map\")")
     if not ishermitian:
         raise ValueError("the device eigensolver implements the Hermitian (:SR) case")
     D = H.D

@@ -27,7 +27,9 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
     Both Krylov methods run on the device: method="sparse" (the default, as in the reference)
     builds the stored matrix on the device and solves with it (MomentumED.jl:259-333);
     method="map" applies the matrix-free map (MomentumED.jl:375-480). KrylovKit keywords
-    krylovdim, tol, maxiter, vec0 are honoured, plus `seed` for the start vector. The dense
+    krylovdim, tol, maxiter, vec0 are honoured, plus `seed` for the start vector and, for
+    method="map", `blocksize` (block Lanczos with that many vectors per step; 1, the default,
+    is the single-vector iteration KrylovKit runs). The dense
     method (host LAPACK) is not part of this package; the reference's automatic switch from
     :sparse to :dense below min_sparse_dim is therefore not taken: small subspaces stay on
     the stored-matrix path, which gives the same eigenpairs."""

@@ -270,6 +270,74 @@ def test_projected_eigh_matches_lapack(kind):
         assert np.abs(A @ Z.T - Z.T * w).max() <= 1e-13 * scale * A.shape[0]
 
 
+def _heigh_lib(A):
+    from momentumed import _lib
+    L = _lib.load()
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    w = np.empty(n)
+    Z = np.empty((n, n), dtype=np.complex128)
+    _lib.check(L.edm_projected_heigh(n, A.ctypes.data, w.ctypes.data, Z.ctypes.data))
+    return w, Z            # Z[i] = eigenvector i
+
+
+@pytest.mark.parametrize("kind", ["random", "block_arrow", "tridiag", "degenerate", "real",
+                                  "diag", "tiny"])
+def test_projected_heigh_matches_lapack(kind):
+    """The complex Hermitian Rayleigh-Ritz eigensolver (block Lanczos) against LAPACK
+    (numpy.linalg.eigh, zheevd): eigenvalues to 1e-13 of ||A||, orthonormal vectors, residuals
+    A z = w z to 1e-13 of ||A||. A real input must give the real solver's eigenvalues."""
+    rng = np.random.default_rng({"random": 11, "block_arrow": 12, "tridiag": 13,
+                                 "degenerate": 14, "real": 15, "diag": 16, "tiny": 17}[kind])
+
+    def crand(*shape):
+        return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    mats = []
+    for n in (1, 2, 3, 8, 32, 44):
+        if kind == "random":
+            B = crand(n, n); A = B + B.conj().T
+        elif kind == "block_arrow":    # kept Ritz values, a coupling block, block tridiagonal
+            A = np.zeros((n, n), dtype=complex); b = 4; k = max(0, n - 3 * b)
+            A[np.arange(k), np.arange(k)] = rng.normal(size=k)
+            for j in range(k, n, b):
+                e = min(j + b, n)
+                C = crand(e - j, e - j); A[j:e, j:e] = C + C.conj().T
+                if e < n:
+                    R = np.triu(crand(min(b, n - e), e - j))
+                    A[e:e + R.shape[0], j:e] = R; A[j:e, e:e + R.shape[0]] = R.conj().T
+            if k and k < n:
+                S = crand(min(b, n - k), k) * 1e-3
+                A[k:k + S.shape[0], :k] = S; A[:k, k:k + S.shape[0]] = S.conj().T
+        elif kind == "tridiag":
+            A = np.diag(rng.normal(size=n)).astype(complex)
+            off = crand(n - 1)
+            A += np.diag(off, -1) + np.diag(off.conj(), 1)
+        elif kind == "degenerate":     # repeated eigenvalues
+            Qm, _ = np.linalg.qr(crand(n, n))
+            ev = np.repeat(rng.normal(size=(n + 3) // 4), 4)[:n]
+            A = (Qm * ev) @ Qm.conj().T; A = 0.5 * (A + A.conj().T)
+        elif kind == "real":
+            B = rng.normal(size=(n, n)); A = (B + B.T).astype(complex)
+        elif kind == "diag":
+            A = np.diag(rng.permutation(n).astype(float)).astype(complex)
+        else:
+            A = (np.diag(rng.normal(size=n)) + 1e-3 * np.eye(n, k=1) + 1e-3 * np.eye(n, k=-1)
+                 ).astype(complex) * 1e-200
+        mats.append(A)
+    for A in mats:
+        n = A.shape[0]
+        w, Z = _heigh_lib(A)
+        ref = np.linalg.eigvalsh(A)
+        scale = max(np.abs(A).max(), 1e-300)
+        assert np.all(np.diff(w) >= 0)
+        np.testing.assert_allclose(w, ref, rtol=0, atol=1e-13 * scale * n)
+        np.testing.assert_allclose(Z.conj() @ Z.T, np.eye(n), atol=1e-13)
+        assert np.abs(A @ Z.T - Z.T * w).max() <= 1e-13 * scale * n
+        if kind == "real":
+            wr, _ = _eigh_lib(A.real)
+            np.testing.assert_allclose(w, wr, rtol=0, atol=1e-13 * scale * n)
+
+
 # ------------------------------------------------ batched / served workloads (host side)
 def test_served_stream_rules():
     from fractions import Fraction
