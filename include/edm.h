@@ -92,6 +92,13 @@ int edm_sector_sizes(edm_ctx *ctx, const edm_system *sys, int64_t *counts);
  * unranking + momentum filter + order-preserving compaction; `sector` is the index into
  * momentum_list. The handle carries rank tables for partner lookup. */
 int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm_basis **out);
+/* Every sector's basis at once, for ED_momentum_subspaces without momentum_restriction
+ * (momentum_decomposition.jl:339-413, which walks all states once on the host): one device
+ * counting pass per 64 sectors and one writing pass per 16 sectors over the occupation rank
+ * space, instead of a counting and a writing pass per sector. out[k] (k < the number of
+ * sectors in momentum_list, edm_sector_sizes order) receives sector k's basis, as
+ * edm_basis_enumerate(ctx, sys, k) would return it (also for empty sectors, D = 0). */
+int edm_basis_enumerate_all(edm_ctx *ctx, const edm_system *sys, edm_basis **out);
 /* HilbertSubspace(list) (EDCore/src/types/hilbert_subspace.jl:19-30) uploaded as-is
  * (must be sorted ascending, unique). `sys` may be NULL; when given and the list is the
  * complete sector `sector`, rank tables replace the binary search

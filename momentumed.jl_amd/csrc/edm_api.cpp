@@ -41,10 +41,9 @@ void free_basis(edm_basis *b)
     delete b;
 }
 
-void attach_rank(edm_basis *b)
+// keep freshly built rank tables only when they reproduce every state's position
+void verify_or_drop(edm_basis *b)
 {
-    b->rank_mode = 0;
-    if (!build_rank_tables(b)) return;
     if (verify_rank_tables(b)) {
         b->rank_mode = 1;
     } else {
@@ -54,6 +53,12 @@ void attach_rank(edm_basis *b)
         b->d_jlo = b->d_jhi = nullptr;
         b->d_jlo16 = nullptr;
     }
+}
+
+void attach_rank(edm_basis *b)
+{
+    b->rank_mode = 0;
+    if (build_rank_tables(b)) verify_or_drop(b);
 }
 
 }  // namespace
@@ -190,6 +195,47 @@ int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm
             throw;
         }
         *out = b;
+        return EDM_OK;
+    });
+}
+
+int edm_basis_enumerate_all(edm_ctx *ctx, const edm_system *sys, edm_basis **out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && sys && out, EDM_EINVAL, "NULL argument");
+        SystemSpec sp = SystemSpec::from(sys);
+        for (int64_t k = 0; k < sp.nsec; ++k) out[k] = nullptr;
+        std::vector<uint64_t *> states;
+        std::vector<int64_t> D;
+        enum_all_sector_states(ctx, sp, states, D);
+        try {
+            std::vector<edm_basis *> bs;
+            for (int64_t k = 0; k < sp.nsec; ++k) {
+                edm_basis *b = new edm_basis();
+                b->ctx = ctx;
+                b->nbits = sp.nbits;
+                b->has_spec = true;
+                b->spec = sp;
+                b->sector = k;
+                b->D = D[k];
+                b->d_states = states[k];
+                states[k] = nullptr;
+                out[k] = b;
+                bs.push_back(b);
+            }
+            std::vector<bool> built = build_rank_tables_many(bs);
+            for (size_t k = 0; k < bs.size(); ++k) {
+                bs[k]->rank_mode = 0;
+                if (built[k]) verify_or_drop(bs[k]);
+            }
+        } catch (...) {
+            for (int64_t q = 0; q < sp.nsec; ++q) {
+                if (out[q]) free_basis(out[q]);
+                out[q] = nullptr;
+                if (states[q]) ctx->free(states[q], (size_t)std::max<int64_t>(D[q], 1) * sizeof(uint64_t));
+            }
+            throw;
+        }
         return EDM_OK;
     });
 }

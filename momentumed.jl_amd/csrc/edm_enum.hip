@@ -26,7 +26,8 @@ constexpr int kEnumBlock = 256;
 constexpr int kMaxComp = 8;     // conserved components handled on the device
 constexpr int kMaxChunk = 8;    // 8-bit chunks per component (<= 64 bits)
 constexpr int kMaxSlots = 64;   // sector slots per counting pass
-constexpr int kMaxWriteSlots = 4; // sector slots per writing pass
+constexpr int kWriteSlotsOne = 4;   // sector slots per writing pass (one sector)
+constexpr int kWriteSlotsAll = 16;  // sector slots per writing pass (all sectors)
 
 struct EnumParams {
     int ncc, n, dim, nchunk;
@@ -161,14 +162,16 @@ k_enum_count(EnumParams p, const int *__restrict__ g_mt, const uint64_t *__restr
         block_counts[(uint64_t)blockIdx.x * p.nslots + i] = cnt[i];
 }
 
-// pass B: order-preserving write. block_off[blk*nslots+slot] = first output position of the
-// block's states of that slot (absolute, including the slot's base in the output array).
+// pass B: order-preserving write. Slot sl's states go to outs[sl]; block_off[blk*nslots+sl]
+// = first position in outs[sl] of the block's states of that slot. MAXS bounds nslots (the
+// per-thread counts of every slot live in LDS).
+template <int MAXS>
 __global__ void __launch_bounds__(kEnumBlock)
 k_enum_write(EnumParams p, const int *__restrict__ g_mt, const uint64_t *__restrict__ binom,
-             const int64_t *__restrict__ block_off, uint64_t *__restrict__ out)
+             const int64_t *__restrict__ block_off, uint64_t *const *__restrict__ outs)
 {
     __shared__ int mt[kMaxChunk * 256 * 3];
-    __shared__ uint32_t cnt[kMaxWriteSlots * kEnumBlock];
+    __shared__ uint32_t cnt[MAXS * kEnumBlock];
     load_mt(p, g_mt, mt);
     for (int i = threadIdx.x; i < p.nslots * kEnumBlock; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
@@ -204,7 +207,7 @@ k_enum_write(EnumParams p, const int *__restrict__ g_mt, const uint64_t *__restr
             int slot = state_slot(p, s, mt);
             if (slot >= 0) {
                 uint32_t &c = cnt[slot * kEnumBlock + threadIdx.x];
-                out[boff[slot] + c] = concat_state(p, s);
+                outs[slot][boff[slot] + c] = concat_state(p, s);
                 c += 1;
             }
             if (it + 1 < nh) advance(p, s);
@@ -340,6 +343,7 @@ uint64_t *enum_sector_states(edm_ctx *ctx, const SystemSpec &sp, int64_t sector,
     EnumSetup s = make_setup(ctx, sp);
     uint64_t *d_out = nullptr;
     int64_t *d_off = nullptr;
+    uint64_t **d_outs = nullptr;
     size_t obytes = 0;
     try {
         std::vector<uint32_t> h = run_count(ctx, s, sector, 1);
@@ -356,9 +360,13 @@ uint64_t *enum_sector_states(edm_ctx *ctx, const SystemSpec &sp, int64_t sector,
         EDM_HIP_CHECK(hipMemcpyAsync(d_off, off.data(), obytes, hipMemcpyHostToDevice, ctx->stream));
         s.p.slot_base = sector;
         s.p.nslots = 1;
+        d_outs = (uint64_t **)ctx->alloc(sizeof(uint64_t *));
+        EDM_HIP_CHECK(hipMemcpyAsync(d_outs, &d_out, sizeof(uint64_t *), hipMemcpyHostToDevice,
+                                     ctx->stream));
         if (run > 0) {
-            hipLaunchKernelGGL(k_enum_write, dim3((unsigned)s.nblocks), dim3(kEnumBlock), 0,
-                               ctx->stream, s.p, s.d_mt, ctx->d_binom, d_off, d_out);
+            hipLaunchKernelGGL(k_enum_write<kWriteSlotsOne>, dim3((unsigned)s.nblocks),
+                               dim3(kEnumBlock), 0, ctx->stream, s.p, s.d_mt, ctx->d_binom, d_off,
+                               d_outs);
             EDM_HIP_CHECK(hipGetLastError());
         }
         EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -366,10 +374,83 @@ uint64_t *enum_sector_states(edm_ctx *ctx, const SystemSpec &sp, int64_t sector,
         (void)hipStreamSynchronize(ctx->stream);
         if (d_out) ctx->free(d_out, (size_t)std::max<int64_t>(*D_out, 1) * sizeof(uint64_t));
         if (d_off) ctx->free(d_off, obytes);
+        if (d_outs) ctx->free(d_outs, sizeof(uint64_t *));
+        ctx->free(s.d_mt, s.mt_bytes);
+        throw;
+    }
+    ctx->free(d_outs, sizeof(uint64_t *));
+    ctx->free(d_off, obytes);
+    ctx->free(s.d_mt, s.mt_bytes);
+    return d_out;
+}
+
+void enum_all_sector_states(edm_ctx *ctx, const SystemSpec &sp, std::vector<uint64_t *> &outs,
+                            std::vector<int64_t> &D)
+{
+    EnumSetup s = make_setup(ctx, sp);
+    const int64_t nsec = sp.nsec;
+    outs.assign((size_t)nsec, nullptr);
+    D.assign((size_t)nsec, 0);
+    int64_t *d_off = nullptr;
+    uint64_t **d_outs = nullptr;
+    const size_t obytes = (size_t)s.nblocks * kWriteSlotsAll * sizeof(int64_t);
+    auto release_outs = [&]() {
+        for (int64_t k = 0; k < nsec; ++k)
+            if (outs[k]) ctx->free(outs[k], (size_t)std::max<int64_t>(D[k], 1) * sizeof(uint64_t));
+    };
+    try {
+        // one counting pass per 64 sectors: per-(block, sector) counts, then each sector's
+        // block offsets (exclusive scan over blocks) and size
+        std::vector<int64_t> off((size_t)s.nblocks * nsec);
+        for (int64_t base = 0; base < nsec; base += kMaxSlots) {
+            int ns = (int)std::min<int64_t>(kMaxSlots, nsec - base);
+            std::vector<uint32_t> h = run_count(ctx, s, base, ns);
+            for (int i = 0; i < ns; ++i) {
+                int64_t run = 0;
+                for (uint64_t b = 0; b < s.nblocks; ++b) {
+                    off[b * nsec + base + i] = run;
+                    run += h[b * ns + i];
+                }
+                D[base + i] = run;
+            }
+        }
+        for (int64_t k = 0; k < nsec; ++k)
+            outs[k] = (uint64_t *)ctx->alloc((size_t)std::max<int64_t>(D[k], 1) * sizeof(uint64_t));
+        d_off = (int64_t *)ctx->alloc(obytes);
+        d_outs = (uint64_t **)ctx->alloc(kWriteSlotsAll * sizeof(uint64_t *));
+        // one writing pass per 16 sectors
+        std::vector<int64_t> hoff((size_t)s.nblocks * kWriteSlotsAll);
+        for (int64_t base = 0; base < nsec; base += kWriteSlotsAll) {
+            int ns = (int)std::min<int64_t>(kWriteSlotsAll, nsec - base);
+            int64_t tot = 0;
+            for (int i = 0; i < ns; ++i) tot += D[base + i];
+            if (tot == 0) continue;
+            for (uint64_t b = 0; b < s.nblocks; ++b)
+                for (int i = 0; i < ns; ++i) hoff[b * ns + i] = off[b * nsec + base + i];
+            // the previous pass reads d_off / d_outs: wait before overwriting them
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            EDM_HIP_CHECK(hipMemcpyAsync(d_off, hoff.data(), (size_t)s.nblocks * ns * sizeof(int64_t),
+                                         hipMemcpyHostToDevice, ctx->stream));
+            EDM_HIP_CHECK(hipMemcpyAsync(d_outs, outs.data() + base, ns * sizeof(uint64_t *),
+                                         hipMemcpyHostToDevice, ctx->stream));
+            s.p.slot_base = base;
+            s.p.nslots = ns;
+            hipLaunchKernelGGL(k_enum_write<kWriteSlotsAll>, dim3((unsigned)s.nblocks),
+                               dim3(kEnumBlock), 0, ctx->stream, s.p, s.d_mt, ctx->d_binom, d_off,
+                               d_outs);
+            EDM_HIP_CHECK(hipGetLastError());
+        }
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        release_outs();
+        outs.clear();
+        if (d_off) ctx->free(d_off, obytes);
+        if (d_outs) ctx->free(d_outs, kWriteSlotsAll * sizeof(uint64_t *));
         ctx->free(s.d_mt, s.mt_bytes);
         throw;
     }
     ctx->free(d_off, obytes);
+    ctx->free(d_outs, kWriteSlotsAll * sizeof(uint64_t *));
     ctx->free(s.d_mt, s.mt_bytes);
-    return d_out;
 }

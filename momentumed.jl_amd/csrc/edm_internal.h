@@ -193,8 +193,15 @@ constexpr int kBatchRotChunk = 8;      // output columns per rotation job
 void enum_sector_counts(edm_ctx *ctx, const SystemSpec &sp, std::vector<int64_t> &counts);
 // enumerate one sector; returns a device array of *D states (allocated on ctx)
 uint64_t *enum_sector_states(edm_ctx *ctx, const SystemSpec &sp, int64_t sector, int64_t *D);
+// enumerate every sector at once: one counting pass per 64 sectors and one writing pass per
+// 16 sectors over the rank space; outs[k] (allocated on ctx, D[k] states) for every k < nsec
+void enum_all_sector_states(edm_ctx *ctx, const SystemSpec &sp, std::vector<uint64_t *> &outs,
+                            std::vector<int64_t> &D);
 // rank tables (edm_rank.cpp)
 bool build_rank_tables(edm_basis *b);
+// the same for many complete sectors of one system (J_lo shared, J_hi on host threads);
+// entry k tells whether bs[k] got tables
+std::vector<bool> build_rank_tables_many(const std::vector<edm_basis *> &bs);
 // matvec (edm_matvec.hip)
 enum class ApplyMode { Forward, Adjoint };
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,

@@ -14,6 +14,8 @@
 #include "edm_internal.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <unordered_map>
 
 namespace {
@@ -56,37 +58,55 @@ QKey qkey(const SystemSpec &sp, uint64_t v, int bit0)
     return q;
 }
 
-}  // namespace
+// J_lo and the per-quantum-number counts of the low field: the same for every sector of a
+// system (they do not involve the sector's momentum)
+struct LowTables {
+    int nlo = 0, nhi = 0;
+    std::vector<uint32_t> jlo;
+    std::unordered_map<QKey, uint32_t, QHash> cnt;
+};
 
-bool build_rank_tables(edm_basis *b)
+bool rank_applicable(const edm_basis *b)
 {
     if (!b->has_spec || b->sector < 0) return false;
     const SystemSpec &sp = b->spec;
     if (sp.ncc > 8) return false;
     if (b->D >= (int64_t)0xFFFFFFFFLL) return false;
-    int nbits = sp.nbits;
-    if (nbits > 44 || nbits < 2) return false;
-    int nlo = nbits / 2, nhi = nbits - nlo;
-    uint64_t nlo_vals = 1ULL << nlo, nhi_vals = 1ULL << nhi;
+    return sp.nbits <= 44 && sp.nbits >= 2;
+}
 
-    std::vector<uint32_t> jlo(nlo_vals), jhi(nhi_vals);
-    std::unordered_map<QKey, uint32_t, QHash> cnt;
-    cnt.reserve(4096);
+LowTables low_tables(const SystemSpec &sp)
+{
+    LowTables t;
+    t.nlo = sp.nbits / 2;
+    t.nhi = sp.nbits - t.nlo;
+    const uint64_t nlo_vals = 1ULL << t.nlo;
+    t.jlo.resize(nlo_vals);
+    t.cnt.reserve(4096);
     for (uint64_t v = 0; v < nlo_vals; ++v) {
         QKey q = qkey(sp, v, 0);
-        uint32_t &c = cnt[q];
-        jlo[v] = c;
+        uint32_t &c = t.cnt[q];
+        t.jlo[v] = c;
         ++c;
     }
+    return t;
+}
+
+// J_hi of one sector; false when the count differs from D (not the complete sector)
+bool high_table(const SystemSpec &sp, const LowTables &lo, int64_t sector, int64_t D,
+                std::vector<uint32_t> &jhi)
+{
+    const uint64_t nhi_vals = 1ULL << lo.nhi;
+    jhi.assign(nhi_vals, 0);
     // target quantum numbers of the sector
     QKey Q{0, {0, 0, 0}};
     for (int c = 0; c < sp.ncc; ++c) Q.counts += (uint64_t)sp.n_each[c] << (8 * c);
     int64_t K[3] = {0, 0, 0};
-    sp.sector_momentum(b->sector, K);
+    sp.sector_momentum(sector, K);
     uint64_t run = 0;
     for (uint64_t u = 0; u < nhi_vals; ++u) {
         jhi[u] = (uint32_t)run;
-        QKey qh = qkey(sp, u, nlo);
+        QKey qh = qkey(sp, u, lo.nlo);
         // residual quantum numbers needed from the low field
         QKey need{0, {0, 0, 0}};
         bool ok = true;
@@ -98,29 +118,74 @@ bool build_rank_tables(edm_basis *b)
         }
         if (!ok) continue;
         for (int d = 0; d < sp.dim; ++d) need.m[d] = residue(K[d] - qh.m[d], sp.gk[d]);
-        auto it = cnt.find(need);
-        if (it != cnt.end()) run += it->second;
+        auto it = lo.cnt.find(need);
+        if (it != lo.cnt.end()) run += it->second;
     }
-    if ((int64_t)run != b->D) return false;   // not the complete sector
+    return (int64_t)run == D;
+}
 
+void upload_tables(edm_basis *b, const LowTables &lo, const std::vector<uint32_t> &jhi)
+{
     edm_ctx *ctx = b->ctx;
-    b->nlo = nlo;
-    b->nhi = nhi;
+    const uint64_t nlo_vals = 1ULL << lo.nlo, nhi_vals = 1ULL << lo.nhi;
+    b->nlo = lo.nlo;
+    b->nhi = lo.nhi;
     b->d_jlo = (uint32_t *)ctx->alloc(nlo_vals * sizeof(uint32_t));
     b->d_jhi = (uint32_t *)ctx->alloc(nhi_vals * sizeof(uint32_t));
-    EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo, jlo.data(), nlo_vals * sizeof(uint32_t),
+    EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo, lo.jlo.data(), nlo_vals * sizeof(uint32_t),
                                  hipMemcpyHostToDevice, ctx->stream));
     EDM_HIP_CHECK(hipMemcpyAsync(b->d_jhi, jhi.data(), nhi_vals * sizeof(uint32_t),
                                  hipMemcpyHostToDevice, ctx->stream));
     uint32_t mx = 0;
-    for (uint32_t v : jlo) mx = std::max(mx, v);
+    for (uint32_t v : lo.jlo) mx = std::max(mx, v);
     std::vector<uint16_t> j16;   // must outlive the asynchronous copy below
     if (mx <= 0xFFFF) {
-        j16.assign(jlo.begin(), jlo.end());
+        j16.assign(lo.jlo.begin(), lo.jlo.end());
         b->d_jlo16 = (uint16_t *)ctx->alloc(nlo_vals * sizeof(uint16_t));
         EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo16, j16.data(), nlo_vals * sizeof(uint16_t),
                                      hipMemcpyHostToDevice, ctx->stream));
     }
     EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace
+
+bool build_rank_tables(edm_basis *b)
+{
+    if (!rank_applicable(b)) return false;
+    LowTables lo = low_tables(b->spec);
+    std::vector<uint32_t> jhi;
+    if (!high_table(b->spec, lo, b->sector, b->D, jhi)) return false;
+    upload_tables(b, lo, jhi);
     return true;
+}
+
+std::vector<bool> build_rank_tables_many(const std::vector<edm_basis *> &bs)
+{
+    std::vector<bool> ok(bs.size(), false);
+    if (bs.empty()) return ok;
+    // every basis here belongs to one system: J_lo once, J_hi per sector on host threads
+    for (const edm_basis *b : bs)
+        if (!rank_applicable(b) || b->spec.nbits != bs[0]->spec.nbits) return ok;
+    const SystemSpec &sp = bs[0]->spec;
+    LowTables lo = low_tables(sp);
+    std::vector<std::vector<uint32_t>> jhi(bs.size());
+    std::vector<char> good(bs.size(), 0);
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nth = std::min(bs.size(), hw);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < bs.size();)
+            good[k] = high_table(sp, lo, bs[k]->sector, bs[k]->D, jhi[k]);
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    for (size_t k = 0; k < bs.size(); ++k)
+        if (good[k]) {
+            upload_tables(bs[k], lo, jhi[k]);
+            ok[k] = true;
+        }
+    return ok;
 }

@@ -43,6 +43,7 @@ _SIGS = [
     ("edm_num_sectors", c_i32, [P(EdmSystem), P(c_i64)]),
     ("edm_sector_sizes", c_i32, [c_vp, P(EdmSystem), c_vp]),
     ("edm_basis_enumerate", c_i32, [c_vp, P(EdmSystem), c_i64, P(c_vp)]),
+    ("edm_basis_enumerate_all", c_i32, [c_vp, P(EdmSystem), c_vp]),
     ("edm_basis_upload", c_i32, [c_vp, c_vp, c_i64, c_i32, P(EdmSystem), c_i64, P(c_vp)]),
     ("edm_basis_size", c_i32, [c_vp, P(c_i64)]),
     ("edm_basis_download", c_i32, [c_vp, c_vp]),

@@ -140,6 +140,17 @@ class DeviceBasis:
         return cls(ctx, h)
 
     @classmethod
+    def enumerate_all(cls, ctx: Context, system) -> list:
+        """Every sector's basis (momentum_list order, empty sectors included) from one
+        device enumeration of the whole occupation space (edm_basis_enumerate_all)."""
+        n = ctypes.c_int64()
+        check(ctx.L.edm_num_sectors(ctypes.byref(system), ctypes.byref(n)), ctx.h)
+        nsec = n.value
+        hs = (ctypes.c_void_p * nsec)()
+        check(ctx.L.edm_basis_enumerate_all(ctx.h, ctypes.byref(system), hs), ctx.h)
+        return [cls(ctx, ctypes.c_void_p(h)) for h in hs]
+
+    @classmethod
     def upload(cls, ctx: Context, states: np.ndarray, nbits: int, system=None,
                sector: int = -1) -> "DeviceBasis":
         states = np.ascontiguousarray(states, dtype=np.uint64)

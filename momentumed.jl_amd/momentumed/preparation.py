@@ -146,8 +146,13 @@ def ED_momentum_subspaces(para: EDPara, N_each_component, *, momentum_restrictio
     else:
         sectors = list(range(len(mlist)))
     subspaces, ks = [], []
-    for s in sectors:
-        b = dev.DeviceBasis.enumerate(ctx, system, s)
+    if momentum_restriction:
+        bases = [dev.DeviceBasis.enumerate(ctx, system, s) for s in sectors]
+    else:   # every sector from one pass over the occupation space
+        bases = dev.DeviceBasis.enumerate_all(ctx, system)
+        if len(bases) != len(mlist):
+            raise AssertionError("device sector count differs from momentum_list")
+    for s, b in zip(sectors, bases):
         if b.D == 0:
             b.close()
             continue

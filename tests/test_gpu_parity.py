@@ -84,12 +84,45 @@ def test_enumeration_matches_oracle_and_reference(gpu_ctx, oracle_mod, entry):
     if "sizes" in entry:
         assert [c for c in counts if c] == entry["sizes"]
     j = 0
+    alls = dev.DeviceBasis.enumerate_all(gpu_ctx, sysd)
+    assert len(alls) == nsec.value
     for s in range(nsec.value):
         b = dev.DeviceBasis.enumerate(gpu_ctx, sysd, s)
-        assert b.D == counts[s]
+        assert b.D == counts[s] == alls[s].D
+        assert b.rank_mode == alls[s].rank_mode
         if b.D:
             np.testing.assert_array_equal(b.download(), lists[j])
+            np.testing.assert_array_equal(alls[s].download(), lists[j])
             j += 1
+
+
+@pytest.mark.parametrize("case", ["free_momentum_4x4_N3", "medium_3x9_N9"])
+def test_enumerate_all_sectors_in_one_pass(gpu_ctx, oracle_mod, case):
+    """edm_basis_enumerate_all against the oracle and the per-sector enumeration: 100
+    sectors with Gk = (0, 0) (two counting chunks of 64, seven writing chunks of 16, empty
+    sectors among them) and the 27 sectors of the 3x9 N=9 system."""
+    if case.startswith("free"):
+        k, gk, N = W.mesh(4, 4), (0, 0), 3
+    else:
+        k, gk, N = W.mesh(3, 9), (3, 9), 9
+    para = me.EDPara(k_list=k, Gk=gk)
+    sysd = _system(para, [N])
+    alls = dev.DeviceBasis.enumerate_all(gpu_ctx, sysd)
+    lists, ks = oracle_mod.enumerate_sectors(k, gk, 1, 1, [N])
+    mlist = me.preparation._momentum_list(para, [N])
+    assert len(alls) == len(mlist)
+    nonempty = [(mlist[s], b) for s, b in enumerate(alls) if b.D]
+    assert [kk for kk, _ in nonempty] == list(ks)
+    for (kk, b), ref in zip(nonempty, lists):
+        np.testing.assert_array_equal(b.download(), ref)
+    for s in (0, len(alls) // 2, len(alls) - 1):
+        one = dev.DeviceBasis.enumerate(gpu_ctx, sysd, s)
+        assert one.D == alls[s].D and one.rank_mode == alls[s].rank_mode
+        if one.D:
+            np.testing.assert_array_equal(one.download(), alls[s].download())
+    subs, sks = me.ED_momentum_subspaces(para, N)
+    assert sks == list(ks)
+    assert [sp.length for sp in subs] == [len(v) for v in lists]
 
 
 def test_api_momentum_subspaces(gpu_ctx, oracle_mod):
