@@ -35,9 +35,12 @@ def log(*a):
 
 
 def _kernel_name(rank_mode: int, block: int = 512) -> str:
-    """Name of the matvec kernel libedm.so launches for the bucketed path (EDM_MATVEC)."""
+    """Name of the matvec kernel libedm.so launches for the bucketed path (EDM_MATVEC, or the
+    tuner's code: rows per workgroup | 0x10000 for the LDS-amplitude kernel)."""
     rank = "RankLin" if rank_mode == 1 else "RankSearch"
     v = os.environ.get("EDM_MATVEC", "u1")
+    if v == "u1a" or (v == "u1" and block & 0x10000):
+        return f"k_matvec_ldsa<{rank}, {block & 0xFFFF if block & 0x10000 else 1024}>"
     if v == "basic":
         return f"k_matvec_fast<{rank}>"
     if v in ("hi", "lo"):
@@ -297,7 +300,7 @@ def main():
         block, tune_ms = 512, None
     else:
         block, tune_ms = lm.tune()
-        log(f"[bench] tuner: {block}-row workgroups ({tune_ms} ms per matvec)")
+        log(f"[bench] tuner: kernel {block:#x} ({tune_ms} ms per matvec)")
         block = block or 512
 
     stream = torch.cuda.ExternalStream(ctx.stream, device="cuda:0")
@@ -346,7 +349,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "mesh": [int(g) for g in w.para.Gk], "N": list(w.N_each),
                    "sector": list(w.sector), "D": D, "S_full": S, "nnz_offdiag": nnz_off.value,
-                   "parallelism": "single-gpu", "matvec_workgroup": block,
+                   "parallelism": "single-gpu", "matvec_workgroup": block & 0xFFFF,
+                   "matvec_amplitudes_in_lds": bool(block & 0x10000),
                    "tuner_ms": tune_ms},
         "roofline": roofline,
         "hbm_gbps_algorithmic": round(achieved, 1),

@@ -170,12 +170,16 @@ int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, 
 
 /* Autotuner (no reference counterpart; the reference has one GPU kernel configuration): times
  * the default matvec kernel with 256-, 512- and 1024-row workgroups on this operator and
- * basis (three launches each, on scratch vectors) and uses the fastest for every later
- * edm_matvec / edm_eigsolve on the pair. The per-row arithmetic is the same in all three, so
- * y does not change. *block = chosen size (0 when the pair runs another kernel); ms3 (may be
- * NULL) = mean milliseconds per launch at 256, 512, 1024. */
-int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *block,
-                    double *ms3);
+ * basis, and the variant that also stages the (distinct) amplitudes in LDS at the same three
+ * sizes when they fit (three launches each, on scratch vectors), and uses the fastest for
+ * every later edm_matvec / edm_eigsolve on the pair. The per-row arithmetic is the same in
+ * all six, so y does not change. *kernel = chosen rows per workgroup, | EDM_TUNE_AMP_LDS for
+ * the LDS-amplitude variant (0 when the pair runs another kernel); ms6 (may be NULL) = mean
+ * milliseconds per launch for 256, 512, 1024, then LDS-amplitude 256, 512, 1024 (-1: not
+ * run). */
+#define EDM_TUNE_AMP_LDS 0x10000
+int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel,
+                    double *ms6);
 
 /* ----------------------------------------------------------------- eigensolve (a9-a10) */
 /* krylov_map_solve(H::AbstractGPULinearMap, N; ishermitian=true, vec0, krylovdim, tol,

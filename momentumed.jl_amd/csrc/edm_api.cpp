@@ -646,16 +646,16 @@ int edm_csr_destroy(edm_csr *m)
     });
 }
 
-int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *block,
-                    double *ms3)
+int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel,
+                    double *ms6)
 {
     return guarded(ctx, [&]() -> int {
-        EDM_REQUIRE(ctx && op && b && block, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(ctx && op && b && kernel, EDM_EINVAL, "NULL argument");
         EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
-        double t[3];
-        *block = matvec_tune(ctx, op, b, t);
-        if (ms3)
-            for (int k = 0; k < 3; ++k) ms3[k] = t[k];
+        double t[6];
+        *kernel = matvec_tune(ctx, op, b, t);
+        if (ms6)
+            for (int k = 0; k < 6; ++k) ms6[k] = t[k];
         return EDM_OK;
     });
 }

@@ -133,6 +133,10 @@ struct CompactTable {
     double2 *d_diag = nullptr;      // nkeys (zero where a key has no diagonal term)
     size_t lds_bytes = 0;
     size_t off_diag = 0, off_code = 0;   // byte offsets inside the LDS image
+    // optional tail of the same image: a 16-bit index per entry into a table of distinct
+    // amplitudes (bit 15: use the complex conjugate), so the amplitudes can be staged in LDS
+    // too; lds_full = bytes of image + tail (0 when the tail does not fit in LDS)
+    size_t off_ampidx = 0, off_amps = 0, lds_full = 0;
 };
 
 struct edm_op {
@@ -140,7 +144,8 @@ struct edm_op {
     edm_ctx *ctx = nullptr;
     // cached op_conserves() results keyed by basis id
     mutable std::vector<std::pair<uint64_t, bool>> conserve_cache;
-    // matvec workgroup size per basis id chosen by matvec_tune (default 512)
+    // matvec kernel per basis id chosen by matvec_tune: rows per workgroup (default 512),
+    // | EDM_TUNE_AMP_LDS when the amplitudes are staged in LDS
     mutable std::vector<std::pair<uint64_t, int>> tuned_block;
     int nbits = 0;
     bool upper = false;
@@ -241,9 +246,11 @@ int run_eigsolve_block_op(edm_ctx *ctx, const edm_op *op, const edm_basis *b, in
 // contribution, written at rowptr[r] (edm_matvec.hip)
 void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const int64_t *d_rowptr,
                      uint64_t *d_keys);
-// time the default matvec kernel at 256/512/1024-row workgroups on this (op, basis), keep
-// the fastest for later launches; ms3 gets the three timings; returns the chosen size or 0
-int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms3);
+// time the default matvec kernel at 256/512/1024-row workgroups, and the same with the
+// amplitudes staged in LDS when the image has that tail, on this (op, basis); keep the
+// fastest for later launches. ms6 gets the six timings (-1: not run); returns the chosen
+// rows per workgroup (| EDM_TUNE_AMP_LDS for the LDS-amplitude kernel) or 0
+int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms6);
 // eigenvector consumers (edm_matvec.hip)
 void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
                            const edm_basis *ket, const double2 *x, double2 *y);

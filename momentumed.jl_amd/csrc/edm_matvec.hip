@@ -24,10 +24,20 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
+#include <utility>
 
 namespace {
 
 constexpr int kRowBlock = 256;
+
+struct PairHash {
+    size_t operator()(const std::pair<uint64_t, uint64_t> &p) const
+    {
+        uint64_t h = (p.first ^ (p.second * 0x9E3779B97F4A7C15ULL)) * 0xBF58476D1CE4E5B9ULL;
+        return (size_t)(h ^ (h >> 31));
+    }
+};
 
 struct OpDev {
     const uint32_t *keyptr;
@@ -262,6 +272,8 @@ k_verify_rank(int64_t D, const uint64_t *__restrict__ states, Rank rank,
 // the term table) and touches global memory only for the accepted entries' amplitudes, the
 // rank tables and the x gather.
 constexpr int kLdsBlock = 512;
+// LDS a workgroup may use on gfx950 (160 KiB per CU)
+constexpr size_t kLdsMaxBytes = 160 * 1024;
 constexpr int kNumXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin across them
 
 struct CompactDev {
@@ -274,7 +286,17 @@ struct CompactDev {
     int off_diag;   // byte offsets in the LDS image
     int off_code;
     int lds_bytes;
+    int off_ampidx; // amplitude tail of the image (valid when lds_full > 0)
+    int off_amps;
+    int lds_full;
 };
+
+inline CompactDev compact_dev(const CompactTable &ct)
+{
+    return CompactDev{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
+                      (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes,
+                      (int)ct.off_ampidx, (int)ct.off_amps, (int)ct.lds_full};
+}
 
 // bits p+1..q-1 (pair p<q) — parity mask of a two-orbital set (scattering.jl:7-18)
 __device__ __forceinline__ uint64_t pm_pair(int p, int q)
@@ -447,7 +469,10 @@ __device__ __forceinline__ int64_t rank_find(const RankSearch &r, uint64_t t) { 
 // One row, one candidate at a time: pair buckets (key >= 64) hold only two-body entries and
 // single-orbital buckets only one-body entries, so each bucket kind has its own loop; the
 // per-bucket part of the fermion sign is hoisted; the sign is folded into the amplitude.
-template <class Rank, bool HighFirst>
+// AmpLds: the amplitudes come from the LDS tail of the image (distinct values, with a
+// conjugation flag per entry) instead of the global per-entry table; the values are the same
+// bits, so the result is too.
+template <class Rank, bool HighFirst, bool AmpLds = false>
 __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned char *base,
                                            const CompactDev &t, const Rank &rank,
                                            const double2 *__restrict__ x, int64_t jmin,
@@ -456,6 +481,8 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
     const uint16_t *s_keyptr = (const uint16_t *)base;
     const double2 *s_diag = (const double2 *)(base + t.off_diag);
     const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
+    const uint16_t *s_ampidx = (const uint16_t *)(base + t.off_ampidx);
+    const double2 *s_amps = (const double2 *)(base + t.off_amps);
     double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
     const auto rrow = rank_row32(rank, s);
 
@@ -463,9 +490,17 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
         int64_t j = rank_find(rrow, target);
         if (RankMayMiss<Rank>::value && j < 0) return;
         if (windowed && (j < jmin || j >= jmax)) return;
-        double2 a = t.amp[e];
+        double2 a;
+        uint32_t cj = 0;
+        if constexpr (AmpLds) {
+            const uint32_t m = s_ampidx[e];
+            a = s_amps[m & 0x7FFFu];
+            cj = m >> 15;
+        } else {
+            a = t.amp[e];
+        }
         double2 xv = x[j];
-        cmac(yr, yi, flip_sign(a.x, neg), flip_sign(a.y, neg), xv);
+        cmac(yr, yi, flip_sign(a.x, neg), flip_sign(a.y, neg ^ cj), xv);
     };
     auto enter = [&](int key, int &e0, int &e1) {
         if (with_diag) {
@@ -618,6 +653,14 @@ __device__ __forceinline__ void stage_image(const CompactDev &t, uint4 *lds)
     for (int k = threadIdx.x; k < n16; k += blockDim.x) lds[k] = g[k];
 }
 
+// the image with its amplitude tail
+__device__ __forceinline__ void stage_image_full(const CompactDev &t, uint4 *lds)
+{
+    const uint4 *g = (const uint4 *)t.keyptr;
+    int n16 = t.lds_full / 16;
+    for (int k = threadIdx.x; k < n16; k += blockDim.x) lds[k] = g[k];
+}
+
 // Same traversal as k_matvec_lds, with U candidates of a bucket resolved together (U table
 // lookups in flight, then U gathers in flight), a per-row rank view, and an optional partner
 // window: only partners j in [jmin, jmax) are gathered, and the result is added to y when
@@ -642,6 +685,23 @@ k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
         v.y += o.y;
     }
     y[i] = v;
+}
+
+// k_matvec_ldsu<Rank, true, 1, BLOCK> with the amplitudes staged in LDS as well (the whole
+// image, lds_full bytes): per accepted candidate one 16-byte global load fewer. y is bitwise
+// the same.
+template <class Rank, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_matvec_ldsa(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
+              const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    extern __shared__ uint4 lds_u4[];
+    stage_image_full(t, lds_u4);
+    __syncthreads();
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= D) return;
+    y[i] = row_one<Rank, true, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
+                                     INT64_MAX, 1, false);
 }
 
 // ---------------------------------------------------------------- block matvec
@@ -1010,8 +1070,10 @@ bool verify_rank_tables(edm_basis *b)
 // time (u1) — the fastest measured on the 6x6 N=12 sector. EDM_MATVEC=u2|u4 resolves 2|4
 // candidates together; hi|lo select the LDS kernel without the per-row rank view, with
 // high- or low-bit-first pair order (EDM_XCD_SWIZZLE=1 then remaps workgroups per XCD);
-// basic forces the global-memory table kernel. EDM_MATVEC_SPLIT=n splits the u* kernels into
-// n passes over windows of x.
+// basic forces the global-memory table kernel; u1a the u1 row code with the amplitudes
+// staged in LDS too (k_matvec_ldsa, 1024-row workgroups; u1 when they do not fit).
+// EDM_MATVEC_SPLIT=n splits the u* kernels into n passes over windows of x. Without
+// EDM_MATVEC, a (operator, basis) pair that went through matvec_tune runs the tuner's pick.
 static int matvec_variant()
 {
     const char *v = getenv("EDM_MATVEC");
@@ -1020,6 +1082,7 @@ static int matvec_variant()
     if (!strcmp(v, "lo")) return 1;
     if (!strcmp(v, "hi")) return 2;
     if (!strcmp(v, "u1")) return 11;
+    if (!strcmp(v, "u1a")) return 13;
     if (!strcmp(v, "u2")) return 12;
     if (!strcmp(v, "u4")) return 14;
     return 11;
@@ -1042,8 +1105,7 @@ template <class Rank, int U>
 static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
                         const double2 *x, double2 *y, int block = kLdsBlock)
 {
-    CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
-                  (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    CompactDev cd = compact_dev(ct);
     int passes = matvec_passes();
     if (U == 1 && passes == 1 && block != kLdsBlock) {
         // workgroup size chosen by edm_matvec_tune: the same per-row code, so the same y
@@ -1068,18 +1130,52 @@ static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct
     }
 }
 
+template <class Rank, int BLOCK>
+static void launch_ldsa_block(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                              const double2 *x, double2 *y)
+{
+    static bool attr_set = false;   // LDS beyond 64 KiB needs the per-kernel opt-in
+    if (!attr_set) {
+        EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_ldsa<Rank, BLOCK>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)kLdsMaxBytes));
+        attr_set = true;
+    }
+    unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL((k_matvec_ldsa<Rank, BLOCK>), dim3(nb), dim3(BLOCK), ct.lds_full,
+                       ctx->stream, b->D, b->d_states, compact_dev(ct), rank, x, y);
+}
+
+template <class Rank>
+static void launch_ldsa(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                        const double2 *x, double2 *y, int block)
+{
+    if (block == 256) launch_ldsa_block<Rank, 256>(ctx, b, ct, rank, x, y);
+    else if (block == 512) launch_ldsa_block<Rank, 512>(ctx, b, ct, rank, x, y);
+    else launch_ldsa_block<Rank, 1024>(ctx, b, ct, rank, x, y);
+}
+
 template <class Rank>
 static void launch_compact(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
                            const double2 *x, double2 *y, int variant, int block = kLdsBlock)
 {
+    // block: rows per workgroup, | EDM_TUNE_AMP_LDS for the kernel with LDS amplitudes
+    if (variant == 13 && ct.lds_full > 0) {
+        launch_ldsa(ctx, b, ct, rank, x, y, 1024);
+        return;
+    }
+    if (variant == 11 && (block & EDM_TUNE_AMP_LDS) && ct.lds_full > 0) {
+        launch_ldsa(ctx, b, ct, rank, x, y, block & 0xFFFF);
+        return;
+    }
+    block &= 0xFFFF;
     if (variant >= 11) {
         if (variant == 12) launch_ldsu<Rank, 2>(ctx, b, ct, rank, x, y);
         else if (variant == 14) launch_ldsu<Rank, 4>(ctx, b, ct, rank, x, y);
         else launch_ldsu<Rank, 1>(ctx, b, ct, rank, x, y, block);
         return;
     }
-    CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
-                  (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    CompactDev cd = compact_dev(ct);
     unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
     int swz = xcd_swizzle_enabled();
     if (variant == 1)
@@ -1143,8 +1239,7 @@ static void launch_block_nv(edm_ctx *ctx, const edm_basis *b, const CompactTable
     }
     hipLaunchKernelGGL(k_interleave<NV>, dim3(nblk(b->D)), dim3(kRowBlock), 0, ctx->stream, b->D,
                        xp, xi);
-    CompactDev cd{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
-                  (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    CompactDev cd = compact_dev(ct);
     unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
     hipLaunchKernelGGL((k_matvec_block<Rank, NV, kLdsBlock>), dim3(nb), dim3(kLdsBlock),
                        ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, rank, xi, yp);
@@ -1197,8 +1292,7 @@ bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, doubl
     d.states = b->d_states;
     d.x = x;
     d.y = y;
-    d.t = CompactDev{ct.d_keyptr, ct.d_code, ct.d_amp, ct.d_diag, ct.nkeys, (int)ct.noff,
-                     (int)ct.off_diag, (int)ct.off_code, (int)ct.lds_bytes};
+    d.t = compact_dev(ct);
     d.use_lin = use_lin(op, b) ? 1 : 0;
     if (d.use_lin) d.lin = make_lin(b);
     d.search = make_search(b);
@@ -1263,9 +1357,9 @@ void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const i
     EDM_HIP_CHECK(hipGetLastError());
 }
 
-int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms3)
+int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms6)
 {
-    for (int k = 0; k < 3; ++k) ms3[k] = -1.0;
+    for (int k = 0; k < 6; ++k) ms6[k] = -1.0;
     if (!op->fast || !op->cfwd.ok || matvec_variant() != 11 || matvec_passes() != 1 || b->D == 0)
         return 0;
     size_t vb = (size_t)b->D * sizeof(double2);
@@ -1280,12 +1374,15 @@ int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms3)
         EDM_HIP_CHECK(hipEventCreate(&e1));
         const int blocks[3] = {256, 512, 1024};
         double bestms = 1e300;
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 6; ++k) {
+            const bool amp_lds = k >= 3;
+            if (amp_lds && op->cfwd.lds_full == 0) continue;
+            const int code = blocks[k % 3] | (amp_lds ? EDM_TUNE_AMP_LDS : 0);
             auto run = [&]() {
                 if (use_lin(op, b))
-                    launch_compact(ctx, b, op->cfwd, make_lin(b), x, y, 11, blocks[k]);
+                    launch_compact(ctx, b, op->cfwd, make_lin(b), x, y, 11, code);
                 else
-                    launch_compact(ctx, b, op->cfwd, make_search(b), x, y, 11, blocks[k]);
+                    launch_compact(ctx, b, op->cfwd, make_search(b), x, y, 11, code);
             };
             run();   // warm-up
             EDM_HIP_CHECK(hipEventRecord(e0, ctx->stream));
@@ -1294,10 +1391,10 @@ int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms3)
             EDM_HIP_CHECK(hipEventSynchronize(e1));
             float ms = 0.f;
             EDM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            ms3[k] = ms / 3.0;
-            if (ms3[k] < bestms) {
-                bestms = ms3[k];
-                best = blocks[k];
+            ms6[k] = ms / 3.0;
+            if (ms6[k] < bestms) {
+                bestms = ms6[k];
+                best = code;
             }
         }
     } catch (...) {
@@ -1515,7 +1612,38 @@ void upload_compact(edm_ctx *ctx, edm_op *op, CompactTable &t, bool adjoint)
     size_t off_code = off_diag + (size_t)nkeys * 16;
     size_t lds = (off_code + ents.size() * 2 + 15) & ~(size_t)15;
     if (lds > 64 * 1024) return;
-    std::vector<unsigned char> img(lds, 0);
+    // amplitude tail: distinct amplitudes, an adjoint entry pointing at its partner's value
+    // with the conjugation flag (exact: only the sign bit of the imaginary part differs)
+    std::vector<uint16_t> aidx(ents.size());
+    std::vector<double2> uniq;
+    {
+        std::unordered_map<std::pair<uint64_t, uint64_t>, uint32_t, PairHash> seen;
+        for (size_t q = 0; q < ents.size(); ++q) {
+            uint64_t rb, ib, nb;
+            const double nim = -ents[q].im;
+            std::memcpy(&rb, &ents[q].re, 8);
+            std::memcpy(&ib, &ents[q].im, 8);
+            std::memcpy(&nb, &nim, 8);
+            auto it = seen.find({rb, ib});
+            if (it != seen.end()) { aidx[q] = (uint16_t)it->second; continue; }
+            it = seen.find({rb, nb});
+            if (it != seen.end()) { aidx[q] = (uint16_t)(it->second | 0x8000u); continue; }
+            if (uniq.size() >= 0x8000) { uniq.clear(); break; }
+            seen[{rb, ib}] = (uint32_t)uniq.size();
+            aidx[q] = (uint16_t)uniq.size();
+            uniq.push_back(make_double2(ents[q].re, ents[q].im));
+        }
+    }
+    size_t off_ampidx = lds;
+    size_t off_amps = (off_ampidx + ents.size() * 2 + 15) & ~(size_t)15;
+    size_t lds_full = off_amps + uniq.size() * 16;
+    const bool tail = !ents.empty() && !uniq.empty() && lds_full <= kLdsMaxBytes;
+    const size_t img_bytes = tail ? lds_full : lds;
+    std::vector<unsigned char> img(img_bytes, 0);
+    if (tail) {
+        std::memcpy(img.data() + off_ampidx, aidx.data(), aidx.size() * 2);
+        std::memcpy(img.data() + off_amps, uniq.data(), uniq.size() * 16);
+    }
     uint16_t *kp = (uint16_t *)img.data();
     double2 *dg = (double2 *)(img.data() + off_diag);
     uint16_t *cd = (uint16_t *)(img.data() + off_code);
@@ -1536,11 +1664,17 @@ void upload_compact(edm_ctx *ctx, edm_op *op, CompactTable &t, bool adjoint)
     t.off_diag = off_diag;
     t.off_code = off_code;
     t.lds_bytes = lds;
-    t.d_keyptr = (uint16_t *)ctx->alloc(lds);
+    if (tail) {
+        t.off_ampidx = off_ampidx;
+        t.off_amps = off_amps;
+        t.lds_full = lds_full;
+    }
+    t.d_keyptr = (uint16_t *)ctx->alloc(img_bytes);
     t.d_diag = (double2 *)((unsigned char *)t.d_keyptr + off_diag);
     t.d_code = (uint16_t *)((unsigned char *)t.d_keyptr + off_code);
     t.d_amp = (double2 *)ctx->alloc(amp.size() * sizeof(double2));
-    EDM_HIP_CHECK(hipMemcpyAsync(t.d_keyptr, img.data(), lds, hipMemcpyHostToDevice, ctx->stream));
+    EDM_HIP_CHECK(hipMemcpyAsync(t.d_keyptr, img.data(), img_bytes, hipMemcpyHostToDevice,
+                                 ctx->stream));
     EDM_HIP_CHECK(hipMemcpyAsync(t.d_amp, amp.data(), amp.size() * sizeof(double2),
                                  hipMemcpyHostToDevice, ctx->stream));
     EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -1549,7 +1683,7 @@ void upload_compact(edm_ctx *ctx, edm_op *op, CompactTable &t, bool adjoint)
 
 void free_compact(edm_ctx *ctx, CompactTable &t)
 {
-    if (t.d_keyptr) ctx->free(t.d_keyptr, t.lds_bytes);
+    if (t.d_keyptr) ctx->free(t.d_keyptr, t.lds_full ? t.lds_full : t.lds_bytes);
     if (t.d_amp) ctx->free(t.d_amp, std::max<int64_t>(t.noff, 1) * sizeof(double2));
     t = CompactTable{};
 }

@@ -121,6 +121,9 @@ def system_struct(k_list, Gk, Nc_mix, Nc_conserve, N_each):
 
 
 # ------------------------------------------------------------------------- basis
+TUNE_AMP_LDS = 0x10000   # edm.h EDM_TUNE_AMP_LDS
+
+
 class DeviceBasis:
     """A sorted basis resident on the device (with rank tables when it is a complete sector)."""
 
@@ -256,13 +259,16 @@ class ROCmLinearMap:
             raise ValueError("Dimension of Hamiltonian linear map mismatches vector length.")
 
     def tune(self):
-        """Pick the matvec workgroup size for this operator and basis by timing (edm.h
-        edm_matvec_tune); returns (chosen rows per workgroup or 0, {size: ms per matvec})."""
-        blk = ctypes.c_int32()
-        ms = (ctypes.c_double * 3)()
-        check(self.ctx.L.edm_matvec_tune(self.ctx.h, self.op.h, self.basis.h, ctypes.byref(blk),
-                                         ms), self.ctx.h)
-        return blk.value, {256: ms[0], 512: ms[1], 1024: ms[2]}
+        """Pick the matvec kernel for this operator and basis by timing (edm.h
+        edm_matvec_tune); returns (chosen kernel code or 0, {candidate: ms per matvec}).
+        The code is the rows per workgroup, | TUNE_AMP_LDS when the amplitudes are staged in
+        LDS; candidates are named 256, 512, 1024 and "lds256", "lds512", "lds1024"."""
+        code = ctypes.c_int32()
+        ms = (ctypes.c_double * 6)()
+        check(self.ctx.L.edm_matvec_tune(self.ctx.h, self.op.h, self.basis.h,
+                                         ctypes.byref(code), ms), self.ctx.h)
+        names = [256, 512, 1024, "lds256", "lds512", "lds1024"]
+        return code.value, {n: ms[k] for k, n in enumerate(names) if ms[k] >= 0}
 
     def matvec_device(self, y_ptr: int, x_ptr: int):
         """Stream-ordered y = H x on raw device pointers (no synchronisation)."""

@@ -1,16 +1,21 @@
 """Lowest-Landau-level torus interaction — TEST INFRASTRUCTURE ONLY (see oracle.py header).
 
-Restatement of the reference example's LandauInteraction for n = 0 levels
+Restatement of the reference example's LandauInteraction
 (examples/landau level torus/Landau level torus.jl): ReciprocalLattice (:35-60), BZ fold
 (:121-141), V_Coulomb (:188-211), V_Haldane (:217-232), and the callable (:243-347) with the
-magnetic-translation phases. Only level index 0 is supported, where the form factor
-level_form_factor(0, 0, ql) (:157-173) is exp(-|ql|^2/4) exactly (laguerrel(0, 0, x) = 1), so
-no third-party special functions are needed.
+magnetic-translation phases. Scattering within one Landau level n is restated (n_f = n_i for
+both form factors): level_form_factor(n, n, ql) (:157-173) is exp(-|ql|^2/4) L_n(|ql|^2/2)
+with the ordinary Laguerre polynomial L_n = laguerrel(n, 0, .) of ClassicalOrthogonalPolynomials
+(absent here; computed by its three-term recurrence, pinned against the closed forms of
+L_0..L_3); at n = 0 the factor is exp(-|ql|^2/4) exactly. Level-changing form factors are not
+restated.
 
 Used to reproduce numbers the reference's notebooks print: the 1734 upper two-body terms of the
 4x6, Ne=8 triangular Coulomb system ("4. cuda support.ipynb"), the lowest energies of the
-3x5, Ne=5 sector (0,0) ("1.FQH_Laughlin.ipynb"), and the bilayer Halperin energies, term count
-and many-body Chern numbers ("2.BilayerFQH_Halperin.ipynb"; orbital_overlap, :352-393).
+3x5, Ne=5 sector (0,0) ("1.FQH_Laughlin.ipynb"), the bilayer Halperin energies, term count
+and many-body Chern numbers ("2.BilayerFQH_Halperin.ipynb"; orbital_overlap, :352-393), and
+the n = 2 Landau-level charge-density-wave system ("1.CDW.ipynb": term count, sector sizes,
+energies).
 """
 from __future__ import annotations
 
@@ -106,7 +111,7 @@ def _cross(a, b1, b2):
 
 
 class LandauInteraction:
-    """Landau level torus.jl:243-347 for components (layer, level=0, Chern, other)."""
+    """Landau level torus.jl:243-347 for components (layer, level, Chern, other)."""
 
     def __init__(self, lattice: ReciprocalLattice, *components):
         self.lattice = lattice
@@ -119,17 +124,20 @@ class LandauInteraction:
         self.V_inter = [1.5, 0.0]
         self.mix = 0.0
         self.layer = [c[0] for c in components]
-        assert all(c[1] == 0 for c in components), "only level index 0 is restated"
+        self.level = [c[1] for c in components]
         self.chern = [c[2] for c in components]
         self.other = [c[3] for c in components]
 
     def __call__(self, kf1, kf2, ki2, ki1, cf1=1, cf2=1, ci2=1, ci1=1) -> complex:
         rd = lambda v: (v[cf1 - 1], v[cf2 - 1], v[ci2 - 1], v[ci1 - 1])  # noqa: E731
         lf1, lf2, li2, li1 = rd(self.layer)
+        nf1, nf2, ni2, ni1 = rd(self.level)
         Cf1, Cf2, Ci2, Ci1 = rd(self.chern)
         sf1, sf2, si2, si1 = rd(self.other)
         if li1 != lf1 or li2 != lf2 or Ci1 != Cf1 or Ci2 != Cf2 or si1 != sf1 or si2 != sf2:
             return 0j
+        if nf1 != ni1 or nf2 != ni2:
+            raise NotImplementedError("level-changing form factors are not restated")
         kf1 = tuple(Fraction(v) for v in kf1)
         kf2 = tuple(Fraction(v) for v in kf2)
         ki1 = tuple(Fraction(v) for v in ki1)
@@ -159,7 +167,9 @@ class LandauInteraction:
                     V += self.mix * v_haldane(aql, same, self.V_intra, self.V_inter)
                 if self.mix != 1.0:
                     V += (1 - self.mix) * v_coulomb(aql, same, self.d_l, self.D_l)
-                F = math.exp(-aql * aql / 4.0) ** 2      # two n=0 form factors
+                F = math.exp(-aql * aql / 4.0) ** 2      # two form factors, exp part
+                if ni1 or ni2:                           # Laguerre parts (1 at n = 0)
+                    F *= _laguerre0(ni1, aql * aql / 2.0) * _laguerre0(ni2, aql * aql / 2.0)
                 ang = float(-Ci1 * c_i1f1)
                 ang += -Ci1 * (float(s1[0]) * qq2 - float(s1[1]) * qq1)
                 ang += float(-Ci2 * c_i2f2)

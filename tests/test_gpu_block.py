@@ -209,3 +209,23 @@ def test_block_eigsolve_argument_errors(gpu_ctx):
         dev.krylov_map_solve(H, 4, blocksize=9)
     with pytest.raises(NotImplementedError):
         me.EDsolve(sp, w.operator(), N=4, method="sparse", blocksize=4)
+
+
+@pytest.mark.parametrize("how", ["sparse", "map", "map_block4"])
+def test_cdw_reference_energies_with_fourfold_level(gpu_ctx, how):
+    """1.CDW.ipynb (n = 2 Landau level, 6x4 mesh, Ne = 6): the 10 printed E/Nk of momentum
+    (0, 1), which hold a two-fold and a four-fold level, from the device solvers."""
+    import llt
+    g = GOLDEN["cdw"]
+    V = llt.LandauInteraction(llt.ReciprocalLattice(*g["lattice_ratio_cos"]),
+                              *[tuple(c) for c in g["components"]])
+    V.D_l, V.mix = g["D_l"], g["mix"]
+    para = me.EDPara(k_list=np.array(g["k_list"], dtype=np.int64), Gk=tuple(g["Gk"]), H_two=V)
+    subspaces, ks = me.ED_momentum_subspaces(para, g["N"])
+    sp = subspaces[ks.index(tuple(g["sector"]))]
+    H = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    kw = {"method": "sparse"} if how == "sparse" else {"method": "map"}
+    if how == "map_block4":
+        kw["blocksize"] = 4
+    E, V_ = me.EDsolve(sp, H, N=10, method_info=False, **kw)
+    np.testing.assert_allclose(E / para.Nk, g["energies_per_Nk"], rtol=1e-10, atol=0)

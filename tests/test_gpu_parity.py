@@ -220,7 +220,7 @@ def test_matvec_matches_oracle(gpu_ctx, oracle_mod, name):
         np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b2, x), y)
 
 
-@pytest.mark.parametrize("variant", ["basic", "lo", "hi", "u1", "u2", "u4"])
+@pytest.mark.parametrize("variant", ["basic", "lo", "hi", "u1", "u2", "u4", "u1a"])
 @pytest.mark.parametrize("swizzle", ["0", "1"])
 def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizzle):
     """Every bucketed-kernel variant (global table; LDS table with low- or high-bit-first pair
@@ -237,8 +237,9 @@ def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizz
 
 
 def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
-    """The autotuner only changes the workgroup size: y is bitwise the same before and after,
-    for whichever size it picks, and every size gives the same y."""
+    """The autotuner only changes the workgroup size and where the amplitudes are read from:
+    y is bitwise the same before and after, for whichever kernel it picks, and the
+    LDS-amplitude kernel gives the same y as the global-table one."""
     w = W.medium()
     op = w.operator()
     dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, w.para.bits)
@@ -247,15 +248,22 @@ def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
     x = _rand(b.D, 77)
     y0 = _gpu_matvec(gpu_ctx, dop, b, x)
     blk, ms = H.tune()
-    print(f"[tune] medium: chose {blk}, ms {ms}", flush=True)
-    assert blk in (256, 512, 1024) and all(v > 0 for v in ms.values())
+    print(f"[tune] medium: chose {blk:#x}, ms {ms}", flush=True)
+    assert blk & 0xFFFF in (256, 512, 1024) and all(v > 0 for v in ms.values())
+    assert len(ms) == 6     # the medium operator's amplitudes fit in LDS
     np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b, x), y0)
+    import os
+    os.environ["EDM_MATVEC"] = "u1a"
+    try:
+        np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b, x), y0)
+    finally:
+        del os.environ["EDM_MATVEC"]
     part = dev.DeviceBasis.upload(gpu_ctx, b.download()[::5].copy(), w.para.bits)
     Hp = dev.ROCmLinearMap(dop, part)
     xp = _rand(part.D, 78)
     yp0 = _gpu_matvec(gpu_ctx, dop, part, xp)
     blk2, _ = Hp.tune()
-    assert blk2 in (256, 512, 1024)
+    assert blk2 & 0xFFFF in (256, 512, 1024)
     np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, part, xp), yp0)
 
 

@@ -7,6 +7,7 @@ brute force. A planted-mistake test shows the Jordan-Wigner check catches a sign
 import itertools
 import json
 import os
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -221,6 +222,53 @@ def test_bilayer_term_count_reference_notebook(oracle_mod):
     para = _bilayer_para(GOLDEN["bilayer_halperin"])
     op = MBOperator(ED_scatterlist_twobody(para), upper_hermitian=True)
     assert len(op.scats) == g["upper_twobody_terms"]
+
+
+def _cdw_para():
+    import llt
+    g = GOLDEN["cdw"]
+    V = llt.LandauInteraction(llt.ReciprocalLattice(*g["lattice_ratio_cos"]),
+                              *[tuple(c) for c in g["components"]])
+    V.D_l, V.mix = g["D_l"], g["mix"]
+    return EDPara(k_list=np.array(g["k_list"], dtype=np.int64), Gk=tuple(g["Gk"]), H_two=V)
+
+
+def test_laguerre_closed_forms():
+    """The recurrence behind the level-n form factor against L_0..L_3 written out."""
+    import llt
+    for x in (0.0, 0.3, 1.7, 6.25, 40.0):
+        assert llt._laguerre0(0, x) == 1.0
+        assert abs(llt._laguerre0(1, x) - (1 - x)) <= 1e-15 * max(1, x)
+        assert abs(llt._laguerre0(2, x) - (1 - 2 * x + x * x / 2)) <= 1e-14 * max(1, x * x)
+        l3 = 1 - 3 * x + 1.5 * x * x - x ** 3 / 6
+        assert abs(llt._laguerre0(3, x) - l3) <= 1e-13 * max(1, x ** 3)
+
+
+def test_cdw_term_count_reference_notebook(oracle_mod):
+    """1.CDW.ipynb: the n = 2 Landau level on the 6x4 mesh (ratio 0.77 square-angle lattice,
+    k_list ordered k2 fastest) prints 1734 upper two-body terms."""
+    para = _cdw_para()
+    op = MBOperator(ED_scatterlist_twobody(para), upper_hermitian=True)
+    assert len(op.scats) == GOLDEN["cdw"]["upper_twobody_terms"]
+    import llt
+    V0 = llt.LandauInteraction(llt.ReciprocalLattice(0.77, 0), (1, 0, 1, 0))
+    k = ((0, 0), (Fraction(1, 6), 0), (0, Fraction(1, 4)), (Fraction(1, 6), Fraction(1, 4)))
+    # the level-2 amplitude differs from the level-0 one (the Laguerre factor enters)
+    assert abs(para.H_two(*k) - V0(*k)) > 1e-3
+
+
+def test_cdw_energies_reference_notebook(oracle_mod):
+    """1.CDW.ipynb prints the 10 lowest E/Nk of subspace 7 (momentum (0, 1), D = 5620),
+    with a two-fold and a four-fold level: the oracle's dense matrix gives them to 1e-10."""
+    import scipy.linalg as sl
+    g = GOLDEN["cdw"]
+    para = _cdw_para()
+    lists, ks, full = _dense_from_terms(oracle_mod, para, [g["N"]], [ED_scatterlist_twobody(para)])
+    basis = lists[ks.index(tuple(g["sector"]))]
+    assert len(basis) == 5620
+    e = sl.eigh(oracle_mod.dense_matrix(basis, full), eigvals_only=True,
+                subset_by_index=[0, 9], driver="evr") / para.Nk
+    np.testing.assert_allclose(e, g["energies_per_Nk"], rtol=1e-10, atol=0)
 
 
 def test_orbital_overlap_closed_forms():
