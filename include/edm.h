@@ -214,6 +214,15 @@ int edm_eigsolve_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32
  * bra == ket it is edm_matvec. One- and two-body number-conserving terms. */
 int edm_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
                        const edm_basis *ket, const void *x_dev, void *y_dev);
+/* ED_bracket(bra, op2, op1, ket) / ED_bracket_threaded (EDCore/src/multiplication.jl:
+ * 170-183, 214-223, 286-307): <bra| op2 op1 |ket> with both operators' full term lists
+ * (an upper-Hermitian operator includes the adjoints of its off-diagonal terms, as
+ * hermitian_complete does), applied from the left to every ket state in turn; final states
+ * outside the bra basis are dropped. bra and ket may be different subspaces (e.g. a density
+ * operator pair rho(-q) rho(q)); no intermediate subspace is needed. Any term shape.
+ * out[0..1] = the complex result. */
+int edm_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
+                 const edm_basis *ket, const void *bra_dev, const void *ket_dev, double *out);
 /* PES_1rdm (MomentumED/src/analysis/particle_reduced_density_matrix.jl:4-33): for each of
  * npairs (i, j) (0-based orbitals, pairs[2p], pairs[2p+1]) out[2p..2p+1] = <psi|c_i^+ c_j|psi>
  * for the device vector psi in basis b. On a complete sector, pairs that do not conserve it

@@ -732,6 +732,38 @@ int edm_rdm1(edm_ctx *ctx, const edm_basis *b, const void *psi_dev, int32_t npai
     });
 }
 
+int edm_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
+                 const edm_basis *ket, const void *bra_dev, const void *ket_dev, double *out)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op2 && op1 && bra && ket && out, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op2->ctx == ctx && op1->ctx == ctx && bra->ctx == ctx && ket->ctx == ctx,
+                    EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(op1->nbits == ket->nbits && op2->nbits == bra->nbits && bra->nbits == ket->nbits,
+                    EDM_EDIM, "operators and bases have different bits");
+        out[0] = out[1] = 0.0;
+        if (ket->D == 0 || bra->D == 0) return EDM_OK;
+        EDM_REQUIRE(bra_dev && ket_dev, EDM_EINVAL, "NULL vector");
+        double2 *d_out = (double2 *)ctx->alloc(sizeof(double2));
+        double2 h = make_double2(0.0, 0.0);
+        try {
+            launch_bracket2(ctx, op2, op1, bra, ket, (const double2 *)bra_dev,
+                            (const double2 *)ket_dev, d_out);
+            EDM_HIP_CHECK(hipMemcpyAsync(&h, d_out, sizeof(double2), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+            EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->free(d_out, sizeof(double2));
+            throw;
+        }
+        ctx->free(d_out, sizeof(double2));
+        out[0] = h.x;
+        out[1] = h.y;
+        return EDM_OK;
+    });
+}
+
 int edm_step_inner_prod(edm_ctx *ctx, const edm_basis *b, int32_t g, const void *const *vf_dev,
                         const void *const *vi_dev, const double *orbital_ip, double *out)
 {

@@ -256,6 +256,10 @@ void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
                            const edm_basis *ket, const double2 *x, double2 *y);
 void launch_rdm1(edm_ctx *ctx, const edm_basis *b, const double2 *psi, const int2 *d_pairs,
                  int npairs, bool lin, double2 *d_out);
+// d_out[0] = <bra| op2 op1 |ket> (generic term lists, binary search in the bra basis)
+void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
+                     const edm_basis *ket, const double2 *d_bra, const double2 *d_ket,
+                     double2 *d_out);
 void launch_step_gram(edm_ctx *ctx, const edm_basis *b, const double2 *d_u,
                       const double2 *const *d_vf, const double2 *const *d_vi, int g, double2 *d_out);
 struct edm_csr {

@@ -971,6 +971,67 @@ k_step_gram(int64_t D, const uint64_t *__restrict__ states, const double2 *__res
     if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
 }
 
+// <bra| O2 O1 |ket> (ED_bracket with two operators, EDCore/src/multiplication.jl:170-183,
+// 214-223): each thread takes ket states, applies every term of O1 and then every term of O2
+// from the left (multiplication.jl:19-40: in ⊆ state, mid = state & ~in, out ∩ mid = ∅,
+// sign (-1)^popcount(mid & parity)), looks the final state up in the bra basis (binary
+// search: it may lie outside any sector) and accumulates conj(bra_i) amp2 amp1 ket_j. No
+// intermediate space is formed. Cost per ket state: S1 + (applicable O1 terms) * S2 tests.
+__device__ __forceinline__ bool apply_left(uint64_t s, uint64_t out, uint64_t in, uint64_t par,
+                                           uint64_t &t, uint32_t &neg)
+{
+    if ((s & in) != in) return false;
+    const uint64_t mid = s & ~in;
+    if (mid & out) return false;
+    t = mid | out;
+    neg = (uint32_t)__popcll(mid & par) & 1u;
+    return true;
+}
+
+struct TermsDev {
+    int64_t S;
+    const double2 *amp;
+    const uint64_t *out, *in, *par;
+};
+
+__global__ void __launch_bounds__(kRedBlock)
+k_bracket2(int64_t Dk, const uint64_t *__restrict__ kstates, const double2 *__restrict__ ket,
+           RankSearch bra_rank, const double2 *__restrict__ bra, TermsDev o2, TermsDev o1,
+           int64_t rows_per_chunk, double2 *__restrict__ partial)
+{
+    __shared__ double2 sh[kRedBlock / 64];
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = r0 + rows_per_chunk < Dk ? r0 + rows_per_chunk : Dk;
+    double ar = 0.0, ai = 0.0;
+    for (int64_t j = r0 + threadIdx.x; j < r1; j += kRedBlock) {
+        const uint64_t s = kstates[j];
+        const double2 kv = ket[j];
+        for (int64_t t1 = 0; t1 < o1.S; ++t1) {
+            uint64_t m;
+            uint32_t n1;
+            if (!apply_left(s, o1.out[t1], o1.in[t1], o1.par[t1], m, n1)) continue;
+            const double2 a1 = o1.amp[t1];
+            const double a1r = flip_sign(a1.x, n1), a1i = flip_sign(a1.y, n1);
+            for (int64_t t2 = 0; t2 < o2.S; ++t2) {
+                uint64_t f;
+                uint32_t n2;
+                if (!apply_left(m, o2.out[t2], o2.in[t2], o2.par[t2], f, n2)) continue;
+                const int64_t i = bra_rank(f);
+                if (i < 0) continue;
+                const double2 a2 = o2.amp[t2];
+                const double a2r = flip_sign(a2.x, n2), a2i = flip_sign(a2.y, n2);
+                const double pr = a2r * a1r - a2i * a1i, pi = a2r * a1i + a2i * a1r;
+                const double2 b = bra[i];
+                const double cr = b.x * pr + b.y * pi, ci = b.x * pi - b.y * pr;   // conj(b) p
+                ar += cr * kv.x - ci * kv.y;
+                ai += cr * kv.y + ci * kv.x;
+            }
+        }
+    }
+    double2 t = block_sum_fixed(ar, ai, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
 // out[p] = sum over chunks of partial[p * nchunks + c], in chunk order
 __global__ void __launch_bounds__(kRedBlock)
 k_sum_chunks(const double2 *__restrict__ partial, int64_t nchunks, double2 *__restrict__ out)
@@ -1460,6 +1521,22 @@ void launch_rdm1(edm_ctx *ctx, const edm_basis *b, const double2 *psi, const int
                            b->d_states, make_search(b), psi, d_pairs, rows, part);
     hipLaunchKernelGGL(k_sum_chunks, dim3((unsigned)npairs), dim3(kRedBlock), 0, ctx->stream, part,
                        nch, d_out);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
+                     const edm_basis *ket, const double2 *d_bra, const double2 *d_ket,
+                     double2 *d_out)
+{
+    const int64_t D = ket->D;
+    const int64_t rows = chunk_rows(D, 1);
+    const int64_t nch = std::max<int64_t>(1, (D + rows - 1) / rows);
+    double2 *part = (double2 *)ctx->partials((size_t)nch * sizeof(double2));
+    TermsDev t2{op2->S_full, op2->d_amp, op2->d_out, op2->d_in, op2->d_par};
+    TermsDev t1{op1->S_full, op1->d_amp, op1->d_out, op1->d_in, op1->d_par};
+    hipLaunchKernelGGL(k_bracket2, dim3((unsigned)nch), dim3(kRedBlock), 0, ctx->stream, D,
+                       ket->d_states, d_ket, make_search(bra), d_bra, t2, t1, rows, part);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kRedBlock), 0, ctx->stream, part, nch, d_out);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

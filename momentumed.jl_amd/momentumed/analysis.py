@@ -1,7 +1,8 @@
 """Eigenvector consumers on the device: brackets, the one-body density matrix, and the
 kshift step overlaps of many-body Berry connections.
 
-Mirrors EDCore/src/multiplication.jl:153-307 (ED_bracket, ED_bracket_threaded),
+Mirrors EDCore/src/multiplication.jl:153-307 (ED_bracket, ED_bracket_threaded, one- and
+two-operator forms),
 MomentumED/src/analysis/particle_reduced_density_matrix.jl:4-33 (PES_1rdm) and
 MomentumED/src/analysis/manybody_quantum_geometry.jl:1-91 (ED_connection_gaugefixing!,
 ED_step_inner_prod). The vectors are MBS64Vector (host arrays bound to a subspace); each call
@@ -32,12 +33,13 @@ def ED_bracket(bra: MBS64Vector, op: MBOperator, *rest):
     operator's full list (an upper-Hermitian operator includes the adjoints of its
     off-diagonal terms). bra and ket may live in different subspaces (then O maps the ket's
     subspace into the bra's; the reference assumes one subspace for upper-Hermitian
-    operators). The two-operator form <bra|op2 op1|ket> needs an intermediate space that
-    holds every state op1 reaches; it is not provided."""
+    operators). ED_bracket(bra, op2, op1, ket) (multiplication.jl:281-291) is
+    <bra|op2 op1|ket>: both full lists applied from the left to each ket state on the device,
+    final states looked up in the bra's subspace (edm_bracket2)."""
+    if len(rest) == 2:
+        return _bracket2(bra, op, rest[0], rest[1])
     if len(rest) != 1:
-        raise NotImplementedError("ED_bracket(bra, op2, op1, ket) is not provided on the "
-                                  "device; apply op1 with a subspace that holds its targets "
-                                  "and use ED_bracket(bra, op2, ket')")
+        raise TypeError("ED_bracket(bra, op, ket) or ED_bracket(bra, op2, op1, ket)")
     ket = rest[0]
     if bra.space.bits != ket.space.bits:
         raise AssertionError("bra and ket have different bits")
@@ -53,6 +55,26 @@ def ED_bracket(bra: MBS64Vector, op: MBOperator, *rest):
     check(ctx.L.edm_matvec_between(ctx.h, dop.h, bb.h, kb.h, x.data_ptr(), y.data_ptr()), ctx.h)
     ctx.synchronize()
     return complex(torch.vdot(_dev(bra.vec, ctx), y).item())
+
+
+def _bracket2(bra: MBS64Vector, op2: MBOperator, op1: MBOperator, ket: MBS64Vector) -> complex:
+    if bra.space.bits != ket.space.bits:
+        raise AssertionError("bra and ket have different bits")
+    import torch
+    ctx = get_context(0)
+    bb = device_basis_of(bra.space, ctx)
+    kb = bb if ket.space is bra.space else device_basis_of(ket.space, ctx)
+    ops = []
+    for op in (op2, op1):
+        amp, out, inn = op.arrays()
+        ops.append(DeviceOperator(ctx, bra.space.bits, amp, out, inn, op.upper_hermitian))
+    xb = _dev(bra.vec, ctx)
+    xk = _dev(ket.vec, ctx)
+    torch.cuda.current_stream(xb.device).synchronize()
+    res = np.zeros(2)
+    check(ctx.L.edm_bracket2(ctx.h, ops[0].h, ops[1].h, bb.h, kb.h, xb.data_ptr(),
+                             xk.data_ptr(), res.ctypes.data), ctx.h)
+    return complex(res[0], res[1])
 
 
 ED_bracket_threaded = ED_bracket

@@ -349,3 +349,41 @@ ORC_EXPORT int64_t orc_num_sectors(int dim, int nk, int ncc, const int64_t *k_li
     }
     return nsec;
 }
+
+/* EDCore/src/multiplication.jl:170-183,214-223 — <bra| op2 op1 |ket> with both operators as
+ * explicit (full) term lists: for every (scat2, scat1) pair and every ket state j,
+ * (amp, f) = scat2 * (scat1 * state_j) (left application, multiplication.jl:19-40); f is looked
+ * up in the bra basis (dropped when absent) and conj(bra[i]) * amp * ket[j] is accumulated, in
+ * the reference's loop order (scat2 outer, scat1 inner, states innermost). out[0..1] = sum. */
+ORC_EXPORT void orc_bracket2(int64_t Db, const uint64_t *bbasis, const double *bra,
+                             int64_t Dk, const uint64_t *kbasis, const double *ket,
+                             int64_t S2, const double *amp2, const uint64_t *out2,
+                             const uint64_t *in2, const uint64_t *par2,
+                             int64_t S1, const double *amp1, const uint64_t *out1,
+                             const uint64_t *in1, const uint64_t *par1, double *out)
+{
+    double sr = 0.0, si = 0.0;
+    for (int64_t t2 = 0; t2 < S2; ++t2)
+        for (int64_t t1 = 0; t1 < S1; ++t1)
+            for (int64_t j = 0; j < Dk; ++j) {
+                uint64_t m, f;
+                int n1, n2;
+                if (!orc_apply_left(kbasis[j], out1[t1], in1[t1], par1[t1], &m, &n1)) continue;
+                if (!orc_apply_left(m, out2[t2], in2[t2], par2[t2], &f, &n2)) continue;
+                double a1r = n1 ? -amp1[2 * t1] : amp1[2 * t1];
+                double a1i = n1 ? -amp1[2 * t1 + 1] : amp1[2 * t1 + 1];
+                double a2r = n2 ? -amp2[2 * t2] : amp2[2 * t2];
+                double a2i = n2 ? -amp2[2 * t2 + 1] : amp2[2 * t2 + 1];
+                double ar = a2r * a1r - a2i * a1i, ai = a2r * a1i + a2i * a1r;   /* amp2 * amp1 */
+                if (ar == 0.0 && ai == 0.0) continue;
+                int64_t i = orc_search(bbasis, Db, f);
+                if (i < 0) continue;
+                double br = bra[2 * i], bi = -bra[2 * i + 1];                   /* conj(bra) */
+                double kr = ket[2 * j], ki = ket[2 * j + 1];
+                double cr = br * ar - bi * ai, ci = br * ai + bi * ar;
+                sr += cr * kr - ci * ki;
+                si += cr * ki + ci * kr;
+            }
+    out[0] = sr;
+    out[1] = si;
+}

@@ -60,6 +60,9 @@ def lib() -> ctypes.CDLL:
         L.orc_enumerate.restype = i64
         L.orc_num_sectors.argtypes = [i32, i32, i32, vp, vp, vp]
         L.orc_num_sectors.restype = i64
+        L.orc_bracket2.argtypes = [i64, vp, vp, i64, vp, vp, i64, vp, vp, vp, vp, i64, vp, vp,
+                                   vp, vp, vp]
+        L.orc_bracket2.restype = None
         _ = dbl
         _LIB = L
     return _LIB
@@ -201,6 +204,19 @@ def connectivity(basis: np.ndarray, full: TermList, row0: int = 0, nrows: int | 
                        _p(full.parity), row0, nrows, 0, _p(rowptr), _p(cols), _p(terms),
                        _p(negs), nthreads)
     return rowptr, cols[:nnz], terms[:nnz], negs[:nnz]
+
+
+def bracket2(bra_basis, bra, full2: TermList, full1: TermList, ket_basis, ket) -> complex:
+    """multiplication.jl:170-183,214-223: <bra| op2 op1 |ket> over explicit term lists."""
+    bb = np.ascontiguousarray(bra_basis, dtype=np.uint64)
+    kb = np.ascontiguousarray(ket_basis, dtype=np.uint64)
+    bv = np.ascontiguousarray(bra, dtype=np.complex128)
+    kv = np.ascontiguousarray(ket, dtype=np.complex128)
+    out = np.zeros(2)
+    lib().orc_bracket2(len(bb), _p(bb), _p(bv), len(kb), _p(kb), _p(kv), len(full2),
+                       _p(full2.amp), _p(full2.out), _p(full2.inn), _p(full2.parity), len(full1),
+                       _p(full1.amp), _p(full1.out), _p(full1.inn), _p(full1.parity), _p(out))
+    return complex(out[0], out[1])
 
 
 def dense_matrix(basis: np.ndarray, full: TermList) -> np.ndarray:

@@ -72,6 +72,25 @@ def test_bracket_between_sectors(gpu_ctx, oracle_mod):
                          me.MBS64Vector(b, ket_sp)) == 0
 
 
+def test_bracket_two_operators(gpu_ctx, oracle_mod):
+    """ED_bracket(bra, op2, op1, ket) on the device against the oracle's restatement (itself
+    pinned against dense products): density operators moving momentum between sectors, and
+    a two-body upper-Hermitian op2 with the bra spanning all N-particle states."""
+    import test_oracle as T
+    for bra_b, bra, op2, op1, ket_b, ket, ref in T._bracket2_cases(oracle_mod):
+        print(f"[bracket2] {len(op2.scats)} x {len(op1.scats)} terms, D {len(bra_b)} / "
+              f"{len(ket_b)}", flush=True)
+        bsp = me.HilbertSubspace(bra_b, bits=12)
+        ksp = me.HilbertSubspace(ket_b, bits=12)
+        got = me.ED_bracket(me.MBS64Vector(bra, bsp), op2, op1, me.MBS64Vector(ket, ksp))
+        orc = oracle_mod.bracket2(bra_b, bra, _oracle_full(oracle_mod, op2),
+                                  _oracle_full(oracle_mod, op1), ket_b, ket)
+        assert abs(got - orc) <= 1e-12 * max(1.0, abs(orc))
+        assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+        assert me.ED_bracket_threaded(me.MBS64Vector(bra, bsp), op2, op1,
+                                      me.MBS64Vector(ket, ksp)) == got
+
+
 def test_rdm1_closed_forms_and_oracle(gpu_ctx, oracle_mod):
     w = W.small()
     op = w.operator()

@@ -298,6 +298,73 @@ def test_laughlin_energies_reference_notebook(oracle_mod):
     np.testing.assert_allclose(e, g["values"], rtol=1e-10, atol=0)
 
 
+# ------------------------------------------------------------------ two-operator bracket
+def _density_op(q, f, nx=3, ny=4):
+    """sum_k f[k] c^+_{k+q} c_k on an nx x ny mesh (orbital = kx + nx*ky + 1)."""
+    from momentumed import Scatter
+    terms = []
+    for kx in range(nx):
+        for ky in range(ny):
+            a = (kx + q[0]) % nx + nx * ((ky + q[1]) % ny) + 1
+            b = kx + nx * ky + 1
+            terms.append(Scatter(complex(f[b - 1]), a, b, bits=nx * ny))
+    return MBOperator(terms, upper_hermitian=False)
+
+
+def _bracket2_cases(O):
+    """(bra basis, bra, op2, op1, ket basis, ket, dense reference) on a 3x4 mesh, N = 3."""
+    from momentumed import Scatter
+    rng = np.random.default_rng(21)
+    bits, N = 12, 3
+    F = np.array([s for s in range(1 << bits) if bin(s).count("1") == N], dtype=np.uint64)
+    lists, ks = O.enumerate_sectors(mesh(3, 4), (3, 4), 1, 1, [N])
+
+    def cvec(n):
+        return rng.normal(size=n) + 1j * rng.normal(size=n)
+
+    def full(op):
+        amp, out, inn = op.arrays()
+        return O.full_list(O.TermList(amp, out, inn), op.upper_hermitian)
+    rho1 = _density_op((1, 0), cvec(bits))
+    rho2 = _density_op((0, 1), cvec(bits))
+    two = []                      # a random upper-Hermitian two-body operator
+    for _ in range(25):
+        o = rng.choice(bits, 2, replace=False) + 1
+        i = rng.choice(bits, 2, replace=False) + 1
+        two.append(Scatter(complex(rng.normal(), rng.normal()), *map(int, o), *map(int, i),
+                           bits=bits, upper_hermitian=True))
+    two = MBOperator(two, upper_hermitian=True)
+    cases = []
+    for op2, op1, kk, kb in ((rho2, rho1, (0, 0), (1, 1)), (two, rho1, (2, 3), None)):
+        f2, f1 = full(op2), full(op1)
+        M = O.dense_matrix(F, f2) @ O.dense_matrix(F, f1)
+        ket_b = lists[ks.index(kk)]
+        ket = cvec(len(ket_b))
+        if kb is None:           # the bra is the whole N-particle space
+            bra_b = F
+        else:
+            bra_b = lists[ks.index(kb)]
+        bra = cvec(len(bra_b))
+        kf = np.zeros(len(F), complex)
+        kf[np.searchsorted(F, ket_b)] = ket
+        bf = np.zeros(len(F), complex)
+        bf[np.searchsorted(F, bra_b)] = bra
+        cases.append((bra_b, bra, op2, op1, ket_b, ket, np.vdot(bf, M @ kf)))
+    return cases
+
+
+def test_bracket2_oracle_equals_dense_products(oracle_mod):
+    """The oracle's <bra|op2 op1|ket> (multiplication.jl:170-183) against bra^H M2 M1 ket with
+    dense matrices over all N-particle states: density operators that move momentum between
+    sectors, and a two-body upper-Hermitian op2 with the bra spanning every sector."""
+    for bra_b, bra, op2, op1, ket_b, ket, ref in _bracket2_cases(oracle_mod):
+        f2, f1 = (oracle_mod.full_list(oracle_mod.TermList(*op.arrays()), op.upper_hermitian)
+                  for op in (op2, op1))
+        got = oracle_mod.bracket2(bra_b, bra, f2, f1, ket_b, ket)
+        assert abs(ref) > 1e-3
+        assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
 # ------------------------------------------------------------------ matvec consistency
 def test_oracle_matvec_equals_dense(oracle_mod):
     from momentumed.workloads import small
