@@ -21,6 +21,7 @@
 #include <climits>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -1195,12 +1196,15 @@ template <class Rank, int BLOCK>
 static void launch_ldsa_block(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
                               const double2 *x, double2 *y)
 {
-    static bool attr_set = false;   // LDS beyond 64 KiB needs the per-kernel opt-in
-    if (!attr_set) {
+    // LDS beyond 64 KiB needs the per-kernel opt-in, once per device (every entry point runs
+    // with the context's device current)
+    static std::atomic<uint64_t> attr_set{0};
+    const uint64_t bit = 1ULL << (ctx->device & 63);
+    if (!(attr_set.load() & bit)) {
         EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_ldsa<Rank, BLOCK>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize,
                                           (int)kLdsMaxBytes));
-        attr_set = true;
+        attr_set.fetch_or(bit);
     }
     unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL((k_matvec_ldsa<Rank, BLOCK>), dim3(nb), dim3(BLOCK), ct.lds_full,
