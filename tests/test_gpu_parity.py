@@ -69,8 +69,9 @@ def _sorted_rows(rowptr, cols, terms, negs):
 # ------------------------------------------------------------------ enumeration (a1)
 @pytest.mark.parametrize("entry", GOLDEN["sector_sizes"], ids=lambda e: e["source"][-30:])
 def test_enumeration_matches_oracle_and_reference(gpu_ctx, oracle_mod, entry):
-    print(f"[enum] {entry['mesh']} N={entry['N']} Nc_mix={entry['Nc_mix']} Ncc={entry['Nc_conserve']}", flush=True)
-    k = W.mesh(*entry["mesh"])
+    print(f"[enum] {entry.get('mesh', 'k_list')} N={entry['N']} Nc_mix={entry['Nc_mix']} "
+          f"Ncc={entry['Nc_conserve']}", flush=True)
+    k = np.array(entry["k_list"], dtype=np.int64) if "k_list" in entry else W.mesh(*entry["mesh"])
     para = me.EDPara(k_list=k, Gk=tuple(entry["Gk"]), Nc_mix=entry["Nc_mix"],
                      Nc_conserve=entry["Nc_conserve"])
     sysd = _system(para, entry["N"])
@@ -83,6 +84,10 @@ def test_enumeration_matches_oracle_and_reference(gpu_ctx, oracle_mod, entry):
     assert [c for c in counts if c] == [len(v) for v in lists]
     if "sizes" in entry:
         assert [c for c in counts if c] == entry["sizes"]
+    if "momenta" in entry:     # the public API, all sectors in one pass
+        subs, sks = me.ED_momentum_subspaces(para, entry["N"])
+        assert [list(q) for q in sks] == entry["momenta"]
+        assert [sp.length for sp in subs] == entry["sizes"]
     j = 0
     alls = dev.DeviceBasis.enumerate_all(gpu_ctx, sysd)
     assert len(alls) == nsec.value

@@ -83,10 +83,12 @@ def test_colex_order_reference_vectors(oracle_mod):
 
 @pytest.mark.parametrize("entry", GOLDEN["sector_sizes"], ids=lambda e: e["source"][-30:])
 def test_sector_sizes_reference_notebooks(oracle_mod, entry):
-    k = mesh(*entry["mesh"])
+    k = np.array(entry["k_list"], dtype=np.int64) if "k_list" in entry else mesh(*entry["mesh"])
     lists, ks = oracle_mod.enumerate_sectors(k, tuple(entry["Gk"]), entry["Nc_mix"],
                                              entry["Nc_conserve"], entry["N"])
     sizes = [len(v) for v in lists]
+    if "momenta" in entry:
+        assert [list(q) for q in ks] == entry["momenta"]
     if "sizes" in entry:
         assert sizes == entry["sizes"]
     else:
