@@ -205,3 +205,27 @@ def test_bilayer_halperin_energies_and_chern_number(gpu_ctx, tunneling):
     chern = np.angle(np.linalg.det(P)) / (2 * np.pi) * n * n / len(psi[0])
     ref_c = g["chern_tunneling" if tunneling else "chern_no_tunneling"]
     assert abs(chern - ref_c) < 1e-8, (chern, ref_c)
+
+
+def test_sphere_laughlin_unique_zero_mode_all_sectors(gpu_ctx, oracle_mod):
+    """The reference's sphere example (19 orbitals, 7 electrons, one-dimensional angular
+    momentum without modulo) with V_1 only: all 85 L_z sectors solved as one batch; the 1/3
+    Laughlin state is the single zero-energy state, in the central sector L_z = 63, and the
+    central sector's lowest 5 levels match the oracle's dense matrix."""
+    import test_oracle as T
+    para = T._sphere_para(19, [0.0, 1.0, 0.0, 0.0])
+    subs, ks = me.ED_momentum_subspaces(para, 7)
+    assert len(subs) == 85 and ks[0] == (21,) and ks[-1] == (105,)
+    H = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    print(f"[sphere] {len(H.scats)} terms, 85 sectors, D max {max(sp.length for sp in subs)}",
+          flush=True)
+    res = me.EDsolve_batch([(sp, H) for sp in subs], N=1, want_vectors=False)
+    e0 = np.array([E[0] for E, _ in res])
+    zero = [ks[i][0] for i in np.flatnonzero(np.abs(e0) < 1e-10)]
+    assert zero == [63]
+    assert np.delete(e0, ks.index((63,))).min() > 1e-4
+    sp = subs[ks.index((63,))]
+    E, V = me.EDsolve(sp, H, N=5, method_info=False)
+    ev = np.linalg.eigvalsh(oracle_mod.dense_matrix(sp.list, _oracle_full(oracle_mod, H)))[:5]
+    np.testing.assert_allclose(E, ev, rtol=1e-10, atol=1e-12)
+    assert abs(E[0]) < 1e-10 and E[1] > 1e-3

@@ -300,6 +300,57 @@ def test_laughlin_energies_reference_notebook(oracle_mod):
     np.testing.assert_allclose(e, g["values"], rtol=1e-10, atol=0)
 
 
+# ------------------------------------------------------------------ sphere geometry
+def test_clebsch_gordan_textbook_values_and_orthogonality():
+    """The Racah-formula coefficients behind the sphere interaction: textbook values for
+    1/2 x 1/2 and 1 x 1, and orthonormality of the 9/2 x 9/2 coupling."""
+    import lls
+    h = Fraction(1, 2)
+    r2, r3, r6 = np.sqrt(2), np.sqrt(3), np.sqrt(6)
+    checks = [((h, h, h, -h, 1, 0), 1 / r2), ((h, h, h, -h, 0, 0), 1 / r2),
+              ((h, -h, h, h, 0, 0), -1 / r2), ((1, 1, 1, -1, 2, 0), 1 / r6),
+              ((1, 0, 1, 0, 2, 0), np.sqrt(2 / 3)), ((1, 0, 1, 0, 1, 0), 0.0),
+              ((1, 1, 1, -1, 1, 0), 1 / r2), ((1, 0, 1, 0, 0, 0), -1 / r3),
+              ((1, 1, 1, 0, 2, 1), 1 / r2), ((2, 2, 1, -1, 1, 1), np.sqrt(3 / 5))]
+    for args, want in checks:
+        assert abs(lls.clebsch_gordan(*args) - want) <= 1e-15, args
+    j = Fraction(9, 2)
+    ms = [Fraction(k, 2) for k in range(-9, 10, 2)]
+    for M in (0, 1, 4):
+        for J in range(abs(M), 10):
+            for Jp in range(abs(M), 10):
+                dot = sum(lls.clebsch_gordan(j, m, j, M - m, J, M) *
+                          lls.clebsch_gordan(j, m, j, M - m, Jp, M) for m in ms)
+                assert abs(dot - (J == Jp)) <= 1e-13
+
+
+def _sphere_para(n_orb, V):
+    import lls
+    from momentumed import EDPara
+    return EDPara(k_list=np.arange(n_orb, dtype=np.int64).reshape(1, -1), Gk=(0,),
+                  H_two=lls.SphereHaldaneInteraction(n_orb, V))
+
+
+@pytest.mark.parametrize("n_orb,zero_sectors", [(10, [18]), (11, [18, 19, 20, 21, 22])])
+def test_sphere_laughlin_zero_modes(oracle_mod, n_orb, zero_sectors):
+    """Haldane sphere, 4 electrons, V_1 only (the reference's sphere example with mix = 1):
+    at 2S = 3(N-1) the 1/3 Laughlin state is the only zero-energy state (L = 0: one zero mode,
+    in the central L_z sector); one flux more gives the N + 1 = 5 quasihole states of an L = 2
+    multiplet, one zero mode in each of the five central L_z sectors. Everything else has
+    energy above 1e-3."""
+    para = _sphere_para(n_orb, [0.0, 1.0])
+    lists, ks, full = _dense_from_terms(oracle_mod, para, [4], [ED_scatterlist_twobody(para)])
+    found = []
+    for basis, k in zip(lists, ks):
+        e = np.linalg.eigvalsh(oracle_mod.dense_matrix(basis, full))
+        assert e.min() > -1e-12
+        nz = int(np.sum(np.abs(e) < 1e-10))
+        assert nz == (1 if k[0] in zero_sectors else 0), (k, e[:3])
+        assert np.sort(e)[nz:].min(initial=1.0) > 1e-3
+        found += [k[0]] * nz
+    assert found == zero_sectors
+
+
 # ------------------------------------------------------------------ two-operator bracket
 def _density_op(q, f, nx=3, ny=4):
     """sum_k f[k] c^+_{k+q} c_k on an nx x ny mesh (orbital = kx + nx*ky + 1)."""
