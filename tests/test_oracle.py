@@ -273,6 +273,60 @@ def test_cdw_energies_reference_notebook(oracle_mod):
     np.testing.assert_allclose(e, g["energies_per_Nk"], rtol=1e-10, atol=0)
 
 
+def _pfaffian_para(level):
+    import llt
+    g = GOLDEN["pfaffian_cfl"]
+    V = llt.LandauInteraction(llt.ReciprocalLattice.named(g["lattice"]), (1, level, 1, 0))
+    V.D_l, V.mix = g["D_l"], g["mix"]
+    return EDPara(k_list=np.array(g["k_list"], dtype=np.int64), Gk=tuple(g["Gk"]), H_two=V)
+
+
+def test_pfaffian_cfl_term_count_reference_notebook(oracle_mod):
+    """3.Pfaffin_vs_CFL.ipynb: the 3x6 triangular mesh prints 729 upper two-body terms."""
+    op = MBOperator(ED_scatterlist_twobody(_pfaffian_para(0)), upper_hermitian=True)
+    assert len(op.scats) == GOLDEN["pfaffian_cfl"]["upper_twobody_terms_n0"]
+
+
+def _match_submultiset(values, exact, rtol=1e-10):
+    """Greedily match every value to a distinct exact eigenvalue; returns the unmatched ones."""
+    left = list(exact)
+    for v in values:
+        j = int(np.argmin([abs(v - x) for x in left]))
+        assert abs(v - left[j]) <= rtol * abs(v), (v, left[j])
+        left.pop(j)
+    return left
+
+
+def pfaffian_exact(oracle_mod, level, n=15):
+    """Lowest n E/Nk of the notebook's sector for `level`, dense on the oracle's matrix."""
+    import scipy.linalg as sl
+    g = GOLDEN["pfaffian_cfl"]
+    gl = g["levels"][level]
+    para = _pfaffian_para(int(level))
+    lists, ks, full = _dense_from_terms(oracle_mod, para, [g["N"]], [ED_scatterlist_twobody(para)])
+    basis = lists[ks.index(tuple(gl["sector"]))]
+    return sl.eigh(oracle_mod.dense_matrix(basis, full), eigvals_only=True,
+                   subset_by_index=[0, n - 1], driver="evr") / para.Nk
+
+
+@pytest.mark.parametrize("level", ["0", "1"])
+def test_pfaffian_cfl_energies_reference_notebook(oracle_mod, level):
+    """The half-filled n = 0 and n = 1 Landau levels (Ne = 9 on 18 orbitals, Coulomb with gate
+    distance 10): the 10 printed E/Nk of one sector each against the oracle's dense spectrum.
+    n = 0: the lowest 10 exactly. n = 1: the dense spectrum has a four-fold level at
+    1.01396316673242 of which the reference's Lanczos run printed three copies; every printed
+    value is an eigenvalue (matched one to one), and the lowest 11 exact values are the
+    printed ten plus that missing copy."""
+    gl = GOLDEN["pfaffian_cfl"]["levels"][level]
+    e = pfaffian_exact(oracle_mod, level)
+    if level == "0":
+        np.testing.assert_allclose(e[:10], gl["energies_per_Nk"], rtol=1e-10, atol=0)
+        return
+    left = _match_submultiset(gl["energies_per_Nk"], e[:11])
+    assert len(left) == 1 and abs(left[0] - 1.0139631667324) < 1e-12
+    assert np.sum(np.abs(e - 1.0139631667324) < 1e-12) == 4
+
+
 def test_orbital_overlap_closed_forms():
     """The restated orbital overlap: equal shifts give exp(-1/4) on every orbital (q = 0, no
     phase); the phases of a shift and its reverse are conjugate; |overlap| = exp(-1/4)."""
