@@ -8,12 +8,13 @@ library's stream, the CPU baseline on a bounded row sample, setup times). Progre
 stderr.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload large|medium|small]
-                    [--solve] [--no-cpu-baseline] [--cpu-seconds S] [--no-batched]
+                    [--no-solve] [--no-cpu-baseline] [--cpu-seconds S] [--no-batched]
                     [--no-served] [--batch-ratios R] [--serve-seconds T]
 
-Beside the matvec line it reports, in the same JSON object, the batched sweep (3,840
-two-component sector solves, batched vs a plain loop of single solves) and the served stream
-(Poisson requests of mixed sizes, continuous batching vs one solve at a time, p50/p99
+Beside the matvec line it reports, in the same JSON object, the block matvec (4 vectors per
+launch), the sector's lowest-nev eigensolve (time, matvec count, residual), the batched sweep
+(3,840 two-component sector solves, batched vs a plain loop of single solves) and the served
+stream (Poisson requests of mixed sizes, continuous batching vs one solve at a time, p50/p99
 latency).
 """
 import argparse
@@ -234,7 +235,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="large", choices=["small", "medium", "large"])
-    ap.add_argument("--solve", action="store_true", help="also time the lowest-nev eigensolve")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip timing the lowest-nev eigensolve of the sector")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true",
@@ -363,13 +365,17 @@ def main():
         out["block4"] = block_section(ctx, dop, basis, x, y, stream, args, nnz_off.value, S,
                                       ms_per)
 
-    if args.solve:
+    if not args.no_solve:
         log(f"[bench] eigensolve nev={w.nev} krylovdim={w.krylovdim} tol={w.tol}")
         ts = time.perf_counter()
         vals, _, info = dev.krylov_map_solve(lm, w.nev, krylovdim=w.krylovdim, tol=w.tol,
                                              want_vectors=False)
-        out["solve"] = {"seconds": round(time.perf_counter() - ts, 3), "eigenvalues": vals.tolist(),
-                        **info}
+        secs = time.perf_counter() - ts
+        mv_s = info["matvecs"] * ms_per * 1e-3
+        out["solve"] = {"seconds": round(secs, 3), "nev": w.nev, "krylovdim": w.krylovdim,
+                        "tol": w.tol, "eigenvalues": vals.tolist(), **info,
+                        "matvec_seconds": round(mv_s, 3),
+                        "matvec_share": round(mv_s / secs, 3)}
         log(f"[bench] solve: {out['solve']}")
 
     if not args.no_cpu_baseline:
