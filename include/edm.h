@@ -254,6 +254,11 @@ int edm_csr_matvec(edm_ctx *ctx, const edm_csr *m, const void *x_dev, void *y_de
 int edm_csr_eigsolve(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylovdim, double tol,
                      int32_t maxiter, uint64_t seed, const double *vec0, double *vals,
                      double *vecs, edm_solve_info *info);
+/* Matrix(H) of the reference's :dense method (MomentumED.jl:340-369, which densifies the
+ * sparse matrix and calls LAPACK on the host): the stored matrix written as a dense D x D
+ * complex row-major array at a_dev (device, 16 * D * D bytes, entry (i, j) = <i|H|j>), for a
+ * dense Hermitian eigensolver on the device. Stream-ordered. */
+int edm_csr_dense(edm_ctx *ctx, const edm_csr *m, void *a_dev);
 int edm_csr_destroy(edm_csr *m);
 
 /* ----------------------------------------------------------------- batched / served solves */

@@ -636,6 +636,16 @@ int edm_csr_eigsolve(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylov
     });
 }
 
+int edm_csr_dense(edm_ctx *ctx, const edm_csr *m, void *a_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && m && a_dev, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(m->ctx == ctx, EDM_EINVAL, "matrix belongs to another context");
+        csr_dense(ctx, m, (double2 *)a_dev);
+        return EDM_OK;
+    });
+}
+
 int edm_csr_destroy(edm_csr *m)
 {
     if (!m) return EDM_OK;

@@ -92,6 +92,19 @@ k_csr_spmv(int64_t D, const int64_t *__restrict__ rp, const int32_t *__restrict_
     if (r < D && l == 0) y[r] = make_double2(yr, yi);
 }
 
+// dense row-major copy: workgroup r zeroes row r of A, then writes the row's stored entries
+// (columns are distinct after merging)
+__global__ void __launch_bounds__(kBlk)
+k_csr_dense(int64_t D, const int64_t *__restrict__ rp, const int32_t *__restrict__ cols,
+            const double2 *__restrict__ vals, double2 *__restrict__ A)
+{
+    const int64_t r = blockIdx.x;
+    double2 *row = A + r * D;
+    for (int64_t c = threadIdx.x; c < D; c += kBlk) row[c] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int64_t k = rp[r] + threadIdx.x; k < rp[r + 1]; k += kBlk) row[cols[k]] = vals[k];
+}
+
 inline unsigned blocks(int64_t n) { return (unsigned)((n + kBlk - 1) / kBlk); }
 
 // exclusive prefix of counts[0..n) into out[0..n] (host; the counts are copied back)
@@ -204,5 +217,14 @@ void csr_matvec(edm_ctx *ctx, const edm_csr *m, const double2 *x, double2 *y)
     else
         hipLaunchKernelGGL(k_csr_spmv<16>, dim3(blocks(D * 16)), dim3(kBlk), 0, ctx->stream, D,
                            m->d_rowptr, m->d_cols, m->d_vals, x, y);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void csr_dense(edm_ctx *ctx, const edm_csr *m, double2 *A)
+{
+    if (m->D == 0) return;
+    EDM_REQUIRE(m->D <= (1LL << 31) - 1, EDM_EINVAL, "dimension too large for a dense matrix");
+    hipLaunchKernelGGL(k_csr_dense, dim3((unsigned)m->D), dim3(kBlk), 0, ctx->stream, m->D,
+                       m->d_rowptr, m->d_cols, m->d_vals, A);
     EDM_HIP_CHECK(hipGetLastError());
 }

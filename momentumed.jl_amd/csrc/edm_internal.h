@@ -272,6 +272,8 @@ struct edm_csr {
 edm_csr *csr_build(edm_ctx *ctx, const edm_op *op, const edm_basis *b);
 void csr_destroy(edm_csr *m);
 void csr_matvec(edm_ctx *ctx, const edm_csr *m, const double2 *x, double2 *y);
+// A (D x D complex, row-major, device) = the stored matrix with zeros elsewhere
+void csr_dense(edm_ctx *ctx, const edm_csr *m, double2 *A);
 const char *edm_global_error();
 // eigensolver for a small real symmetric matrix (edm_lanczos.cpp)
 void sym_eigh(int n, std::vector<double> &A, std::vector<double> &w, std::vector<double> &Z);

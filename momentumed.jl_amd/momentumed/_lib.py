@@ -88,6 +88,7 @@ _SIGS = [
     ("edm_csr_matvec", c_i32, [c_vp, c_vp, c_vp, c_vp]),
     ("edm_csr_eigsolve", c_i32, [c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_vp, c_vp,
                                  P(EdmSolveInfo)]),
+    ("edm_csr_dense", c_i32, [c_vp, c_vp, c_vp]),
     ("edm_csr_destroy", c_i32, [c_vp]),
     ("edm_projected_eigh", c_i32, [c_i32, c_vp, c_vp, c_vp]),
     ("edm_projected_heigh", c_i32, [c_i32, c_vp, c_vp, c_vp]),

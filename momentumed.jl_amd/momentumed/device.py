@@ -448,6 +448,26 @@ def create_gpu_matrix(op, space, device: str = "rocm", ctx: Context | None = Non
     return DeviceCSR(dop, basis, space)
 
 
+def dense_matrix_solve(H: DeviceCSR, N_eigen: int, *, want_vectors: bool = True):
+    """The :dense method (MomentumED.jl:340-369: eigen(Hermitian(Matrix(H)), 1:N)) on the
+    device: the stored matrix is written as a dense D x D array by edm_csr_dense and
+    diagonalised by the GPU's dense Hermitian eigensolver (torch.linalg.eigh, rocSOLVER).
+    Returns (vals, vecs) with the N lowest eigenpairs."""
+    import torch
+    D = H.D
+    N_eigen = min(N_eigen, D)
+    ctx = H.ctx
+    A = torch.empty((D, D), dtype=torch.complex128, device=f"cuda:{ctx.device}")
+    torch.cuda.current_stream(A.device).synchronize()
+    check(ctx.L.edm_csr_dense(ctx.h, H.h, A.data_ptr()), ctx.h)
+    ctx.synchronize()
+    w, V = torch.linalg.eigh(A)
+    vals = w[:N_eigen].cpu().numpy()
+    vecs = [V[:, i].cpu().numpy() for i in range(N_eigen)] if want_vectors else None
+    del A, w, V
+    return vals, vecs
+
+
 def krylov_matrix_solve(H: DeviceCSR, N_eigen: int, *, ishermitian: bool = True, vec0=None,
                         krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100,
                         seed: int = 0x5EED, want_vectors: bool = True, blocksize: int = 1,

@@ -1,6 +1,7 @@
 """EDsolve: the user-facing solve (MomentumED/src/MomentumED.jl:217-480), device map path."""
 from __future__ import annotations
 
+import logging
 import time
 import warnings
 
@@ -12,27 +13,30 @@ from .core import HilbertSubspace, MBOperator, MBS64Vector, Scatter, isupper
 __all__ = ["EDsolve"]
 
 _DEVICES = ("cpu", "gpu", "rocm", "cuda", "oneapi", "metal", "multi_cuda")
+_log = logging.getLogger("momentumed")
 
 
 def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method: str = "sparse",
             device: str = "gpu", showtime: bool = False, ishermitian: bool = True,
             element_type=float, index_type=np.int32, min_sparse_dim: int = 100,
-            max_dense_dim: int = 200, map_warning_dim: int = 20000, method_info: bool = True,
-            **krylovkit_kwargs):
+            max_dense_dim: int | None = None, map_warning_dim: int = 20000,
+            method_info: bool = True, **krylovkit_kwargs):
     """Lowest N eigenpairs of `hamiltonian` in `subspace`.
 
     Same call shape and return value as the reference: ``energies, vectors = EDsolve(subspace,
     H; N, method, device, ...)`` or ``EDsolve(subspace, scat_list1, scat_list2, ...; ...)``, with
     energies ascending (float64[N]) and vectors a list of MBS64Vector (MomentumED.jl:217-480).
-    Both Krylov methods run on the device: method="sparse" (the default, as in the reference)
+    All methods run on the device: method="sparse" (the default, as in the reference)
     builds the stored matrix on the device and solves with it (MomentumED.jl:259-333);
-    method="map" applies the matrix-free map (MomentumED.jl:375-480). KrylovKit keywords
-    krylovdim, tol, maxiter, vec0 are honoured, plus `seed` for the start vector and, for
-    method="map", `blocksize` (block Lanczos with that many vectors per step; 1, the default,
-    is the single-vector iteration KrylovKit runs). The dense
-    method (host LAPACK) is not part of this package; the reference's automatic switch from
-    :sparse to :dense below min_sparse_dim is therefore not taken: small subspaces stay on
-    the stored-matrix path, which gives the same eigenpairs."""
+    method="map" applies the matrix-free map (MomentumED.jl:375-480); method="dense" writes
+    the stored matrix as a dense array on the device and calls the GPU's dense Hermitian
+    eigensolver (the reference runs LAPACK on the host, MomentumED.jl:340-369). As in the
+    reference, "sparse" switches to "dense" below min_sparse_dim and "dense" to "sparse"
+    above max_dense_dim (default 200 for an MBOperator, 500 for scatter lists,
+    MomentumED.jl:221,379). KrylovKit keywords krylovdim, tol, maxiter, vec0 are honoured,
+    plus `seed` for the start vector and, for method="map", `blocksize` (block Lanczos with
+    that many vectors per step; 1, the default, is the single-vector iteration KrylovKit
+    runs)."""
     if N < 1:
         raise AssertionError("N should be at least 1.")
     if method not in ("sparse", "dense", "map"):
@@ -51,9 +55,10 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
             raise AssertionError("Current Hamiltonian matrix construction assumes it being "
                                  "Hermitian.")
         hamiltonian = MBOperator(*lists, upper_hermitian=True)
-    if method == "dense":
-        raise NotImplementedError("method='dense' (host LAPACK) is not part of this package; "
-                                  "use method='sparse' or 'map' on the device")
+        if max_dense_dim is None:
+            max_dense_dim = 500
+    if max_dense_dim is None:
+        max_dense_dim = 200
     if ishermitian and not isupper(hamiltonian):
         raise AssertionError("Use upper_hermitian form of Hamiltonian operator when "
                              "ishermitian = true.")
@@ -62,6 +67,24 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
             raise AssertionError(f"Hilbert space dimension {len(subspace)} exceeds limit of "
                                  f"index_type={index_type}.")
     dim = len(subspace)
+    if method in ("sparse", "dense"):
+        # MomentumED.jl:233-244
+        if min_sparse_dim > max_dense_dim:
+            if method_info:
+                _log.info("Trying to set min_sparse_dim as %d, larger than max_dense_dim(=%d). "
+                          "Reset to %d automatically.", min_sparse_dim, max_dense_dim,
+                          max_dense_dim)
+            min_sparse_dim = max_dense_dim
+        if method == "sparse" and dim < min_sparse_dim:
+            if method_info:
+                _log.info("Hilbert space dimension < %d; switch to method=:dense "
+                          "automatically.", min_sparse_dim)
+            method = "dense"
+        if method == "dense" and dim > max_dense_dim:
+            if method_info:
+                _log.info("Hilbert space dimension > %d; switch to method=:sparse "
+                          "automatically.", max_dense_dim)
+            method = "sparse"
     if method == "map" and dim < map_warning_dim and method_info:
         warnings.warn(f"Linear map may be slow for dim={dim}. Consider using :sparse method.")
     if device == "cpu":
@@ -79,6 +102,18 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
     dev.throw_gpu_unavailable("rocm")
 
     t0 = time.perf_counter()
+    if method == "dense":
+        if not ishermitian:
+            raise NotImplementedError("the device dense method is the Hermitian one")
+        H_gpu = dev.create_gpu_matrix(hamiltonian, subspace, "rocm")
+        t1 = time.perf_counter()
+        vals, vecs = dev.dense_matrix_solve(H_gpu, N)
+        H_gpu.close()
+        if showtime:
+            print(f"  {t1 - t0:.6f} seconds (device matrix, {H_gpu.nnz} stored entries)")
+            print(f"  {time.perf_counter() - t1:.6f} seconds (device dense eigensolver)")
+        n = min(N, dim)
+        return np.asarray(vals[:n]), [MBS64Vector(vecs[i], subspace) for i in range(n)]
     if method == "sparse":
         H_gpu = dev.create_gpu_matrix(hamiltonian, subspace, "rocm")
         what = f"device matrix, {H_gpu.nnz} stored entries"

@@ -149,3 +149,41 @@ def test_csr_rejects_generic_terms(gpu_ctx):
     sp = me.HilbertSubspace([s for s in range(1 << bits) if bin(s).count("1") == 3], bits=bits)
     with pytest.raises(EDMError):
         dev.create_gpu_matrix(op, sp)
+
+
+def test_edsolve_dense_and_method_switching(gpu_ctx, oracle_mod, caplog):
+    """method="dense" (MomentumED.jl:340-369) on the device against LAPACK, and the
+    reference's automatic switches (MomentumED.jl:233-244): sparse below min_sparse_dim runs
+    dense, dense above max_dense_dim runs sparse, scatter lists allow dense up to 500."""
+    import logging
+    w = W.small()
+    op = w.operator()
+    full = _oracle_full(oracle_mod, op)
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][3]
+    Hd = oracle_mod.dense_matrix(sp.list, full)
+    ev = np.linalg.eigvalsh(Hd)
+    E, V = me.EDsolve(sp, op, N=w.nev, method="dense", max_dense_dim=500, method_info=False)
+    np.testing.assert_allclose(E, ev[: w.nev], rtol=0, atol=1e-12 * np.abs(ev).max())
+    for i, v in enumerate(V):
+        assert np.linalg.norm(Hd @ v.vec - E[i] * v.vec) < 1e-10
+    caplog.set_level(logging.INFO, logger="momentumed")
+    caplog.clear()
+    E2, _ = me.EDsolve(sp, op, N=w.nev, min_sparse_dim=300, max_dense_dim=500)
+    assert "switch to method=:dense" in caplog.text
+    np.testing.assert_allclose(E2, E, rtol=0, atol=1e-12 * np.abs(ev).max())
+    caplog.clear()
+    assert sp.length == 200
+    E3, _ = me.EDsolve(sp, op, N=w.nev, method="dense", max_dense_dim=150)
+    assert "switch to method=:sparse" in caplog.text
+    np.testing.assert_allclose(E3, ev[: w.nev], rtol=1e-10, atol=1e-12)
+    caplog.clear()
+    l2 = me.ED_scatterlist_twobody(w.para)
+    l1 = me.ED_scatterlist_onebody(w.para)
+    big = me.ED_momentum_subspaces(w.para, 5)[0][0]
+    assert big.length == 201
+    E5, _ = me.EDsolve(big, op, N=w.nev, method="dense")       # 201 > 200 for an MBOperator
+    assert "switch to method=:sparse" in caplog.text
+    caplog.clear()
+    E4, _ = me.EDsolve(sp, l1, l2, N=w.nev, method="dense")    # scatter lists: up to 500
+    assert "switch to method" not in caplog.text
+    np.testing.assert_allclose(E4, E, rtol=0, atol=1e-12 * np.abs(ev).max())

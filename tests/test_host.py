@@ -173,7 +173,7 @@ def test_edsolve_argument_errors():
     with pytest.raises(AssertionError):
         me.EDsolve(sp, op, N=0)
     with pytest.raises(NotImplementedError):
-        me.EDsolve(sp, op, N=1, method="dense")
+        me.EDsolve(sp, op, N=1, method="dense", device="cpu")
     with pytest.raises(NotImplementedError):
         me.EDsolve(sp, op, N=1, device="cpu", method_info=False)
     with pytest.raises(NotImplementedError):
