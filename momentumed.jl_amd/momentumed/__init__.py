@@ -10,7 +10,8 @@ The compute runs in libedm.so (hand-written gfx950 HIP kernels behind a C-ABI).
 from .core import (HilbertSubspace, MBOperator, MBS64, MBS64Vector, Scatter, get_body,
                    index_fit, isdiagonal, isupper, make_mask64, occ_list, parity_mask,
                    sort_merge_scatlist)
-from .device import (DEFAULT_GPU, GPU_AVAILABLE, activate_gpu_device, create_gpu_linearmap,
+from .device import (DEFAULT_GPU, GPU_AVAILABLE, GPU_MEMORY_MONITOR, LinearMap,
+                     SparseHmltMatrix, activate_gpu_device, create_gpu_linearmap,
                      create_gpu_matrix, krylov_map_solve, krylov_matrix_solve,
                      release_gpu_memory)
 from .preparation import (EDPara, ED_momentum_subspaces, ED_scatterlist_onebody,
@@ -28,6 +29,7 @@ __all__ = [
     "get_body", "index_fit", "isdiagonal", "isupper", "make_mask64", "occ_list", "parity_mask",
     "sort_merge_scatlist", "EDsolve", "GPU_AVAILABLE", "DEFAULT_GPU", "activate_gpu_device",
     "create_gpu_linearmap", "krylov_map_solve", "release_gpu_memory", "BatchSolver",
-    "create_gpu_matrix", "krylov_matrix_solve",
-    "EDsolve_batch",
+    "create_gpu_matrix", "krylov_matrix_solve", "GPU_MEMORY_MONITOR", "LinearMap",
+    "SparseHmltMatrix", "EDsolve_batch", "ED_bracket", "ED_bracket_threaded",
+    "ED_connection_gaugefixing", "ED_step_inner_prod", "PES_1rdm",
 ]

@@ -25,6 +25,15 @@ __all__ = [
 # gpu_interface.jl:25-30,39
 GPU_AVAILABLE = {"rocm": False}
 DEFAULT_GPU = {"value": "nogpu"}
+# gpu_interface.jl:36: when set, every device eigensolve reports the context's HBM use
+GPU_MEMORY_MONITOR = {"value": False}
+
+
+def _memory_report(ctx):
+    if GPU_MEMORY_MONITOR["value"]:
+        used, budget = ctx.memory()
+        print(f"rocm context {ctx.device}: {used / 2**30:.3f} GiB in use"
+              + (f" of a {budget / 2**30:.3f} GiB budget" if budget else ""))
 
 
 def activate_gpu_device(device: str = "rocm", set_default_gpu: bool = True) -> bool:
@@ -308,9 +317,46 @@ class ROCmLinearMap:
         return y
 
 
-def create_gpu_linearmap(op, space, device: str = "rocm", ctx: Context | None = None):
+class LinearMap:
+    """LinearMap(H::MBOperator, subspace) (linear_map.jl:18-42): the operator and subspace a
+    map is made of. Here it only names them for create_gpu_linearmap(A, "rocm"); the full-list
+    expansion and every product run on the device."""
+
+    def __init__(self, op, space):
+        self.op = op
+        self.space = space
+
+    def size(self):
+        return (len(self.space), len(self.space))
+
+
+class SparseHmltMatrix:
+    """SparseHmltMatrix(subspace, scat_list; element_type, index_type)
+    (sparse_matrix.jl:65-191): the Hermitian matrix of upper-Hermitian terms on a subspace.
+    The reference assembles it on the host; here it names the terms and subspace for
+    create_gpu_matrix(H, "rocm"), which assembles it on the device."""
+
+    def __init__(self, subspace, scat_list, element_type=float, index_type=np.int32):
+        from .core import MBOperator
+        if len(subspace) > np.iinfo(index_type).max:
+            raise AssertionError(f"Hilbert space dimension {len(subspace)} exceeds limit of "
+                                 f"index_type={index_type}.")
+        self.space = subspace
+        self.op = MBOperator(list(scat_list), upper_hermitian=True)
+
+    def size(self):
+        return (len(self.space), len(self.space))
+
+
+def create_gpu_linearmap(op, space=None, device: str = "rocm", ctx: Context | None = None):
     """gpu_interface.jl:85 / MomentumEDCUDAExt.jl:94-113: upload the operator (and the basis,
-    unless the subspace is already on the device) and return the device map."""
+    unless the subspace is already on the device) and return the device map. Called as
+    create_gpu_linearmap(op, space) or, as in the reference, create_gpu_linearmap(A, "rocm")
+    with A a LinearMap."""
+    if isinstance(op, LinearMap):
+        if isinstance(space, str):
+            device = space
+        op, space = op.op, op.space
     if device != "rocm":
         raise ValueError(f"Unknown GPU {device}.")
     ctx = ctx or get_context(0)
@@ -372,6 +418,7 @@ def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True
     out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
                 "converged": info.converged, "krylovdim": info.krylovdim,
                 "max_residual": info.max_residual, "seconds": info.seconds}
+    _memory_report(ctx)
     return vals, (list(vecs) if vecs is not None else None), out_info
 
 
@@ -436,10 +483,15 @@ class DeviceCSR:
         self._fin()
 
 
-def create_gpu_matrix(op, space, device: str = "rocm", ctx: Context | None = None) -> DeviceCSR:
+def create_gpu_matrix(op, space=None, device: str = "rocm", ctx: Context | None = None) -> DeviceCSR:
     """create_gpu_matrix(SparseHmltMatrix(subspace, scats), Val(device)) in one step
     (sparse_matrix.jl:65-191 + MomentumEDCUDAExt.jl:147-152): the matrix is built on the
-    device from the operator's term list and the subspace's device basis."""
+    device from the operator's term list and the subspace's device basis. Called as
+    create_gpu_matrix(op, space) or create_gpu_matrix(H, "rocm") with H a SparseHmltMatrix."""
+    if isinstance(op, SparseHmltMatrix):
+        if isinstance(space, str):
+            device = space
+        op, space = op.op, op.space
     if device != "rocm":
         raise ValueError(f"Unknown GPU {device}.")
     ctx = ctx or get_context(0)
@@ -498,4 +550,5 @@ def krylov_matrix_solve(H: DeviceCSR, N_eigen: int, *, ishermitian: bool = True,
     out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
                 "converged": info.converged, "krylovdim": info.krylovdim,
                 "max_residual": info.max_residual, "seconds": info.seconds}
+    _memory_report(ctx)
     return vals, (list(vecs) if vecs is not None else None), out_info

@@ -187,3 +187,26 @@ def test_edsolve_dense_and_method_switching(gpu_ctx, oracle_mod, caplog):
     E4, _ = me.EDsolve(sp, l1, l2, N=w.nev, method="dense")    # scatter lists: up to 500
     assert "switch to method" not in caplog.text
     np.testing.assert_allclose(E4, E, rtol=0, atol=1e-12 * np.abs(ev).max())
+
+
+def test_reference_constructor_forms(gpu_ctx, capsys):
+    """create_gpu_matrix(SparseHmltMatrix(subspace, scats), :rocm) and
+    create_gpu_linearmap(LinearMap(H, subspace), :rocm) as reference scripts write them, with
+    GPU_MEMORY_MONITOR reporting the context's HBM use after each solve."""
+    w = W.small()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][2]
+    scats = me.ED_scatterlist_onebody(w.para) + me.ED_scatterlist_twobody(w.para)
+    H = me.SparseHmltMatrix(sp, scats)
+    assert H.size() == (sp.length, sp.length)
+    me.GPU_MEMORY_MONITOR["value"] = True
+    try:
+        v1, _, i1 = me.krylov_matrix_solve(me.create_gpu_matrix(H, "rocm"), 4)
+        op = me.MBOperator(scats, upper_hermitian=True)
+        v2, _, _ = me.krylov_map_solve(me.create_gpu_linearmap(me.LinearMap(op, sp), "rocm"), 4)
+    finally:
+        me.GPU_MEMORY_MONITOR["value"] = False
+    assert "GiB in use" in capsys.readouterr().out
+    E, _ = me.EDsolve(sp, op, N=4, method_info=False)
+    np.testing.assert_allclose(v1, E, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(v2, E, rtol=1e-10, atol=1e-12)
+    assert i1["converged"] >= 4
