@@ -260,6 +260,11 @@ void launch_rdm1(edm_ctx *ctx, const edm_basis *b, const double2 *psi, const int
 void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
                      const edm_basis *ket, const double2 *d_bra, const double2 *d_ket,
                      double2 *d_out);
+// M (nA x B->D, row-major, zeroed by the caller) of the particle reduced density matrix
+// rdm = M M^H; *d_misses counts complements not found in B
+void launch_pes_fill(edm_ctx *ctx, const edm_basis *psi_b, const double2 *d_psi, int64_t nA,
+                     const uint64_t *d_A, uint64_t ph_mask, const edm_basis *B, double2 *d_M,
+                     unsigned int *d_misses);
 void launch_step_gram(edm_ctx *ctx, const edm_basis *b, const double2 *d_u,
                       const double2 *const *d_vf, const double2 *const *d_vi, int g, double2 *d_out);
 struct edm_csr {

@@ -1033,6 +1033,46 @@ k_bracket2(int64_t Dk, const uint64_t *__restrict__ kstates, const double2 *__re
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// Particle reduced density matrix of a momentum block (PES_MomtBlock_rdm,
+// MomentumED/src/analysis/particle_reduced_density_matrix.jl:37-88) as rdm = M M^H with
+// M[a, b] = (-1)^occ(s, a) psi[s] for s ^ ph = a | b (a a part-A state, b its complement in
+// the block B): every (basis state, A state) pair is one thread; a pair writes at most one
+// entry and every entry has at most one writer. occ(s, a): occupied orbitals of s in the
+// parity mask of a (scat_occ_number, EDCore/src/types/manybodystate_basis.jl:397-409).
+__device__ __forceinline__ uint32_t occ_parity(uint64_t n, uint64_t m)
+{
+    uint64_t mask = 0;
+    while (m) {
+        const int t1 = __builtin_ctzll(m);
+        m &= m - 1;
+        const uint64_t hi = m ? (1ULL << __builtin_ctzll(m)) : 0ULL;
+        m &= m - 1;
+        mask += hi - (t1 >= 63 ? 0ULL : (2ULL << t1));
+    }
+    return (uint32_t)__popcll(n & mask) & 1u;
+}
+
+__global__ void __launch_bounds__(kRowBlock)
+k_pes_fill(int64_t D, const uint64_t *__restrict__ states, const double2 *__restrict__ psi,
+           int64_t nA, const uint64_t *__restrict__ A, uint64_t ph, RankSearch bsearch,
+           double2 *__restrict__ M, unsigned int *__restrict__ misses)
+{
+    const int64_t n = D * nA, stride = (int64_t)gridDim.x * kRowBlock;
+    for (int64_t t = (int64_t)blockIdx.x * kRowBlock + threadIdx.x; t < n; t += stride) {
+        const int64_t x = t / nA, a = t - x * nA;
+        const uint64_t s = states[x], sph = s ^ ph, am = A[a];
+        if ((sph & am) != am) continue;
+        const int64_t jb = bsearch(sph & ~am);
+        if (jb < 0) {
+            atomicAdd(misses, 1u);
+            continue;
+        }
+        const double2 v = psi[x];
+        const uint32_t neg = occ_parity(s, am);
+        M[a * bsearch.D + jb] = make_double2(flip_sign(v.x, neg), flip_sign(v.y, neg));
+    }
+}
+
 // out[p] = sum over chunks of partial[p * nchunks + c], in chunk order
 __global__ void __launch_bounds__(kRedBlock)
 k_sum_chunks(const double2 *__restrict__ partial, int64_t nchunks, double2 *__restrict__ out)
@@ -1541,6 +1581,19 @@ void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const e
     hipLaunchKernelGGL(k_bracket2, dim3((unsigned)nch), dim3(kRedBlock), 0, ctx->stream, D,
                        ket->d_states, d_ket, make_search(bra), d_bra, t2, t1, rows, part);
     hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kRedBlock), 0, ctx->stream, part, nch, d_out);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pes_fill(edm_ctx *ctx, const edm_basis *psi_b, const double2 *d_psi, int64_t nA,
+                     const uint64_t *d_A, uint64_t ph_mask, const edm_basis *B, double2 *d_M,
+                     unsigned int *d_misses)
+{
+    const int64_t n = psi_b->D * nA;
+    if (n == 0) return;
+    const int64_t blocks = std::min<int64_t>((n + kRowBlock - 1) / kRowBlock, 1 << 20);
+    hipLaunchKernelGGL(k_pes_fill, dim3((unsigned)blocks),
+                       dim3(kRowBlock), 0, ctx->stream, psi_b->D, psi_b->d_states, d_psi, nA,
+                       d_A, ph_mask, make_search(B), d_M, d_misses);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

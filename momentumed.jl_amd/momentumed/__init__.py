@@ -20,7 +20,7 @@ from .preparation import (EDPara, ED_momentum_subspaces, ED_scatterlist_onebody,
 from .solve import EDsolve
 from .batch import BatchSolver, EDsolve_batch
 from .analysis import (ED_bracket, ED_bracket_threaded, ED_connection_gaugefixing,
-                       ED_step_inner_prod, PES_1rdm)
+                       ED_step_inner_prod, PES_1rdm, PES_MomtBlock_rdm)
 
 __all__ = [
     "EDPara", "ED_momentum_subspaces", "ED_scatterlist_onebody", "ED_scatterlist_twobody",
@@ -31,5 +31,5 @@ __all__ = [
     "create_gpu_linearmap", "krylov_map_solve", "release_gpu_memory", "BatchSolver",
     "create_gpu_matrix", "krylov_matrix_solve", "GPU_MEMORY_MONITOR", "LinearMap",
     "SparseHmltMatrix", "EDsolve_batch", "ED_bracket", "ED_bracket_threaded",
-    "ED_connection_gaugefixing", "ED_step_inner_prod", "PES_1rdm",
+    "ED_connection_gaugefixing", "ED_step_inner_prod", "PES_1rdm", "PES_MomtBlock_rdm",
 ]

@@ -3,7 +3,8 @@ kshift step overlaps of many-body Berry connections.
 
 Mirrors EDCore/src/multiplication.jl:153-307 (ED_bracket, ED_bracket_threaded, one- and
 two-operator forms),
-MomentumED/src/analysis/particle_reduced_density_matrix.jl:4-33 (PES_1rdm) and
+MomentumED/src/analysis/particle_reduced_density_matrix.jl:4-88 (PES_1rdm,
+PES_MomtBlock_rdm) and
 MomentumED/src/analysis/manybody_quantum_geometry.jl:1-91 (ED_connection_gaugefixing!,
 ED_step_inner_prod). The vectors are MBS64Vector (host arrays bound to a subspace); each call
 copies them to the device, runs the libedm.so kernel and returns host results.
@@ -15,11 +16,11 @@ import ctypes
 import numpy as np
 
 from ._lib import check
-from .core import MBOperator, MBS64Vector
+from .core import HilbertSubspace, MBOperator, MBS64Vector
 from .device import DeviceOperator, device_basis_of, get_context
 
-__all__ = ["ED_bracket", "ED_bracket_threaded", "PES_1rdm", "ED_step_inner_prod",
-           "ED_connection_gaugefixing"]
+__all__ = ["ED_bracket", "ED_bracket_threaded", "PES_1rdm", "PES_MomtBlock_rdm",
+           "ED_step_inner_prod", "ED_connection_gaugefixing"]
 
 
 def _dev(vec: np.ndarray, ctx):
@@ -104,6 +105,62 @@ def PES_1rdm(psi: MBS64Vector) -> np.ndarray:
         for j in range(i):
             rdm[i, j] = np.conj(rdm[j, i])
     return rdm
+
+
+def PES_MomtBlock_rdm(para, psi: MBS64Vector, part_subspace, ph_transform=None) -> np.ndarray:
+    """Particle reduced density matrix of psi on one momentum block of part A
+    (particle_reduced_density_matrix.jl:37-88), |A| x |A| Hermitian. Computed on the device
+    as M M^H: M[a, b] = (-1)^occ(s, a) psi[s] for s = a | b with b in the complementary block
+    (the remaining particles of every conserved component, the remaining momentum), filled
+    by edm_pes_block_matrix, and the product as one dense GEMM on the GPU. The
+    particle-hole transformed form (ph_transform true for a component) is not provided."""
+    import torch
+    from .preparation import ED_momentum_subspaces, MBS_totalmomentum, momentum_residue
+    if ph_transform is not None and any(bool(t) for t in ph_transform):
+        raise NotImplementedError("particle-hole transformed parts are not provided")
+    if ph_transform is not None and len(ph_transform) != para.Nc_conserve:
+        raise AssertionError("lengths of subspaces and particle_hole should == "
+                             "para.Nc_conserve.")
+    if psi.space.bits != para.Nk * para.Nc:
+        raise AssertionError("bits == para.Nk * para.Nc")
+    A = np.ascontiguousarray(part_subspace.list, dtype=np.uint64)
+    nA = len(A)
+    rdm = np.zeros((nA, nA), dtype=np.complex128)
+    if nA == 0 or len(psi.vec) == 0:
+        return rdm
+    width = para.Nk * para.Nc_mix
+    comp = (1 << width) - 1
+
+    def counts(state):
+        return tuple(bin((int(state) >> (c * width)) & comp).count("1")
+                     for c in range(para.Nc_conserve))
+    n_psi, n_A = counts(psi.space.list[0]), counts(A[0])
+    n_B = tuple(a - b for a, b in zip(n_psi, n_A))
+    if any(v < 0 for v in n_B):
+        return rdm
+    k_psi = MBS_totalmomentum(para, int(psi.space.list[0]))
+    k_A = MBS_totalmomentum(para, int(A[0]))
+    k_B = momentum_residue(tuple(a - b for a, b in zip(k_psi, k_A)), para.Gk)
+    if sum(n_B) == 0:   # part A holds every particle: the complement is the empty state
+        if any(k_B):
+            return rdm
+        B = HilbertSubspace([0], bits=psi.space.bits)
+    else:
+        subs, _ = ED_momentum_subspaces(para, n_B, momentum_restriction=True,
+                                        momentum_list=[k_B])
+        if not subs:
+            return rdm
+        B = subs[0]
+    ctx = get_context(0)
+    pb = device_basis_of(psi.space, ctx)
+    bb = device_basis_of(B, ctx)
+    x = _dev(psi.vec, ctx)
+    M = torch.zeros((nA, B.length), dtype=torch.complex128, device=x.device)
+    torch.cuda.current_stream(x.device).synchronize()
+    check(ctx.L.edm_pes_block_matrix(ctx.h, pb.h, x.data_ptr(), nA, A.ctypes.data, 0, bb.h,
+                                     M.data_ptr()), ctx.h)
+    ctx.synchronize()
+    return (M @ M.conj().T).cpu().numpy()
 
 
 def ED_step_inner_prod(psi_f, psi_i, orbital_inner_prod):

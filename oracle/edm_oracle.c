@@ -387,3 +387,58 @@ ORC_EXPORT void orc_bracket2(int64_t Db, const uint64_t *bbasis, const double *b
     out[0] = sr;
     out[1] = si;
 }
+
+/* EDCore/src/types/manybodystate_basis.jl:397-409 scat_occ_number: occupied orbitals of n
+ * in the parity mask of i_mask (its set bits paired from the lowest: bits strictly between
+ * the two of a pair; an unpaired top bit covers everything above it). */
+static int occ_number(uint64_t n, uint64_t i_mask)
+{
+    uint64_t mask = 0;
+    while (i_mask) {
+        int t1 = ctz64(i_mask);
+        i_mask &= i_mask - 1;
+        int t2 = ctz64(i_mask);
+        i_mask &= i_mask - 1;
+        mask += shl1(t2);
+        mask -= shl1(t1 + 1);
+    }
+    return popc64(n & mask);
+}
+
+/* MomentumED/src/analysis/particle_reduced_density_matrix.jl:37-88 PES_MomtBlock_rdm, in
+ * the reference's loop order: for every pair (anni = A[i], crea = A[j]), i <= j, and every
+ * state s of psi's basis: s_ph = s ^ ph_mask; anni must be occupied in s_ph and crea empty
+ * after removing it; s_out = ((s_ph & ~anni) | crea) ^ ph_mask must be in the basis; then
+ * rdm[i, j] += (-1)^(occ(s, anni) + occ(s_out, crea)) conj(psi[s_out]) psi[s]. The lower
+ * triangle is filled as the conjugate of the upper one (Hermitian(rdm, :U)). rdm: nA*nA
+ * complex, row-major. */
+ORC_EXPORT void orc_pes_block_rdm(int64_t D, const uint64_t *basis, const double *psi,
+                                  int64_t nA, const uint64_t *A, uint64_t ph_mask, double *rdm)
+{
+    for (int64_t i = 0; i < nA; ++i)
+        for (int64_t j = 0; j < nA; ++j) {
+            if (i > j) continue;
+            double sr = 0.0, si = 0.0;
+            const uint64_t anni = A[i], crea = A[j];
+            for (int64_t x = 0; x < D; ++x) {
+                uint64_t s = basis[x], s_ph = s ^ ph_mask;
+                if ((s_ph & anni) != anni) continue;
+                uint64_t mid = s_ph & ~anni;
+                if (mid & crea) continue;
+                uint64_t s_out = (mid | crea) ^ ph_mask;
+                int64_t y = orc_search(basis, D, s_out);
+                if (y < 0) continue;
+                double pr = psi[2 * y] * psi[2 * x] + psi[2 * y + 1] * psi[2 * x + 1];
+                double pi = psi[2 * y] * psi[2 * x + 1] - psi[2 * y + 1] * psi[2 * x];
+                if ((occ_number(s, anni) + occ_number(s_out, crea)) & 1) { sr -= pr; si -= pi; }
+                else { sr += pr; si += pi; }
+            }
+            rdm[2 * (i * nA + j)] = sr;
+            rdm[2 * (i * nA + j) + 1] = si;
+        }
+    for (int64_t i = 0; i < nA; ++i)
+        for (int64_t j = 0; j < i; ++j) {
+            rdm[2 * (i * nA + j)] = rdm[2 * (j * nA + i)];
+            rdm[2 * (i * nA + j) + 1] = -rdm[2 * (j * nA + i) + 1];
+        }
+}

@@ -63,6 +63,8 @@ def lib() -> ctypes.CDLL:
         L.orc_bracket2.argtypes = [i64, vp, vp, i64, vp, vp, i64, vp, vp, vp, vp, i64, vp, vp,
                                    vp, vp, vp]
         L.orc_bracket2.restype = None
+        L.orc_pes_block_rdm.argtypes = [i64, vp, vp, i64, vp, u64, vp]
+        L.orc_pes_block_rdm.restype = None
         _ = dbl
         _LIB = L
     return _LIB
@@ -217,6 +219,17 @@ def bracket2(bra_basis, bra, full2: TermList, full1: TermList, ket_basis, ket) -
                        _p(full2.amp), _p(full2.out), _p(full2.inn), _p(full2.parity), len(full1),
                        _p(full1.amp), _p(full1.out), _p(full1.inn), _p(full1.parity), _p(out))
     return complex(out[0], out[1])
+
+
+def pes_block_rdm(basis, psi, A_states, ph_mask: int = 0) -> np.ndarray:
+    """particle_reduced_density_matrix.jl:37-88: the particle reduced density matrix of psi
+    on the part-A block A_states (nA x nA, Hermitian)."""
+    b = np.ascontiguousarray(basis, dtype=np.uint64)
+    v = np.ascontiguousarray(psi, dtype=np.complex128)
+    A = np.ascontiguousarray(A_states, dtype=np.uint64)
+    out = np.zeros((len(A), len(A)), dtype=np.complex128)
+    lib().orc_pes_block_rdm(len(b), _p(b), _p(v), len(A), _p(A), int(ph_mask), _p(out))
+    return out
 
 
 def dense_matrix(basis: np.ndarray, full: TermList) -> np.ndarray:

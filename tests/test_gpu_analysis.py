@@ -229,3 +229,66 @@ def test_sphere_laughlin_unique_zero_mode_all_sectors(gpu_ctx, oracle_mod):
     ev = np.linalg.eigvalsh(oracle_mod.dense_matrix(sp.list, _oracle_full(oracle_mod, H)))[:5]
     np.testing.assert_allclose(E, ev, rtol=1e-10, atol=1e-12)
     assert abs(E[0]) < 1e-10 and E[1] > 1e-3
+
+
+def test_pes_block_rdm_oracle(gpu_ctx, oracle_mod):
+    """PES_MomtBlock_rdm on the device against the oracle's reference-order loops
+    (particle_reduced_density_matrix.jl:37-88) within 1e-13: every 2-particle block of a
+    complex state on the 3x5 mesh; a part holding all 5 particles (psi psi^H on psi's own
+    sector, zero elsewhere); and a two-component sector split (1, 1) | (1, 2)."""
+    w = W.small()
+    subs, ks = me.ED_momentum_subspaces(w.para, 5)
+    sp = subs[ks.index((1, 2))]
+    psi = me.MBS64Vector(_rand(sp.length, 11), sp)
+    parts, pks = me.ED_momentum_subspaces(w.para, 2)
+    assert len(parts) == 15
+    for A, k in zip(parts, pks):
+        got = me.PES_MomtBlock_rdm(w.para, psi, A)
+        ref = oracle_mod.pes_block_rdm(sp.list, psi.vec, A.list)
+        assert np.abs(ref).max() > 1e-3
+        assert np.abs(got - ref).max() <= 1e-13, k
+    whole = me.PES_MomtBlock_rdm(w.para, psi, sp)
+    assert np.abs(whole - np.outer(psi.vec, psi.vec.conj())).max() <= 1e-14
+    assert np.abs(me.PES_MomtBlock_rdm(w.para, psi, subs[ks.index((0, 0))])).max() == 0.0
+    para = W.batched_para(1.2)
+    subs2, ks2 = me.ED_momentum_subspaces(para, (2, 3))
+    sp2 = subs2[3]
+    psi2 = me.MBS64Vector(_rand(sp2.length, 12), sp2)
+    for A in me.ED_momentum_subspaces(para, (1, 1))[0][:5]:
+        got = me.PES_MomtBlock_rdm(para, psi2, A)
+        ref = oracle_mod.pes_block_rdm(sp2.list, psi2.vec, A.list)
+        assert np.abs(ref).max() > 1e-3
+        assert np.abs(got - ref).max() <= 1e-13
+
+
+def test_pes_block_rdm_sphere_laughlin_counting(gpu_ctx):
+    """The reference's sphere example (19 orbitals, 7 electrons, V_1 only): the particle
+    entanglement spectrum of the Laughlin zero mode for a part of 3 particles. In each of the
+    49 L_z blocks of part A the number of nonzero eigenvalues equals the number of
+    (1,3)-admissible 3-particle configurations with that L_z (455 in all, C(15, 3)); the
+    traces add up to C(7, 3) = 35."""
+    import itertools
+    import math
+    import test_oracle as T
+    para = T._sphere_para(19, [0.0, 1.0, 0.0, 0.0])
+    H = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    sp = me.ED_momentum_subspaces(para, 7, momentum_restriction=True,
+                                  momentum_list=[(63,)])[0][0]
+    E, V = me.EDsolve(sp, H, N=2, method_info=False)
+    assert abs(E[0]) < 1e-10 < E[1]
+    parts, pks = me.ED_momentum_subspaces(para, 3)
+    assert len(parts) == 49
+    total_trace, total_count = 0.0, 0
+    for A, k in zip(parts, pks):
+        rdm = me.PES_MomtBlock_rdm(para, V[0], A)
+        assert np.abs(rdm - rdm.conj().T).max() <= 1e-14
+        lam = np.linalg.eigvalsh(rdm)
+        assert lam.min() > -1e-12
+        count = int(np.sum(lam > 1e-10))
+        want = sum(1 for c in itertools.combinations(range(19), 3)
+                   if sum(c) == k[0] and c[1] - c[0] >= 3 and c[2] - c[1] >= 3)
+        assert count == want, (k, count, want)
+        total_count += count
+        total_trace += float(np.trace(rdm).real)
+    assert total_count == math.comb(15, 3) == 455
+    assert abs(total_trace - 35) <= 1e-10

@@ -6,6 +6,7 @@ brute force. A planted-mistake test shows the Jordan-Wigner check catches a sign
 """
 import itertools
 import json
+import math
 import os
 from fractions import Fraction
 
@@ -487,3 +488,121 @@ def test_oracle_matvec_equals_dense(oracle_mod):
         np.testing.assert_allclose(y, H @ x, rtol=0, atol=1e-13 * np.abs(y).max())
         rows = np.array([0, 5, len(basis) - 1])
         np.testing.assert_array_equal(oracle_mod.matvec(basis, full, x, rows=rows), y[rows])
+
+
+# ------------------------------------------------------------------ particle-block PES
+def _jw_block_rdm(basis, psi, A):
+    """<psi| C_j^+ C_i |psi> with C = c_{a1} c_{a2} ... (a1 < a2 < ...) applied with textbook
+    Jordan-Wigner strings (occupied orbitals below the one acted on)."""
+    phis = []
+    for a_state in A:
+        a_state = int(a_state)
+        orbs = sorted(o for o in range(64) if a_state >> o & 1)
+        phi = {}
+        for s, v in zip(basis, psi):
+            s = int(s)
+            if s & a_state != a_state:
+                continue
+            cur, sign = s, 1
+            for o in reversed(orbs):
+                if bin(cur & ((1 << o) - 1)).count("1") & 1:
+                    sign = -sign
+                cur ^= 1 << o
+            phi[cur] = sign * v
+        phis.append(phi)
+    n = len(A)
+    rdm = np.zeros((n, n), complex)
+    for i in range(n):
+        for j in range(n):
+            rdm[i, j] = sum(np.conj(phis[j][b]) * v for b, v in phis[i].items() if b in phis[j])
+    return rdm
+
+
+def test_pes_block_rdm_equals_jordan_wigner(oracle_mod):
+    """The oracle's PES_MomtBlock_rdm (particle_reduced_density_matrix.jl:37-88, signs from
+    scat_occ_number's paired mask) equals <psi|C_j^+ C_i|psi> built with plain Jordan-Wigner
+    strings, for parts of 1, 2 and 3 particles (unpaired, paired and mixed mask rules) and a
+    complex psi on a 3x4-mesh sector of 5 particles."""
+    rng = np.random.default_rng(8)
+    lists, ks = oracle_mod.enumerate_sectors(mesh(3, 4), (3, 4), 1, 1, [5])
+    basis = lists[ks.index((1, 2))]
+    psi = rng.normal(size=len(basis)) + 1j * rng.normal(size=len(basis))
+    psi /= np.linalg.norm(psi)
+    checked = 0
+    for NA in (1, 2, 3):
+        a_lists, a_ks = oracle_mod.enumerate_sectors(mesh(3, 4), (3, 4), 1, 1, [NA])
+        for A in a_lists[:4]:
+            got = oracle_mod.pes_block_rdm(basis, psi, A)
+            want = _jw_block_rdm(basis, psi, A)
+            assert np.abs(got - want).max() <= 1e-14
+            assert np.abs(want).max() > 1e-3
+            checked += 1
+    assert checked == 12
+
+
+def _laughlin_sphere(oracle_mod, n_orb=10, ne=4):
+    para = _sphere_para(n_orb, [0.0, 1.0])
+    lists, ks, full = _dense_from_terms(oracle_mod, para, [ne], [ED_scatterlist_twobody(para)])
+    basis = lists[ks.index((3 * (ne - 1) * ne // 2,))]
+    e, v = np.linalg.eigh(oracle_mod.dense_matrix(basis, full))
+    assert abs(e[0]) < 1e-10 < e[1]
+    return para, basis, v[:, 0]
+
+
+def _admissible_count(n_orb, NA, L):
+    """(1,3)-admissible configurations: NA of n_orb orbitals, neighbours at least 3 apart,
+    orbital indices summing to L."""
+    return sum(1 for c in itertools.combinations(range(n_orb), NA)
+               if sum(c) == L and all(b - a >= 3 for a, b in zip(c, c[1:])))
+
+
+def _pes_blocks(oracle_mod, para, basis, psi, NA):
+    a_lists, a_ks = oracle_mod.enumerate_sectors(para.k_list, para.Gk, 1, 1, [NA])
+    return {k[0]: oracle_mod.pes_block_rdm(basis, psi, A) for A, k in zip(a_lists, a_ks)}
+
+
+@pytest.mark.parametrize("NA", [1, 2, 3])
+def test_pes_block_rdm_laughlin_counting(oracle_mod, NA):
+    """Particle entanglement spectrum of the 4-electron 1/3 Laughlin state on the sphere
+    (2S = 9, the zero mode of V_1, total L_z index 18). For NA <= 2 the number of nonzero
+    eigenvalues in every L_z block of part A equals the number of (1,3)-admissible
+    configurations of NA particles with that L_z (the quasihole counting); for NA = 3 the
+    nonzero spectrum of block L equals that of the 1-particle block 18 - L (the Schmidt
+    decomposition pairs part and complement). The traces over all blocks add up to C(4, NA)
+    for a normalised psi."""
+    n_orb, ne = 10, 4
+    para, basis, psi = _laughlin_sphere(oracle_mod, n_orb, ne)
+    L_psi = 3 * (ne - 1) * ne // 2
+    blocks = _pes_blocks(oracle_mod, para, basis, psi, NA)
+    partner = _pes_blocks(oracle_mod, para, basis, psi, ne - NA) if NA > ne - NA else None
+    total_trace, total_count = 0.0, 0
+    for L, rdm in blocks.items():
+        assert np.abs(rdm - rdm.conj().T).max() <= 1e-15
+        w = np.linalg.eigvalsh(rdm)
+        assert w.min() > -1e-12
+        nz = np.sort(w[w > 1e-10])
+        if partner is None:
+            assert len(nz) == _admissible_count(n_orb, NA, L), (L, w)
+        else:
+            wp = np.linalg.eigvalsh(partner[L_psi - L]) if L_psi - L in partner else np.zeros(0)
+            wp = np.sort(wp[wp > 1e-10])
+            assert len(nz) == len(wp) == _admissible_count(n_orb, ne - NA, L_psi - L)
+            assert np.abs(nz - wp).max(initial=0.0) <= 1e-13
+        total_trace += float(np.trace(rdm).real)
+        total_count += len(nz)
+    assert abs(total_trace - math.comb(ne, NA)) <= 1e-11
+    m = min(NA, ne - NA)
+    assert total_count == math.comb(n_orb - 2 * (m - 1), m)
+
+
+def test_pes_block_rdm_whole_state(oracle_mod):
+    """A part holding every particle: the block of psi's own sector is psi psi^H (each state
+    is its own part, the sign (-1)^occ(s, s) is +1), every other block is zero."""
+    rng = np.random.default_rng(3)
+    lists, ks = oracle_mod.enumerate_sectors(mesh(3, 4), (3, 4), 1, 1, [3])
+    basis = lists[ks.index((2, 1))]
+    psi = rng.normal(size=len(basis)) + 1j * rng.normal(size=len(basis))
+    assert np.abs(oracle_mod.pes_block_rdm(basis, psi, basis)
+                  - np.outer(psi, psi.conj())).max() <= 1e-14
+    other = lists[ks.index((0, 0))]
+    assert np.abs(oracle_mod.pes_block_rdm(basis, psi, other)).max() == 0.0
