@@ -231,15 +231,19 @@ int edm_rdm1(edm_ctx *ctx, const edm_basis *b, const void *psi_dev, int32_t npai
              const int32_t *pairs, double *out);
 /* PES_MomtBlock_rdm (MomentumED/src/analysis/particle_reduced_density_matrix.jl:37-88)
  * factorised: the particle reduced density matrix of psi on the part-A block (A: nA host
- * states) is rdm = M M^H with M[a, b] = (-1)^occ(s, A[a]) psi[s] for s ^ ph_mask = A[a] | B[b]
- * (occ = scat_occ_number, EDCore/src/types/manybodystate_basis.jl:397-409). This writes M
- * (nA x D_B complex, row-major, device, zeroed by the caller) for psi (device vector in
- * psi_basis) and the complementary block B (every state with the remaining particles and the
- * remaining momentum); the product is one dense GEMM. EDM_EINVAL when a complement is not in
- * B. */
+ * states) is rdm = sum over complement blocks B of M M^H, with M[a, b] = (-1)^occ(s, A[a])
+ * psi[s] for s ^ ph_mask = A[a] | B[b] (occ = scat_occ_number of the untransformed state,
+ * EDCore/src/types/manybodystate_basis.jl:397-409; ph_mask = the bits of the particle-hole
+ * transformed components). This writes M (nA x D_B complex, row-major, device, zeroed by
+ * the caller) for psi (device vector in psi_basis) and one complement block B; the product
+ * is one dense GEMM. counts NULL: every (state, part) pair with the part inside the
+ * transformed state must find its complement in B (EDM_EINVAL otherwise). counts non-NULL:
+ * counts[0] = such pairs, counts[1] = those found in B, the others skipped, so that the
+ * caller can split the complements over several blocks and check that their counts[1] add
+ * up to counts[0]. */
 int edm_pes_block_matrix(edm_ctx *ctx, const edm_basis *psi_basis, const void *psi_dev,
                          int64_t nA, const uint64_t *A, uint64_t ph_mask, const edm_basis *B,
-                         void *m_dev);
+                         void *m_dev, int64_t *counts);
 /* ED_step_inner_prod (MomentumED/src/analysis/manybody_quantum_geometry.jl:43-91): for g
  * vectors each (device pointers vf_dev[a], vi_dev[b] in basis b) and orbital inner products
  * u[nbits] (host, complex), out[a*g+b] = sum_x conj(f_a[x]) i_b[x] prod_{k occupied in x} u[k]. */

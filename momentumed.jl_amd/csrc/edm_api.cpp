@@ -776,38 +776,43 @@ int edm_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_b
 
 int edm_pes_block_matrix(edm_ctx *ctx, const edm_basis *psi_basis, const void *psi_dev,
                          int64_t nA, const uint64_t *A, uint64_t ph_mask, const edm_basis *B,
-                         void *m_dev)
+                         void *m_dev, int64_t *counts)
 {
     return guarded(ctx, [&]() -> int {
         EDM_REQUIRE(ctx && psi_basis && B && (A || nA == 0), EDM_EINVAL, "NULL argument");
         EDM_REQUIRE(psi_basis->ctx == ctx && B->ctx == ctx, EDM_EINVAL,
                     "handles from another context");
         EDM_REQUIRE(nA >= 0, EDM_EINVAL, "nA must be >= 0");
-        if (nA == 0 || psi_basis->D == 0 || B->D == 0) return EDM_OK;
-        EDM_REQUIRE(psi_dev && m_dev, EDM_EINVAL, "NULL vector");
+        if (counts) counts[0] = counts[1] = 0;
+        if (nA == 0 || psi_basis->D == 0) return EDM_OK;
+        EDM_REQUIRE(psi_dev && (m_dev || B->D == 0), EDM_EINVAL, "NULL vector");
         EDM_REQUIRE(nA <= INT64_MAX / psi_basis->D, EDM_EINVAL, "D * nA overflows int64");
-        const size_t ab = (size_t)nA * sizeof(uint64_t);
+        const size_t ab = (size_t)nA * sizeof(uint64_t), cb = 2 * sizeof(unsigned long long);
         uint64_t *d_A = (uint64_t *)ctx->alloc(ab);
-        unsigned int *d_miss = nullptr;
-        unsigned int miss = 0;
+        unsigned long long *d_counts = nullptr;
+        unsigned long long h[2] = {0, 0};
         try {
-            d_miss = (unsigned int *)ctx->alloc(sizeof(unsigned int));
+            d_counts = (unsigned long long *)ctx->alloc(cb);
             EDM_HIP_CHECK(hipMemcpyAsync(d_A, A, ab, hipMemcpyHostToDevice, ctx->stream));
-            EDM_HIP_CHECK(hipMemsetAsync(d_miss, 0, sizeof(unsigned int), ctx->stream));
+            EDM_HIP_CHECK(hipMemsetAsync(d_counts, 0, cb, ctx->stream));
             launch_pes_fill(ctx, psi_basis, (const double2 *)psi_dev, nA, d_A, ph_mask, B,
-                            (double2 *)m_dev, d_miss);
-            EDM_HIP_CHECK(hipMemcpyAsync(&miss, d_miss, sizeof(miss), hipMemcpyDeviceToHost,
-                                         ctx->stream));
+                            (double2 *)m_dev, d_counts);
+            EDM_HIP_CHECK(hipMemcpyAsync(h, d_counts, cb, hipMemcpyDeviceToHost, ctx->stream));
             EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
         } catch (...) {
             (void)hipStreamSynchronize(ctx->stream);
             ctx->free(d_A, ab);
-            if (d_miss) ctx->free(d_miss, sizeof(unsigned int));
+            if (d_counts) ctx->free(d_counts, cb);
             throw;
         }
         ctx->free(d_A, ab);
-        ctx->free(d_miss, sizeof(unsigned int));
-        EDM_REQUIRE(miss == 0, EDM_EINVAL,
+        ctx->free(d_counts, cb);
+        if (counts) {
+            counts[0] = (int64_t)h[0];
+            counts[1] = (int64_t)h[1];
+            return EDM_OK;
+        }
+        EDM_REQUIRE(h[0] == h[1], EDM_EINVAL,
                     "a complement state is missing from the block B (psi's subspace must hold "
                     "one momentum sector and B the complementary block)");
         return EDM_OK;

@@ -261,10 +261,11 @@ void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const e
                      const edm_basis *ket, const double2 *d_bra, const double2 *d_ket,
                      double2 *d_out);
 // M (nA x B->D, row-major, zeroed by the caller) of the particle reduced density matrix
-// rdm = M M^H; *d_misses counts complements not found in B
+// rdm = M M^H; d_counts[0] += (state, part) pairs with the part inside the transformed state,
+// d_counts[1] += those whose complement is in B
 void launch_pes_fill(edm_ctx *ctx, const edm_basis *psi_b, const double2 *d_psi, int64_t nA,
                      const uint64_t *d_A, uint64_t ph_mask, const edm_basis *B, double2 *d_M,
-                     unsigned int *d_misses);
+                     unsigned long long *d_counts);
 void launch_step_gram(edm_ctx *ctx, const edm_basis *b, const double2 *d_u,
                       const double2 *const *d_vf, const double2 *const *d_vi, int g, double2 *d_out);
 struct edm_csr {

@@ -1055,22 +1055,32 @@ __device__ __forceinline__ uint32_t occ_parity(uint64_t n, uint64_t m)
 __global__ void __launch_bounds__(kRowBlock)
 k_pes_fill(int64_t D, const uint64_t *__restrict__ states, const double2 *__restrict__ psi,
            int64_t nA, const uint64_t *__restrict__ A, uint64_t ph, RankSearch bsearch,
-           double2 *__restrict__ M, unsigned int *__restrict__ misses)
+           double2 *__restrict__ M, unsigned long long *__restrict__ counts)
 {
+    // counts[0]: (state, part) pairs with the part inside the (transformed) state;
+    // counts[1]: those whose complement is in B
+    __shared__ unsigned long long block_counts[2];
+    if (threadIdx.x < 2) block_counts[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned long long applicable = 0, found = 0;
     const int64_t n = D * nA, stride = (int64_t)gridDim.x * kRowBlock;
     for (int64_t t = (int64_t)blockIdx.x * kRowBlock + threadIdx.x; t < n; t += stride) {
         const int64_t x = t / nA, a = t - x * nA;
         const uint64_t s = states[x], sph = s ^ ph, am = A[a];
         if ((sph & am) != am) continue;
+        ++applicable;
         const int64_t jb = bsearch(sph & ~am);
-        if (jb < 0) {
-            atomicAdd(misses, 1u);
-            continue;
-        }
+        if (jb < 0) continue;
+        ++found;
         const double2 v = psi[x];
         const uint32_t neg = occ_parity(s, am);
         M[a * bsearch.D + jb] = make_double2(flip_sign(v.x, neg), flip_sign(v.y, neg));
     }
+    if (applicable) atomicAdd(&block_counts[0], applicable);
+    if (found) atomicAdd(&block_counts[1], found);
+    __syncthreads();
+    if (threadIdx.x < 2 && block_counts[threadIdx.x])
+        atomicAdd(&counts[threadIdx.x], block_counts[threadIdx.x]);
 }
 
 // out[p] = sum over chunks of partial[p * nchunks + c], in chunk order
@@ -1586,14 +1596,14 @@ void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const e
 
 void launch_pes_fill(edm_ctx *ctx, const edm_basis *psi_b, const double2 *d_psi, int64_t nA,
                      const uint64_t *d_A, uint64_t ph_mask, const edm_basis *B, double2 *d_M,
-                     unsigned int *d_misses)
+                     unsigned long long *d_counts)
 {
     const int64_t n = psi_b->D * nA;
     if (n == 0) return;
     const int64_t blocks = std::min<int64_t>((n + kRowBlock - 1) / kRowBlock, 1 << 20);
     hipLaunchKernelGGL(k_pes_fill, dim3((unsigned)blocks),
                        dim3(kRowBlock), 0, ctx->stream, psi_b->D, psi_b->d_states, d_psi, nA,
-                       d_A, ph_mask, make_search(B), d_M, d_misses);
+                       d_A, ph_mask, make_search(B), d_M, d_counts);
     EDM_HIP_CHECK(hipGetLastError());
 }
 

@@ -110,57 +110,82 @@ def PES_1rdm(psi: MBS64Vector) -> np.ndarray:
 def PES_MomtBlock_rdm(para, psi: MBS64Vector, part_subspace, ph_transform=None) -> np.ndarray:
     """Particle reduced density matrix of psi on one momentum block of part A
     (particle_reduced_density_matrix.jl:37-88), |A| x |A| Hermitian. Computed on the device
-    as M M^H: M[a, b] = (-1)^occ(s, a) psi[s] for s = a | b with b in the complementary block
-    (the remaining particles of every conserved component, the remaining momentum), filled
-    by edm_pes_block_matrix, and the product as one dense GEMM on the GPU. The
-    particle-hole transformed form (ph_transform true for a component) is not provided."""
+    as M M^H: M[a, b] = (-1)^occ(s, a) psi[s] for s ^ ph = a | b with b in the complementary
+    block, filled by edm_pes_block_matrix, and the product as one dense GEMM on the GPU.
+
+    ph_transform (one flag per conserved component, default all false) particle-hole
+    transforms the flagged components before the split, as the reference does: part A's
+    states and the complement are then occupation patterns of the transformed state, and the
+    signs are taken from the untransformed states. The complement is the block of the
+    remaining particles of every component of the transformed state and the remaining
+    momentum."""
     import torch
     from .preparation import ED_momentum_subspaces, MBS_totalmomentum, momentum_residue
-    if ph_transform is not None and any(bool(t) for t in ph_transform):
-        raise NotImplementedError("particle-hole transformed parts are not provided")
-    if ph_transform is not None and len(ph_transform) != para.Nc_conserve:
+    if ph_transform is None:
+        ph_transform = [False] * para.Nc_conserve
+    if len(ph_transform) != para.Nc_conserve:
         raise AssertionError("lengths of subspaces and particle_hole should == "
                              "para.Nc_conserve.")
     if psi.space.bits != para.Nk * para.Nc:
         raise AssertionError("bits == para.Nk * para.Nc")
+    width = para.Nk * para.Nc_mix
+    comp = (1 << width) - 1
+    ph = 0
+    for c, flag in enumerate(ph_transform):
+        if flag:
+            ph |= comp << (c * width)
     A = np.ascontiguousarray(part_subspace.list, dtype=np.uint64)
     nA = len(A)
     rdm = np.zeros((nA, nA), dtype=np.complex128)
     if nA == 0 or len(psi.vec) == 0:
         return rdm
-    width = para.Nk * para.Nc_mix
-    comp = (1 << width) - 1
 
-    def counts(state):
+    def numbers(state):
         return tuple(bin((int(state) >> (c * width)) & comp).count("1")
                      for c in range(para.Nc_conserve))
-    n_psi, n_A = counts(psi.space.list[0]), counts(A[0])
-    n_B = tuple(a - b for a, b in zip(n_psi, n_A))
+    s0 = int(psi.space.list[0]) ^ ph        # every state of a sector gives the same numbers
+    n_B = tuple(a - b for a, b in zip(numbers(s0), numbers(A[0])))
     if any(v < 0 for v in n_B):
         return rdm
-    k_psi = MBS_totalmomentum(para, int(psi.space.list[0]))
-    k_A = MBS_totalmomentum(para, int(A[0]))
-    k_B = momentum_residue(tuple(a - b for a, b in zip(k_psi, k_A)), para.Gk)
-    if sum(n_B) == 0:   # part A holds every particle: the complement is the empty state
-        if any(k_B):
-            return rdm
-        B = HilbertSubspace([0], bits=psi.space.bits)
-    else:
-        subs, _ = ED_momentum_subspaces(para, n_B, momentum_restriction=True,
-                                        momentum_list=[k_B])
-        if not subs:
-            return rdm
-        B = subs[0]
+    k_s, k_A = MBS_totalmomentum(para, s0), MBS_totalmomentum(para, int(A[0]))
+    k_B = momentum_residue(tuple(a - b for a, b in zip(k_s, k_A)), para.Gk)
     ctx = get_context(0)
     pb = device_basis_of(psi.space, ctx)
-    bb = device_basis_of(B, ctx)
     x = _dev(psi.vec, ctx)
-    M = torch.zeros((nA, B.length), dtype=torch.complex128, device=x.device)
-    torch.cuda.current_stream(x.device).synchronize()
-    check(ctx.L.edm_pes_block_matrix(ctx.h, pb.h, x.data_ptr(), nA, A.ctypes.data, 0, bb.h,
-                                     M.data_ptr()), ctx.h)
+    acc = torch.zeros((nA, nA), dtype=torch.complex128, device=x.device)
+    counts = np.zeros(2, dtype=np.int64)
+    found = 0
+
+    def add_block(B):
+        nonlocal found
+        M = torch.zeros((nA, max(B.length, 1)), dtype=torch.complex128, device=x.device)
+        torch.cuda.current_stream(x.device).synchronize()
+        check(ctx.L.edm_pes_block_matrix(ctx.h, pb.h, x.data_ptr(), nA, A.ctypes.data, ph,
+                                         device_basis_of(B, ctx).h, M.data_ptr(),
+                                         counts.ctypes.data), ctx.h)
+        found += int(counts[1])
+        if counts[1]:
+            acc.add_(M @ M.conj().T)
+    # the complement block of the first state; with particle-hole transformed components
+    # and several conserved components the complements of one sector spread over several
+    # momenta, and the other blocks are visited as well
+    if sum(n_B) == 0:   # part A holds every particle: the complement is the empty state
+        add_block(HilbertSubspace([0], bits=psi.space.bits))
+    else:
+        subs, ks = ED_momentum_subspaces(para, n_B, momentum_restriction=True,
+                                         momentum_list=[k_B])
+        # an empty block still counts the (state, part) pairs
+        add_block(subs[0] if subs else HilbertSubspace([], bits=psi.space.bits))
+        if found < counts[0]:
+            subs, ks = ED_momentum_subspaces(para, n_B)
+            for sub, k in zip(subs, ks):
+                if k != k_B:
+                    add_block(sub)
+    if found != counts[0]:
+        raise ValueError("psi's states do not share the particle numbers of part A plus one "
+                         "complement (psi must live in one particle-number sector)")
     ctx.synchronize()
-    return (M @ M.conj().T).cpu().numpy()
+    return acc.cpu().numpy()
 
 
 def ED_step_inner_prod(psi_f, psi_i, orbital_inner_prod):

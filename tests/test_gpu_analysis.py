@@ -292,3 +292,32 @@ def test_pes_block_rdm_sphere_laughlin_counting(gpu_ctx):
         total_trace += float(np.trace(rdm).real)
     assert total_count == math.comb(15, 3) == 455
     assert abs(total_trace - 35) <= 1e-10
+
+
+def test_pes_block_rdm_particle_hole_oracle(gpu_ctx, oracle_mod):
+    """ph_transform on the device against the oracle within 1e-13: the single component of
+    the 3x5 mesh flipped (5 particles -> 10 holes, parts of 2 holes), and the second of two
+    conserved components on the 3x4 mesh flipped, where the complements of one sector spread
+    over several momentum blocks."""
+    w = W.small()
+    subs, ks = me.ED_momentum_subspaces(w.para, 5)
+    sp = subs[ks.index((2, 1))]
+    psi = me.MBS64Vector(_rand(sp.length, 21), sp)
+    ph = (1 << 15) - 1
+    for A in me.ED_momentum_subspaces(w.para, 2)[0][:6]:
+        got = me.PES_MomtBlock_rdm(w.para, psi, A, [True])
+        ref = oracle_mod.pes_block_rdm(sp.list, psi.vec, A.list, ph)
+        assert np.abs(ref).max() > 1e-3
+        assert np.abs(got - ref).max() <= 1e-13
+    para = W.batched_para(0.8)
+    sp2 = me.ED_momentum_subspaces(para, (2, 3))[0][5]
+    psi2 = me.MBS64Vector(_rand(sp2.length, 22), sp2)
+    ph2 = ((1 << 12) - 1) << 12
+    trace = 0.0
+    for A in me.ED_momentum_subspaces(para, (1, 1))[0]:
+        got = me.PES_MomtBlock_rdm(para, psi2, A, [False, True])
+        ref = oracle_mod.pes_block_rdm(sp2.list, psi2.vec, A.list, ph2)
+        assert np.abs(ref).max() > 1e-3
+        assert np.abs(got - ref).max() <= 1e-13
+        trace += float(np.trace(got).real)
+    assert abs(trace - 2 * 9) <= 1e-11

@@ -606,3 +606,45 @@ def test_pes_block_rdm_whole_state(oracle_mod):
                   - np.outer(psi, psi.conj())).max() <= 1e-14
     other = lists[ks.index((0, 0))]
     assert np.abs(oracle_mod.pes_block_rdm(basis, psi, other)).max() == 0.0
+
+
+def _occ_mask(a_state):
+    """scat_occ_number's mask for a part state (set bits paired from the lowest; bits
+    strictly between a pair; an unpaired top bit covers every bit above it)."""
+    bits = [o for o in range(64) if a_state >> o & 1]
+    m = 0
+    for p in range(0, len(bits), 2):
+        lo = bits[p]
+        hi = bits[p + 1] if p + 1 < len(bits) else 64
+        for o in range(lo + 1, hi):
+            m |= 1 << o
+    return m
+
+
+def test_pes_block_rdm_particle_hole(oracle_mod):
+    """Particle-hole transformed parts (the reference's ph_transform): for a flipped
+    component the part and its complement are patterns of the transformed state and the
+    signs come from the untransformed states. Then the block equals D R D, where R is the
+    plain block of the explicitly transformed state psi'[s ^ ph] = psi[s] (pinned against
+    Jordan-Wigner strings above) and D[i, i] = (-1)^popcount(ph & mask(A_i)). Two conserved
+    components on a 3x2 mesh, N = (2, 3), second component flipped (3 holes); traces over all
+    (1, 1) part blocks add up to 2 * 3."""
+    rng = np.random.default_rng(17)
+    k = mesh(3, 2)
+    lists, ks = oracle_mod.enumerate_sectors(k, (3, 2), 1, 2, [2, 3])
+    basis = lists[2]
+    psi = rng.normal(size=len(basis)) + 1j * rng.normal(size=len(basis))
+    psi /= np.linalg.norm(psi)
+    ph = ((1 << 6) - 1) << 6
+    flipped = np.array([int(s) ^ ph for s in basis], dtype=np.uint64)
+    order = np.argsort(flipped)
+    a_lists, _ = oracle_mod.enumerate_sectors(k, (3, 2), 1, 2, [1, 1])
+    trace = 0.0
+    for A in a_lists:
+        got = oracle_mod.pes_block_rdm(basis, psi, A, ph)
+        plain = oracle_mod.pes_block_rdm(flipped[order], psi[order], A, 0)
+        d = np.array([(-1.0) ** bin(ph & _occ_mask(int(a))).count("1") for a in A])
+        assert np.abs(got - d[:, None] * plain * d[None, :]).max() <= 1e-15
+        assert np.linalg.eigvalsh(got).min() > -1e-14
+        trace += float(np.trace(got).real)
+    assert abs(trace - 6.0) <= 1e-13
