@@ -7,8 +7,11 @@ Drop-in mirror of the MomentumED.jl API for the matrix-free device path:
     energies, vectors = EDsolve(subspaces[0], H, N=6, method="map", device="gpu")
 The compute runs in libedm.so (hand-written gfx950 HIP kernels behind a C-ABI).
 """
-from .core import (HilbertSubspace, MBOperator, MBS64, MBS64Vector, Scatter, get_body,
-                   index_fit, isdiagonal, isupper, make_mask64, occ_list, parity_mask,
+from .core import (ColexMBS64, ColexMBS64Mask, HilbertSubspace, MBOperator, MBS64,
+                   MBS64_complete, MBS64Vector, Scatter, empty, flip, for_each_occ,
+                   get_bits, get_body,
+                   index_fit, isdiagonal, isempty, isoccupied, isphysical, isupper,
+                   make_mask64, occ_list, occupy, parity_mask, scat_occ_number,
                    sort_merge_scatlist)
 from .device import (DEFAULT_GPU, GPU_AVAILABLE, GPU_MEMORY_MONITOR, LinearMap,
                      SparseHmltMatrix, activate_gpu_device, create_gpu_linearmap,
@@ -32,4 +35,6 @@ __all__ = [
     "create_gpu_matrix", "krylov_matrix_solve", "GPU_MEMORY_MONITOR", "LinearMap",
     "SparseHmltMatrix", "EDsolve_batch", "ED_bracket", "ED_bracket_threaded",
     "ED_connection_gaugefixing", "ED_step_inner_prod", "PES_1rdm", "PES_MomtBlock_rdm",
+    "ColexMBS64", "ColexMBS64Mask", "MBS64_complete", "empty", "flip", "get_bits",
+    "isempty", "isoccupied", "isphysical", "occupy", "scat_occ_number", "for_each_occ",
 ]

@@ -1,9 +1,10 @@
 """Core algebra: occupation bitstrings, scattering terms, operators, subspaces, vectors.
 
 Host-side mirror of EDCore (EDCore/src/EDCore.jl:3-11). Orbital indices are 1-based as in
-the reference (orbital i is bit i-1 of a uint64 state). Only what the device path needs is
-kept; expectation values and analysis (EDCore/src/multiplication.jl:88-307) are not part of
-this package.
+the reference (orbital i is bit i-1 of a uint64 state). The Julia functions that end in "!"
+(occupy!, empty!, flip!) return new states in the reference too and are named here without
+the "!". Brackets and other eigenvector consumers (EDCore/src/multiplication.jl:153-307) run
+on the device and live in analysis.py.
 """
 from __future__ import annotations
 
@@ -14,7 +15,9 @@ import numpy as np
 __all__ = [
     "MBS64", "make_mask64", "occ_list", "parity_mask", "Scatter", "isupper", "isdiagonal",
     "get_body", "sort_merge_scatlist", "MBOperator", "HilbertSubspace", "MBS64Vector",
-    "index_fit",
+    "index_fit", "get_bits", "isphysical", "MBS64_complete", "isoccupied", "isempty",
+    "occupy", "empty", "flip", "scat_occ_number", "ColexMBS64", "ColexMBS64Mask",
+    "for_each_occ",
 ]
 
 
@@ -40,16 +43,27 @@ def occ_list(n: int) -> list[int]:
     return out
 
 
+def for_each_occ(f, mbs) -> None:
+    """manybodystate_basis.jl:142-149: f(i) for every occupied 1-based orbital, ascending."""
+    for i in occ_list(mbs.n if isinstance(mbs, MBS64) else mbs):
+        f(i)
+
+
 class MBS64:
     """manybodystate_basis.jl:24-32: a 64-bit occupation state with `bits` physical bits."""
 
     __slots__ = ("n", "bits")
 
-    def __init__(self, bits: int, occ=None, *, n: int | None = None):
+    def __init__(self, bits: int, occ=None, mask=None, *, n: int | None = None):
+        """MBS64(bits, occ) sets the 1-based orbitals in occ; MBS64(bits, occ, mask) the
+        orbitals mask[o - 1] for o in occ (manybodystate_basis.jl:195-209)."""
         if not 0 <= bits <= 64:
             raise AssertionError("The number of bits must be an integer between 0 and 64.")
         if n is None:
-            occ = sorted(occ or [])
+            occ = list(occ or [])
+            if mask is not None:
+                occ = [mask[o - 1] for o in occ]
+            occ = sorted(occ)
             if len(set(occ)) != len(occ):
                 raise AssertionError("Duplicate orbital indices when constructing MBS64.")
             n = make_mask64(occ)
@@ -73,9 +87,176 @@ class MBS64:
         """Concatenation (manybodystate_basis.jl:81-83): self takes the higher bits."""
         return MBS64(self.bits + other.bits, n=(self.n << other.bits) | other.n)
 
+    def __add__(self, other: "MBS64") -> "MBS64":
+        """Union of the occupations of two states of one space (manybodystate_basis.jl:
+        92-94); a doubly occupied orbital counts once."""
+        if self.bits != other.bits:
+            raise TypeError("+ needs two MBS64 of the same bits")
+        return MBS64(self.bits, n=self.n | other.n)
+
     def __repr__(self):
         s = format(self.n, f"0{max(self.bits, 1)}b")[::-1][: self.bits]
         return f"MBS64: {self.n} = {s} ({self.bits} bits)"
+
+
+def _mask_of(i) -> int:
+    """An orbital selection as a bit mask: an int is a mask already, a list or tuple holds
+    1-based orbitals."""
+    if isinstance(i, (int, np.integer)):
+        return int(i)
+    return make_mask64(i)
+
+
+def get_bits(x) -> int:
+    """manybodystate_basis.jl:53-55 (and the subspace / vector forms): physical bits."""
+    if isinstance(x, MBS64Vector):
+        return x.space.bits
+    return x.bits
+
+
+def isphysical(mbs: MBS64) -> bool:
+    """manybodystate_basis.jl:62-69: no bit at or above `bits` is set."""
+    return mbs.bits == 64 or mbs.n < (1 << mbs.bits)
+
+
+def MBS64_complete(mbs: MBS64) -> MBS64:
+    """manybodystate_basis.jl:216-218: every physical orbital flipped."""
+    return MBS64(mbs.bits, n=((1 << mbs.bits) - 1) ^ mbs.n)
+
+
+def isoccupied(mbs: MBS64, i) -> bool:
+    """manybodystate_basis.jl:228-238: every orbital of i (an int mask, or a list or tuple
+    of 1-based orbitals) occupied."""
+    m = _mask_of(i)
+    return mbs.n & m == m
+
+
+def isempty(mbs: MBS64, i) -> bool:
+    """manybodystate_basis.jl:256-266: every orbital of i empty."""
+    return mbs.n & _mask_of(i) == 0
+
+
+def occupy(mbs: MBS64, i, check: bool = True) -> MBS64:
+    """occupy! (manybodystate_basis.jl:285-301): a new state with the orbitals of i
+    occupied. The mask form checks nothing; the list form checks that they were empty
+    (unless check=False) and that the result fits in `bits`."""
+    if isinstance(i, (int, np.integer)):
+        return _raw(mbs.bits, mbs.n | int(i))
+    m = make_mask64(i)
+    if check and mbs.n & m:
+        raise AssertionError("Some orbitals are already occupied.")
+    return MBS64(mbs.bits, n=mbs.n | m)
+
+
+def empty(mbs: MBS64, i, check: bool = True) -> MBS64:
+    """empty! (manybodystate_basis.jl:326-342): a new state with the orbitals of i emptied;
+    the list form checks that they were occupied (unless check=False)."""
+    if isinstance(i, (int, np.integer)):
+        return _raw(mbs.bits, mbs.n & ~int(i) & ((1 << 64) - 1))
+    m = make_mask64(i)
+    if check and mbs.n & m != m:
+        raise AssertionError("Some orbitals are already empty.")
+    return MBS64(mbs.bits, n=mbs.n & ~m)
+
+
+def flip(mbs: MBS64, i) -> MBS64:
+    """flip! (manybodystate_basis.jl:365-375): occupations of the orbitals of i flipped."""
+    if isinstance(i, (int, np.integer)):
+        return _raw(mbs.bits, mbs.n ^ int(i))
+    return MBS64(mbs.bits, n=mbs.n ^ make_mask64(i))
+
+
+def _raw(bits: int, n: int) -> MBS64:
+    """A state without the range check (the reference's reinterpret of the mask forms)."""
+    out = MBS64.__new__(MBS64)
+    out.n, out.bits = int(n), bits
+    return out
+
+
+def scat_occ_number(mbs, i) -> int:
+    """manybodystate_basis.jl:397-459: the occupied orbitals of mbs (a state or a raw int)
+    that a product of operators on the orbitals of i passes. The set bits of i pair up from
+    the lowest; a pair covers the bits strictly between its two; an unpaired top bit covers
+    every bit above it."""
+    n = mbs.n if isinstance(mbs, MBS64) else int(mbs)
+    if not isinstance(i, (int, np.integer)):
+        if any(not 1 <= o <= (mbs.bits if isinstance(mbs, MBS64) else 64) for o in i):
+            raise AssertionError("Invalid bit positions")
+    bits = occ_list(_mask_of(i))
+    cover = 0
+    for p in range(0, len(bits), 2):
+        lo = bits[p]                                        # 1-based: bit lo - 1
+        hi = bits[p + 1] - 1 if p + 1 < len(bits) else 64   # exclusive upper bit
+        cover |= ((1 << hi) - 1) & ~((1 << lo) - 1)
+    return bin(n & cover).count("1")
+
+
+class ColexMBS64:
+    """colex_mbslist.jl:2-95: the states of n bits with t set, in colex order (= ascending
+    as integers), optionally only the ranks start..end (1-based, inclusive). Iterates by the
+    next-same-popcount bit step (Gosper's rule) from the first state, unranked by the
+    combinatorial number system."""
+
+    def __init__(self, n: int, t: int, start: int = 1, end: int | None = None):
+        import math
+        if not 0 <= n <= 64:
+            raise AssertionError("0 <= n <= 64")
+        if start < 1:
+            raise AssertionError("start_number > 0")
+        full = math.comb(n, t) if 0 <= t <= n else 0
+        end = full if end is None else min(end, full)
+        self.n, self.t, self.start, self.end = n, t, start, end
+
+    def __len__(self):
+        return max(0, self.end - self.start + 1)
+
+    def _unrank(self, k: int) -> int:
+        """Bit pattern of 0-based colex rank k."""
+        import math
+        x, state = k, 0
+        for i in range(self.t, 0, -1):
+            c = i - 1
+            while math.comb(c + 1, i) <= x:
+                c += 1
+            state |= 1 << c
+            x -= math.comb(c, i)
+        return state
+
+    def _patterns(self):
+        if len(self) == 0:
+            return
+        x = self._unrank(self.start - 1)
+        for _ in range(len(self)):
+            yield x
+            if x == 0:
+                return
+            low = x & -x
+            ripple = x + low
+            x = (((ripple ^ x) >> 2) // low) | ripple
+
+    def __iter__(self):
+        for x in self._patterns():
+            yield MBS64(self.n, n=x)
+
+
+class ColexMBS64Mask(ColexMBS64):
+    """colex_mbslist.jl:96-172: the same over the orbitals of `mask` (1-based, sorted,
+    unique): combinations of t of them in colex order of their positions in the mask, as
+    states of n bits (ascending, since the mask is)."""
+
+    def __init__(self, n: int, t: int, mask, start: int = 1, end: int | None = None):
+        self.mask = [int(o) for o in mask]
+        if any(b <= a for a, b in zip(self.mask, self.mask[1:])):
+            raise AssertionError("the mask must be sorted and unique")
+        if any(not 1 <= o <= n for o in self.mask):
+            raise AssertionError("mask orbitals must lie in 1..n")
+        super().__init__(len(self.mask), t, start, end)
+        self.bits_total = n
+
+    def __iter__(self):
+        for x in self._patterns():
+            yield MBS64(self.bits_total, [p + 1 for p in range(self.n) if x >> p & 1],
+                        self.mask)
 
 
 # ---------------------------------------------------------------------------- terms

@@ -101,6 +101,80 @@ def test_mbs64_reference_vectors():
         MBS64(4, n=16)
 
 
+def test_mbs64_manipulation_reference_vectors():
+    """EDCore/test/test_manybodystate_basis.jl:1-46: bits, physicality, union, equality and
+    order, occupation tests and updates, the sign count, flips."""
+    mbs = MBS64(8, n=10)                                     # 00001010
+    assert me.get_bits(mbs) == 8 and me.isphysical(mbs)
+    assert not me.isphysical(me.core._raw(8, 256))
+    assert MBS64(4, n=5) + MBS64(4, n=10) == MBS64(4, n=15)
+    assert MBS64(8, n=10) == MBS64(8, n=10) and MBS64(8, n=10) != MBS64(8, n=11)
+    assert MBS64(8, n=10) < MBS64(8, n=11)
+    assert MBS64(8, [2, 4]) == mbs and MBS64(5, [1, 2], [1, 3, 5]) == MBS64(5, [1, 3])
+    assert me.isoccupied(mbs, [2, 4]) and not me.isoccupied(mbs, [1, 2])
+    assert me.isempty(mbs, [1, 3]) and not me.isempty(mbs, [2, 3])
+    assert me.occupy(MBS64(8, n=0), [2, 4]) == mbs
+    assert me.empty(mbs, [2, 4]) == MBS64(8, n=0)
+    assert me.scat_occ_number(mbs, (2, 4)) == 0
+    assert me.scat_occ_number(mbs, (1, 5)) == 2
+    assert me.flip(mbs, [2, 4]) == MBS64(8, n=0)
+    assert me.flip(mbs, [1, 5]) == MBS64(8, n=27)            # 00011011
+    assert me.flip(MBS64(8, n=0), [2, 4]) == mbs
+    with pytest.raises(AssertionError):
+        me.occupy(mbs, [2, 3])
+    with pytest.raises(AssertionError):
+        me.empty(mbs, [1, 2])
+    assert me.MBS64_complete(mbs) == MBS64(8, n=0b11110101)
+    # mask forms take a raw bit mask and check nothing
+    assert me.occupy(mbs, 1 << 8, check=True).n == 10 | 256
+    assert me.isoccupied(mbs, 10) and me.isempty(mbs, 5)
+    seen = []
+    me.for_each_occ(seen.append, mbs)
+    assert seen == [2, 4]
+
+
+def test_scat_occ_number_matches_parity_mask(oracle_mod):
+    """The sign count of a state and an operator set equals popcount(state & parity_mask)
+    for every pair and single orbital of 10 bits (the single-orbital case: every bit above
+    it), in the mask and the list forms."""
+    rng = np.random.default_rng(5)
+    for n in rng.integers(0, 1 << 10, 40):
+        st = MBS64(10, n=int(n))
+        for a in range(1, 11):
+            want = bin(int(n) & oracle_mod.parity_mask(1 << (a - 1))).count("1")
+            assert me.scat_occ_number(st, (a,)) == want == me.scat_occ_number(st, 1 << (a - 1))
+            for b in range(a + 1, 11):
+                m = (1 << (a - 1)) | (1 << (b - 1))
+                want = bin(int(n) & oracle_mod.parity_mask(m)).count("1")
+                assert me.scat_occ_number(st, [b, a]) == want == me.scat_occ_number(int(n), m)
+
+
+def test_colex_iterators_reference_vectors():
+    """EDCore/test/test_colex_mbslist.jl:4-79: colex order, edge cases t = 0, t = n,
+    n = t = 0, sortedness; masked iteration; rank windows."""
+    import math
+    assert [m.n for m in me.ColexMBS64(4, 2)] == [MBS64(4, c).n for c in
+                                                  [(1, 2), (1, 3), (2, 3), (1, 4), (2, 4),
+                                                   (3, 4)]]
+    assert list(me.ColexMBS64(3, 0)) == [MBS64(3, [])]
+    assert list(me.ColexMBS64(3, 3)) == [MBS64(3, (1, 2, 3))]
+    assert list(me.ColexMBS64(0, 0)) == [MBS64(0, [])]
+    big = [m.n for m in me.ColexMBS64(5, 3)]
+    assert len(big) == math.comb(5, 3) and big == sorted(big)
+    assert list(me.ColexMBS64Mask(5, 2, [1, 3, 5])) == [MBS64(5, (1, 3)), MBS64(5, (1, 5)),
+                                                         MBS64(5, (3, 5))]
+    assert list(me.ColexMBS64Mask(5, 0, [1, 3, 5])) == [MBS64(5, [])]
+    assert list(me.ColexMBS64Mask(5, 2, [2, 4])) == [MBS64(5, (2, 4))]
+    lm = [m.n for m in me.ColexMBS64Mask(10, 3, [1, 2, 3, 4, 5, 6])]
+    assert len(lm) == 20 and lm == sorted(lm)
+    # every state of 12 bits with 5 set, in order, and windows of it from the unranking
+    full = [m.n for m in me.ColexMBS64(12, 5)]
+    assert full == [x for x in range(1 << 12) if bin(x).count("1") == 5]
+    for lo, hi in [(1, 1), (7, 300), (790, 792), (792, 900)]:
+        assert [m.n for m in me.ColexMBS64(12, 5, lo, hi)] == full[lo - 1:hi]
+    assert list(me.ColexMBS64(12, 5, 800, 900)) == [] and len(me.ColexMBS64(4, 5)) == 0
+
+
 def test_parity_mask_matches_oracle(oracle_mod):
     rng = np.random.default_rng(0)
     for _ in range(500):
