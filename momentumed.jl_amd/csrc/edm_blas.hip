@@ -38,34 +38,90 @@ __device__ __forceinline__ double2 block_sum2(double a, double b, double2 *sh)
     return r;
 }
 
-// partial[i * nb + blk] = sum over the block's rows of conj(V_i[r]) * w[r]
-__global__ void __launch_bounds__(kBlk)
-k_multidot_partial(const double2 *const *__restrict__ vptrs, int nvec,
-                   const double2 *__restrict__ w, int64_t D, double2 *__restrict__ partial)
+// The multi-dot of one reduction tile (kTile rows from `base`, D rows in all) for vectors
+// i0 <= i < i1: out[i * stride] = sum over the tile's rows of conj(V_i[r]) * w[r]. Per vector
+// the sum is the fixed-order block sum of block_sum2 (each thread's kRpt rows in order, the
+// wave butterfly, then the waves in order), whichever block computes it; the vectors go
+// kGroup at a time, so a group's loads are in flight together and one pair of barriers
+// serves the group.
+constexpr int kGroup = 4;          // vectors per reduction round
+constexpr int kMinDotBlocks = 2048;   // blocks a multi-dot grid should give the device
+
+// Vectors per block of a multi-dot over nb tiles: all of them when the tiles alone fill the
+// device, else split (in whole groups) over a second grid dimension.
+inline int dot_span(int64_t nb, int nvec)
 {
-    __shared__ double2 sh[kBlk / 64];
-    int64_t base = (int64_t)blockIdx.x * kTile + threadIdx.x;
+    const int groups = (nvec + kGroup - 1) / kGroup;
+    int64_t ny = (kMinDotBlocks + nb - 1) / std::max<int64_t>(nb, 1);
+    ny = std::max<int64_t>(1, std::min<int64_t>(ny, groups));
+    return kGroup * (int)((groups + ny - 1) / ny);
+}
+
+__device__ __forceinline__ void multidot_tile(const double2 *const *__restrict__ vptrs, int i0,
+                                              int i1, const double2 *__restrict__ w, int64_t D,
+                                              int64_t base, double2 *__restrict__ out,
+                                              int64_t stride)
+{
+    __shared__ double2 sh[kGroup][kBlk / 64];
     double2 wv[kRpt];
+    bool in[kRpt];
 #pragma unroll
     for (int k = 0; k < kRpt; ++k) {
         int64_t r = base + (int64_t)k * kBlk;
-        wv[k] = r < D ? w[r] : make_double2(0.0, 0.0);
+        in[k] = r < D;
+        wv[k] = in[k] ? w[r] : make_double2(0.0, 0.0);
     }
-    for (int i = 0; i < nvec; ++i) {
-        const double2 *v = vptrs[i];
-        double ar = 0.0, ai = 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int g0 = i0; g0 < i1; g0 += kGroup) {
+        double2 a[kGroup][kRpt];
 #pragma unroll
-        for (int k = 0; k < kRpt; ++k) {
-            int64_t r = base + (int64_t)k * kBlk;
-            if (r < D) {
-                double2 a = v[r];
-                ar += a.x * wv[k].x + a.y * wv[k].y;
-                ai += a.x * wv[k].y - a.y * wv[k].x;
-            }
+        for (int g = 0; g < kGroup; ++g) {
+            const double2 *v = vptrs[g0 + g < i1 ? g0 + g : g0];
+#pragma unroll
+            for (int k = 0; k < kRpt; ++k)
+                a[g][k] = in[k] ? v[base + (int64_t)k * kBlk] : make_double2(0.0, 0.0);
         }
-        double2 s = block_sum2(ar, ai, sh);
-        if (threadIdx.x == 0) partial[(int64_t)i * gridDim.x + blockIdx.x] = s;
+        double ar[kGroup], ai[kGroup];
+#pragma unroll
+        for (int g = 0; g < kGroup; ++g) {
+            ar[g] = 0.0;
+            ai[g] = 0.0;
+#pragma unroll
+            for (int k = 0; k < kRpt; ++k) {
+                if (in[k]) {
+                    ar[g] += a[g][k].x * wv[k].x + a[g][k].y * wv[k].y;
+                    ai[g] += a[g][k].x * wv[k].y - a[g][k].y * wv[k].x;
+                }
+            }
+            ar[g] = wave_sum(ar[g]);
+            ai[g] = wave_sum(ai[g]);
+        }
+        __syncthreads();   // the previous round's sums have been read
+        if (lane == 0) {
+#pragma unroll
+            for (int g = 0; g < kGroup; ++g) sh[g][wid] = make_double2(ar[g], ai[g]);
+        }
+        __syncthreads();
+        if (threadIdx.x < kGroup && g0 + (int)threadIdx.x < i1) {
+            double2 r = make_double2(0.0, 0.0);
+            for (int q = 0; q < (int)(blockDim.x >> 6); ++q) {
+                r.x += sh[threadIdx.x][q].x;
+                r.y += sh[threadIdx.x][q].y;
+            }
+            out[(int64_t)(g0 + threadIdx.x) * stride] = r;
+        }
     }
+}
+
+// partial[i * nb + blk] = sum over the block's rows of conj(V_i[r]) * w[r]; block (blk, y)
+// takes the vectors [y * span, (y + 1) * span)
+__global__ void __launch_bounds__(kBlk)
+k_multidot_partial(const double2 *const *__restrict__ vptrs, int nvec, int span,
+                   const double2 *__restrict__ w, int64_t D, double2 *__restrict__ partial)
+{
+    const int i0 = (int)blockIdx.y * span, i1 = min(nvec, i0 + span);
+    multidot_tile(vptrs, i0, i1, w, D, (int64_t)blockIdx.x * kTile + threadIdx.x,
+                  partial + blockIdx.x, gridDim.x);
 }
 
 // out[i] = sum_b partial[i * nb + b], one block per output, fixed order
@@ -223,33 +279,14 @@ k_fill_random(double2 *__restrict__ v, int64_t D, uint64_t seed, int mode)
 // live in the problem's own region and are reduced in tile order, so every problem's result
 // is bitwise what the single-problem kernels above produce, whatever else shares the launch.
 __global__ void __launch_bounds__(kBlk)
-k_bmultidot_partial(const BatchVecDesc *__restrict__ descs, const int *__restrict__ tile2prob)
+k_bmultidot_partial(const BatchVecDesc *__restrict__ descs, const int *__restrict__ tile2prob,
+                    int span)
 {
-    __shared__ double2 sh[kBlk / 64];
     const BatchVecDesc &P = descs[tile2prob[blockIdx.x]];
     int64_t lt = (int64_t)blockIdx.x - P.tile0;
-    int64_t base = lt * kTile + threadIdx.x;
-    double2 wv[kRpt];
-#pragma unroll
-    for (int k = 0; k < kRpt; ++k) {
-        int64_t r = base + (int64_t)k * kBlk;
-        wv[k] = r < P.D ? P.w[r] : make_double2(0.0, 0.0);
-    }
-    for (int i = 0; i < P.nvec; ++i) {
-        const double2 *v = P.V[i];
-        double ar = 0.0, ai = 0.0;
-#pragma unroll
-        for (int k = 0; k < kRpt; ++k) {
-            int64_t r = base + (int64_t)k * kBlk;
-            if (r < P.D) {
-                double2 a = v[r];
-                ar += a.x * wv[k].x + a.y * wv[k].y;
-                ai += a.x * wv[k].y - a.y * wv[k].x;
-            }
-        }
-        double2 s = block_sum2(ar, ai, sh);
-        if (threadIdx.x == 0) P.partial[(int64_t)i * P.ntiles + lt] = s;
-    }
+    const int i0 = (int)blockIdx.y * span, i1 = min(P.nvec, i0 + span);
+    if (i0 >= i1) return;   // uniform over the block
+    multidot_tile(P.V, i0, i1, P.w, P.D, lt * kTile + threadIdx.x, P.partial + lt, P.ntiles);
 }
 
 // out[i] = sum of partials, block (problem, i); which = 0 -> h, 1 -> h2
@@ -376,8 +413,10 @@ void blas_multidot(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const 
     if (nvec == 0) return;
     unsigned nb = std::max(1u, tile_blocks(D));
     double2 *part = (double2 *)ctx->partials((size_t)nb * nvec * sizeof(double2));
-    hipLaunchKernelGGL(k_multidot_partial, dim3(nb), dim3(kBlk), 0, ctx->stream, d_vptrs, nvec,
-                       w, D, part);
+    const int span = dot_span(nb, nvec);
+    const unsigned ny = (unsigned)((nvec + span - 1) / span);
+    hipLaunchKernelGGL(k_multidot_partial, dim3(nb, ny), dim3(kBlk), 0, ctx->stream, d_vptrs,
+                       nvec, span, w, D, part);
     hipLaunchKernelGGL(k_reduce_partials, dim3(nvec), dim3(kBlk), 0, ctx->stream, part,
                        (int64_t)nb, d_h);
     EDM_HIP_CHECK(hipGetLastError());
@@ -418,9 +457,11 @@ void bblas_cgs2_norm(edm_ctx *ctx, const BatchVecDesc *d_descs, int nprob, int m
     EDM_REQUIRE(maxvec <= kMaxVec, EDM_EINVAL, "too many vectors in one batched multi-dot");
     if (nprob == 0) return;
     hipStream_t st = ctx->stream;
+    const int span = dot_span(ntiles, std::max(maxvec, 1));
+    const unsigned ny = (unsigned)((std::max(maxvec, 1) + span - 1) / span);
     for (int pass = 0; pass < 2; ++pass) {
-        hipLaunchKernelGGL(k_bmultidot_partial, dim3((unsigned)ntiles), dim3(kBlk), 0, st, d_descs,
-                           d_tile2prob);
+        hipLaunchKernelGGL(k_bmultidot_partial, dim3((unsigned)ntiles, ny), dim3(kBlk), 0, st,
+                           d_descs, d_tile2prob, span);
         hipLaunchKernelGGL(k_breduce, dim3((unsigned)nprob, (unsigned)maxvec), dim3(kBlk), 0, st,
                            d_descs, pass);
         hipLaunchKernelGGL(k_baxpy, dim3((unsigned)nblk), dim3(kBlk), 0, st, d_descs, d_blk2prob,
