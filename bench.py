@@ -37,9 +37,12 @@ def log(*a):
 
 def _kernel_name(rank_mode: int, block: int = 512) -> str:
     """Name of the matvec kernel libedm.so launches for the bucketed path (EDM_MATVEC, or the
-    tuner's code: rows per workgroup | 0x10000 for the LDS-amplitude kernel)."""
+    tuner's code: rows per workgroup | 0x10000 for the LDS-amplitude kernel | lanes per row
+    << 20 for the split-row kernel)."""
     rank = "RankLin" if rank_mode == 1 else "RankSearch"
     v = os.environ.get("EDM_MATVEC", "u1")
+    if v == "u1" and (block >> 20) > 1:
+        return f"k_matvec_split<{rank}, {(block >> 20) & 0xFF}>"
     if v == "u1a" or (v == "u1" and block & 0x10000):
         return f"k_matvec_ldsa<{rank}, {block & 0xFFFF if block & 0x10000 else 1024}>"
     if v == "basic":

@@ -169,17 +169,23 @@ int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, 
                             int64_t *nnz_offdiag);
 
 /* Autotuner (no reference counterpart; the reference has one GPU kernel configuration): times
- * the default matvec kernel with 256-, 512- and 1024-row workgroups on this operator and
- * basis, and the variant that also stages the (distinct) amplitudes in LDS at the same three
- * sizes when they fit (three launches each, on scratch vectors), and uses the fastest for
- * every later edm_matvec / edm_eigsolve on the pair. The per-row arithmetic is the same in
- * all six, so y does not change. *kernel = chosen rows per workgroup, | EDM_TUNE_AMP_LDS for
- * the LDS-amplitude variant (0 when the pair runs another kernel); ms6 (may be NULL) = mean
- * milliseconds per launch for 256, 512, 1024, then LDS-amplitude 256, 512, 1024 (-1: not
- * run). */
+ * on this operator and basis (three launches each, on scratch vectors) the default matvec
+ * kernel with 256-, 512- and 1024-row workgroups, the variant that also stages the (distinct)
+ * amplitudes in LDS at the same three sizes when they fit, and the split-row variant with 2,
+ * 4, 8 and 16 lanes per row (512-lane workgroups), and uses the fastest for every later
+ * edm_matvec / edm_eigsolve on the pair. The per-row arithmetic and its order are the same in
+ * all ten, so y does not change. *kernel = chosen code: rows per workgroup,
+ * | EDM_TUNE_AMP_LDS for the LDS-amplitude variant, | lanes << EDM_TUNE_SPLIT_SHIFT for the
+ * split-row variant (0 when the pair runs another kernel); ms (may be NULL) =
+ * ms[EDM_TUNE_CANDIDATES], the mean milliseconds per launch of 256, 512, 1024, LDS-amplitude
+ * 256, 512, 1024, split 2, 4, 8, 16 (-1: not run). A pair that was not tuned uses one lane
+ * per row when D >= 2^18 and otherwise the smallest power of two of lanes (at most 4) that
+ * brings D * lanes to 2^18. */
 #define EDM_TUNE_AMP_LDS 0x10000
+#define EDM_TUNE_SPLIT_SHIFT 20
+#define EDM_TUNE_CANDIDATES 10
 int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel,
-                    double *ms6);
+                    double *ms);
 
 /* ----------------------------------------------------------------- eigensolve (a9-a10) */
 /* krylov_map_solve(H::AbstractGPULinearMap, N; ishermitian=true, vec0, krylovdim, tol,

@@ -657,15 +657,15 @@ int edm_csr_destroy(edm_csr *m)
 }
 
 int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel,
-                    double *ms6)
+                    double *ms)
 {
     return guarded(ctx, [&]() -> int {
         EDM_REQUIRE(ctx && op && b && kernel, EDM_EINVAL, "NULL argument");
         EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
-        double t[6];
+        double t[EDM_TUNE_CANDIDATES];
         *kernel = matvec_tune(ctx, op, b, t);
-        if (ms6)
-            for (int k = 0; k < 6; ++k) ms6[k] = t[k];
+        if (ms)
+            for (int k = 0; k < EDM_TUNE_CANDIDATES; ++k) ms[k] = t[k];
         return EDM_OK;
     });
 }

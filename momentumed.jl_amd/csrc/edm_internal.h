@@ -145,7 +145,8 @@ struct edm_op {
     // cached op_conserves() results keyed by basis id
     mutable std::vector<std::pair<uint64_t, bool>> conserve_cache;
     // matvec kernel per basis id chosen by matvec_tune: rows per workgroup (default 512),
-    // | EDM_TUNE_AMP_LDS when the amplitudes are staged in LDS
+    // | EDM_TUNE_AMP_LDS when the amplitudes are staged in LDS, | lanes per row <<
+    // EDM_TUNE_SPLIT_SHIFT for the split-row kernel
     mutable std::vector<std::pair<uint64_t, int>> tuned_block;
     int nbits = 0;
     bool upper = false;
@@ -250,7 +251,8 @@ void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const i
 // amplitudes staged in LDS when the image has that tail, on this (op, basis); keep the
 // fastest for later launches. ms6 gets the six timings (-1: not run); returns the chosen
 // rows per workgroup (| EDM_TUNE_AMP_LDS for the LDS-amplitude kernel) or 0
-int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms6);
+// ms[EDM_TUNE_CANDIDATES]: milliseconds per launch of each candidate (-1: not run)
+int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms);
 // eigenvector consumers (edm_matvec.hip)
 void launch_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
                            const edm_basis *ket, const double2 *x, double2 *y);

@@ -556,6 +556,110 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
     return make_double2(yr, yi);
 }
 
+// One row shared by G lanes (G divides 64; the lanes are consecutive in one wavefront). The
+// row's keys are walked as by row_one; within a bucket lane g resolves entries e0 + g,
+// e0 + g + G, ... (test, sign, partner lookup, amplitude and x loads) and parks its signed
+// amplitude and x value in its LDS slot (zeros when the entry does not apply); the row's
+// first lane then adds the G slots in entry order with the same cmac as row_one. Adding a
+// zero contribution leaves the sum's bits unchanged (it starts at +0 and a sum that starts at
+// +0 never becomes -0), so y is bitwise the row_one result for every G. slot: this lane's
+// 2 x double2 in LDS, row0: the row's first lane's.
+template <class Rank, int G>
+__device__ __forceinline__ double2 row_split(int64_t i, uint64_t s, const unsigned char *base,
+                                             const CompactDev &t, const Rank &rank,
+                                             const double2 *__restrict__ x, int g,
+                                             double2 *slot, const double2 *row0)
+{
+    const uint16_t *s_keyptr = (const uint16_t *)base;
+    const double2 *s_diag = (const double2 *)(base + t.off_diag);
+    const uint16_t *s_code = (const uint16_t *)(base + t.off_code);
+    double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
+    const auto rrow = rank_row32(rank, s);
+
+    auto park = [&](int e, bool ok, uint64_t target, uint32_t neg) {
+        double2 a = make_double2(0.0, 0.0), xv = make_double2(0.0, 0.0);
+        if (ok) {
+            const int64_t j = rank_find(rrow, target);
+            if (!(RankMayMiss<Rank>::value && j < 0)) {
+                const double2 am = t.amp[e];
+                a = make_double2(flip_sign(am.x, neg), flip_sign(am.y, neg));
+                xv = x[j];
+            }
+        }
+        slot[0] = a;
+        slot[1] = xv;
+        __builtin_amdgcn_wave_barrier();
+        if (g == 0) {
+#pragma unroll
+            for (int q = 0; q < G; ++q) cmac(yr, yi, row0[2 * q].x, row0[2 * q].y, row0[2 * q + 1]);
+        }
+        __builtin_amdgcn_wave_barrier();
+    };
+    auto enter = [&](int key, int &e0, int &e1) {
+        const double2 dg = s_diag[key];
+        dr += dg.x;
+        di += dg.y;
+        e0 = s_keyptr[key];
+        e1 = s_keyptr[key + 1];
+    };
+    uint64_t rem = s;
+    while (rem) {
+        const int b = 63 - __builtin_clzll(rem);
+        rem ^= 1ULL << b;
+        const uint64_t ob = 1ULL << b;
+        if (b < t.nkeys) {   // one-body entries: in = {c}
+            int e0, e1;
+            enter(b, e0, e1);
+            const uint64_t mid = s & ~ob;
+            const uint32_t pout = __popcll(mid & pm_one(b)) & 1u;
+            for (int c0 = e0; c0 < e1; c0 += G) {
+                const int e = c0 + g;
+                bool ok = false;
+                uint64_t target = 0;
+                uint32_t neg = 0;
+                if (e < e1) {
+                    const int c = s_code[e] & 0xFF;
+                    const uint64_t inm = 1ULL << c;
+                    ok = !(mid & inm);
+                    target = mid | inm;
+                    neg = (pout + (uint32_t)__popcll(mid & pm_one(c))) & 1u;
+                }
+                park(e, ok, target, neg);
+            }
+        }
+        uint64_t low = s & (ob - 1ULL);
+        const int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            const int a = 63 - __builtin_clzll(low);
+            low ^= 1ULL << a;
+            const int key = kb + a;
+            if (key >= t.nkeys) continue;
+            int e0, e1;
+            enter(key, e0, e1);
+            const uint64_t mid = s & ~(ob | (1ULL << a));
+            const uint32_t pout = __popcll(mid & pm_pair(a, b)) & 1u;
+            for (int c0 = e0; c0 < e1; c0 += G) {   // two-body entries: in = {c, d}, c < d
+                const int e = c0 + g;
+                bool ok = false;
+                uint64_t target = 0;
+                uint32_t neg = 0;
+                if (e < e1) {
+                    const unsigned code = s_code[e];
+                    const int c = code & 0xFF, d = code >> 8;
+                    const uint64_t bd = 1ULL << d;
+                    const uint64_t inm = bd | (1ULL << c);
+                    ok = !(mid & inm);
+                    target = mid | inm;
+                    neg = (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u;
+                }
+                park(e, ok, target, neg);
+            }
+        }
+    }
+    cmac(yr, yi, dr, di, x[i]);
+    return make_double2(yr, yi);
+}
+
 // One row of the LDS-staged matvec: y_i (with the diagonal when with_diag) over partners in
 // [jmin, jmax) (jmin = 0, jmax = INT64_MAX: no window). The term image must already be in
 // LDS at `base`.
@@ -686,6 +790,30 @@ k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
         v.y += o.y;
     }
     y[i] = v;
+}
+
+// The default matvec with G lanes per row (row_split): kLdsBlock / G rows per workgroup, the
+// term image in LDS followed by one 32-byte exchange slot per lane. y is bitwise the one-lane
+// kernel's; more lanes per row shorten each thread's walk when there are too few rows to fill
+// the device.
+template <class Rank, int G>
+__global__ void __launch_bounds__(kLdsBlock)
+k_matvec_split(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
+               const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    static_assert(G >= 2 && G <= 64 && (64 % G) == 0, "lanes per row must divide 64");
+    extern __shared__ uint4 lds_u4[];
+    stage_image(t, lds_u4);
+    double2 *slots = (double2 *)((unsigned char *)lds_u4 + ((t.lds_bytes + 15) & ~15));
+    __syncthreads();
+    const int64_t i = ((int64_t)blockIdx.x * kLdsBlock + threadIdx.x) / G;
+    if (i >= D) return;   // all G lanes of a row leave together; no barrier follows
+    const int g = (int)(threadIdx.x % G);
+    double2 *slot = slots + 2 * threadIdx.x;
+    const double2 *row0 = slots + 2 * (threadIdx.x - g);
+    const double2 v = row_split<Rank, G>(i, states[i], (const unsigned char *)lds_u4, t, rank, x,
+                                         g, slot, row0);
+    if (g == 0) y[i] = v;
 }
 
 // k_matvec_ldsu<Rank, true, 1, BLOCK> with the amplitudes staged in LDS as well (the whole
@@ -1200,6 +1328,16 @@ static int matvec_variant()
     return 11;
 }
 
+// EDM_MATVEC_LANES=1|2|4|8|16 fixes the lanes per row of the default kernel (tests compare
+// them; y is the same for all), over the tuner's pick and the size-based default.
+static int forced_lanes()
+{
+    const char *v = getenv("EDM_MATVEC_LANES");
+    if (!v) return 0;
+    const int n = atoi(v);
+    return (n == 1 || n == 2 || n == 4 || n == 8 || n == 16) ? n : 0;
+}
+
 static int xcd_swizzle_enabled()
 {
     const char *v = getenv("EDM_XCD_SWIZZLE");
@@ -1270,11 +1408,61 @@ static void launch_ldsa(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct
     else launch_ldsa_block<Rank, 1024>(ctx, b, ct, rank, x, y);
 }
 
+template <class Rank, int G>
+static void launch_split_g(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                           const double2 *x, double2 *y)
+{
+    const size_t lds = (((size_t)ct.lds_bytes + 15) & ~(size_t)15) +
+                       (size_t)kLdsBlock * 2 * sizeof(double2);
+    if (lds > 64 * 1024) {   // per-kernel opt-in beyond 64 KiB, once per device
+        static std::atomic<uint64_t> attr_set{0};
+        const uint64_t bit = 1ULL << (ctx->device & 63);
+        if (!(attr_set.load() & bit)) {
+            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_split<Rank, G>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kLdsMaxBytes));
+            attr_set.fetch_or(bit);
+        }
+    }
+    const int64_t nb = (b->D * G + kLdsBlock - 1) / kLdsBlock;
+    hipLaunchKernelGGL((k_matvec_split<Rank, G>), dim3((unsigned)nb), dim3(kLdsBlock), lds,
+                       ctx->stream, b->D, b->d_states, compact_dev(ct), rank, x, y);
+}
+
+template <class Rank>
+static void launch_split(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
+                         const double2 *x, double2 *y, int lanes)
+{
+    switch (lanes) {
+    case 2: launch_split_g<Rank, 2>(ctx, b, ct, rank, x, y); break;
+    case 4: launch_split_g<Rank, 4>(ctx, b, ct, rank, x, y); break;
+    case 8: launch_split_g<Rank, 8>(ctx, b, ct, rank, x, y); break;
+    default: launch_split_g<Rank, 16>(ctx, b, ct, rank, x, y); break;
+    }
+}
+
+// Lanes per row of the default kernel when the pair was not tuned: a row per lane once there
+// are 2^18 rows (half of what the device holds at once), otherwise the fewest lanes (a power
+// of two, at most 4) that bring the threads to 2^18. Measured: 174 k rows run fastest with
+// 2 lanes (4 and more are slower), rows of 1 k - 20 k with 4 (8 and 16 within 5 %). The
+// choice only changes speed: y is bitwise the same.
+static int default_kernel_code(int64_t D)
+{
+    int lanes = 1;
+    while (lanes < 4 && D * lanes < ((int64_t)1 << 18)) lanes *= 2;
+    return lanes == 1 ? kLdsBlock : (kLdsBlock | (lanes << EDM_TUNE_SPLIT_SHIFT));
+}
+
 template <class Rank>
 static void launch_compact(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
                            const double2 *x, double2 *y, int variant, int block = kLdsBlock)
 {
-    // block: rows per workgroup, | EDM_TUNE_AMP_LDS for the kernel with LDS amplitudes
+    // block: rows per workgroup, | EDM_TUNE_AMP_LDS for the kernel with LDS amplitudes,
+    // | lanes << EDM_TUNE_SPLIT_SHIFT for the split-row kernel
+    if (variant == 11 && (block >> EDM_TUNE_SPLIT_SHIFT) > 1) {
+        launch_split(ctx, b, ct, rank, x, y, (block >> EDM_TUNE_SPLIT_SHIFT) & 0xFF);
+        return;
+    }
     if (variant == 13 && ct.lds_full > 0) {
         launch_ldsa(ctx, b, ct, rank, x, y, 1024);
         return;
@@ -1310,10 +1498,11 @@ void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const dou
         const CompactTable &ct = (mode == ApplyMode::Adjoint && !op->upper) ? op->cadj : op->cfwd;
         int variant = matvec_variant();
         if (ct.ok && variant != 0) {
-            int block = kLdsBlock;
-            if (mode == ApplyMode::Forward)
-                for (auto &kv : op->tuned_block)
-                    if (kv.first == b->id) block = kv.second;
+            int block = default_kernel_code(b->D);
+            for (auto &kv : op->tuned_block)
+                if (mode == ApplyMode::Forward && kv.first == b->id) block = kv.second;
+            if (const int lanes = forced_lanes())
+                block = kLdsBlock | (lanes > 1 ? lanes << EDM_TUNE_SPLIT_SHIFT : 0);
             if (use_lin(op, b))
                 launch_compact(ctx, b, ct, make_lin(b), x, y, variant, block);
             else
@@ -1472,9 +1661,9 @@ void launch_csr_keys(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const i
     EDM_HIP_CHECK(hipGetLastError());
 }
 
-int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms6)
+int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms)
 {
-    for (int k = 0; k < 6; ++k) ms6[k] = -1.0;
+    for (int k = 0; k < EDM_TUNE_CANDIDATES; ++k) ms[k] = -1.0;
     if (!op->fast || !op->cfwd.ok || matvec_variant() != 11 || matvec_passes() != 1 || b->D == 0)
         return 0;
     size_t vb = (size_t)b->D * sizeof(double2);
@@ -1489,10 +1678,11 @@ int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms6)
         EDM_HIP_CHECK(hipEventCreate(&e1));
         const int blocks[3] = {256, 512, 1024};
         double bestms = 1e300;
-        for (int k = 0; k < 6; ++k) {
-            const bool amp_lds = k >= 3;
+        for (int k = 0; k < EDM_TUNE_CANDIDATES; ++k) {
+            const bool amp_lds = k >= 3 && k < 6;
             if (amp_lds && op->cfwd.lds_full == 0) continue;
-            const int code = blocks[k % 3] | (amp_lds ? EDM_TUNE_AMP_LDS : 0);
+            const int code = k < 6 ? (blocks[k % 3] | (amp_lds ? EDM_TUNE_AMP_LDS : 0))
+                                   : (kLdsBlock | ((2 << (k - 6)) << EDM_TUNE_SPLIT_SHIFT));
             auto run = [&]() {
                 if (use_lin(op, b))
                     launch_compact(ctx, b, op->cfwd, make_lin(b), x, y, 11, code);
@@ -1504,11 +1694,11 @@ int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms6)
             for (int r = 0; r < 3; ++r) run();
             EDM_HIP_CHECK(hipEventRecord(e1, ctx->stream));
             EDM_HIP_CHECK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            EDM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            ms6[k] = ms / 3.0;
-            if (ms6[k] < bestms) {
-                bestms = ms6[k];
+            float t_ms = 0.f;
+            EDM_HIP_CHECK(hipEventElapsedTime(&t_ms, e0, e1));
+            ms[k] = t_ms / 3.0;
+            if (ms[k] < bestms) {
+                bestms = ms[k];
                 best = code;
             }
         }

@@ -242,9 +242,9 @@ def test_matvec_kernel_variants(gpu_ctx, oracle_mod, monkeypatch, variant, swizz
 
 
 def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
-    """The autotuner only changes the workgroup size and where the amplitudes are read from:
-    y is bitwise the same before and after, for whichever kernel it picks, and the
-    LDS-amplitude kernel gives the same y as the global-table one."""
+    """The autotuner only changes the workgroup size, where the amplitudes are read from and
+    how many lanes share a row: y is bitwise the same before and after, for whichever kernel
+    it picks, and the LDS-amplitude kernel gives the same y as the global-table one."""
     w = W.medium()
     op = w.operator()
     dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, w.para.bits)
@@ -255,7 +255,7 @@ def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
     blk, ms = H.tune()
     print(f"[tune] medium: chose {blk:#x}, ms {ms}", flush=True)
     assert blk & 0xFFFF in (256, 512, 1024) and all(v > 0 for v in ms.values())
-    assert len(ms) == 6     # the medium operator's amplitudes fit in LDS
+    assert len(ms) == 10    # the medium operator's amplitudes fit in LDS
     np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b, x), y0)
     import os
     os.environ["EDM_MATVEC"] = "u1a"
@@ -270,6 +270,42 @@ def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
     blk2, _ = Hp.tune()
     assert blk2 & 0xFFFF in (256, 512, 1024)
     np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, part, xp), yp0)
+
+
+def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
+    """The split-row kernel (2, 4, 8, 16 lanes per row, the row's first lane adding the
+    parked contributions in entry order) gives bitwise the one-lane y: a complete sector with
+    rank tables, a partial basis (binary search, partners missing), a system with one-body
+    terms, and two conserved components; the one-lane y matches the oracle."""
+    cases = []
+    w = W.medium()
+    dop = dev.DeviceOperator.from_mboperator(gpu_ctx, w.operator(), w.para.bits)
+    b = dev.DeviceBasis.enumerate(gpu_ctx, _system(w.para, w.N_each), 0)
+    cases.append(("medium", w.operator(), dop, b))
+    part = dev.DeviceBasis.upload(gpu_ctx, b.download()[::7].copy(), w.para.bits)
+    cases.append(("medium/7", w.operator(), dop, part))
+    ws = W.small()
+    dops = dev.DeviceOperator.from_mboperator(gpu_ctx, ws.operator(), ws.para.bits)
+    for sec in (0, 7):
+        cases.append((f"small s{sec}", ws.operator(), dops,
+                      dev.DeviceBasis.enumerate(gpu_ctx, _system(ws.para, ws.N_each), sec)))
+    para = W.batched_para(1.3)
+    op2 = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    dop2 = dev.DeviceOperator.from_mboperator(gpu_ctx, op2, para.bits)
+    cases.append(("two-component", op2, dop2,
+                  dev.DeviceBasis.enumerate(gpu_ctx, _system(para, (2, 3)), 3)))
+    for name, op, d, basis in cases:
+        x = _rand(basis.D, 91)
+        monkeypatch.setenv("EDM_MATVEC_LANES", "1")
+        y1 = _gpu_matvec(gpu_ctx, d, basis, x)
+        if basis.D <= 20000:
+            yo = oracle_mod.matvec(basis.download(), _oracle_full(oracle_mod, op), x)
+            assert np.abs(y1 - yo).max() <= Y_TOL * np.abs(yo).max(), name
+        for lanes in ("2", "4", "8", "16"):
+            monkeypatch.setenv("EDM_MATVEC_LANES", lanes)
+            np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, d, basis, x), y1,
+                                          err_msg=f"{name} lanes={lanes}")
+        print(f"[split] {name}: D={basis.D} bitwise equal for 2..16 lanes", flush=True)
 
 
 @pytest.mark.parametrize("passes", ["2", "3"])
