@@ -294,6 +294,16 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
     dop2 = dev.DeviceOperator.from_mboperator(gpu_ctx, op2, para.bits)
     cases.append(("two-component", op2, dop2,
                   dev.DeviceBasis.enumerate(gpu_ctx, _system(para, (2, 3)), 3)))
+    # 4x6 mesh per component: a 57 KB term image, so image + exchange slots pass 64 KiB and
+    # the kernel runs with the raised LDS limit
+    k46 = W.mesh(4, 6)
+    para46 = me.EDPara(k_list=k46, Gk=(4, 6), Nc_conserve=2,
+                       H_two=W.CoulombGaussian(k46, (4, 6), c=0.25, inter_ratio=0.8),
+                       two_momentum_coordinate=False)
+    op46 = me.MBOperator(me.ED_scatterlist_twobody(para46), upper_hermitian=True)
+    dop46 = dev.DeviceOperator.from_mboperator(gpu_ctx, op46, para46.bits)
+    cases.append(("4x6 two-component", op46, dop46,
+                  dev.DeviceBasis.enumerate(gpu_ctx, _system(para46, (2, 1)), 5)))
     for name, op, d, basis in cases:
         x = _rand(basis.D, 91)
         monkeypatch.setenv("EDM_MATVEC_LANES", "1")
