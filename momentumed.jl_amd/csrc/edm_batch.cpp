@@ -16,8 +16,9 @@
 // per-problem arithmetic and reduction order equal the single-problem kernels'. A problem's
 // eigenvalues and eigenvectors are therefore bitwise the ones edm_eigsolve returns for it
 // alone, whatever else shares the batch. Problems may be added and collected between steps
-// (continuous batching); each add allocates the problem's workspace from the context, so
-// the context's HBM budget is the admission limit.
+// (continuous batching); each add takes the problem's workspace from the context (with an
+// HBM budget, one allocation per problem, so the budget is the admission limit; without,
+// a range of a shared 1 GiB chunk).
 #include "edm_internal.h"
 
 #include <algorithm>
@@ -83,6 +84,80 @@ struct DevArena {
         if (p) ctx->free(p, bytes);
         p = nullptr;
         bytes = 0;
+    }
+};
+
+// Problem workspaces. A context with an HBM budget gives each problem its own allocation, so
+// the budget admits exactly what fits. Without a budget, workspaces are carved first-fit
+// from 1 GiB chunks and returned to the chunk's free list when the problem is collected; a
+// chunk goes back to the device allocator once nothing in it is in use. Collecting thousands
+// of small problems then costs no driver call each (a device free of a workspace of a few MB
+// takes ~0.3 ms). All work on a workspace is queued on the context's one stream, so a range
+// handed to a new problem is only written after the work of its previous owner.
+constexpr size_t kSlabChunk = (size_t)1 << 30;
+
+struct SlabArena {
+    struct Chunk {
+        unsigned char *base;
+        size_t size, in_use;
+        std::map<size_t, size_t> free;   // offset -> length, non-adjacent ranges
+    };
+    std::vector<Chunk> chunks;
+
+    void *alloc(edm_ctx *ctx, size_t bytes)
+    {
+        bytes = align_up(bytes);
+        for (Chunk &c : chunks)
+            for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+                if (it->second < bytes) continue;
+                const size_t off = it->first, len = it->second;
+                c.free.erase(it);
+                if (len > bytes) c.free[off + bytes] = len - bytes;
+                c.in_use += bytes;
+                return c.base + off;
+            }
+        const size_t sz = ctx->budget ? bytes : std::max(bytes, kSlabChunk);
+        Chunk c{(unsigned char *)ctx->alloc(sz), sz, bytes, {}};
+        if (sz > bytes) c.free[bytes] = sz - bytes;
+        chunks.push_back(std::move(c));
+        return chunks.back().base;
+    }
+
+    void free(edm_ctx *ctx, void *p, size_t bytes)
+    {
+        bytes = align_up(bytes);
+        unsigned char *q = (unsigned char *)p;
+        for (size_t k = 0; k < chunks.size(); ++k) {
+            Chunk &c = chunks[k];
+            if (q < c.base || q >= c.base + c.size) continue;
+            c.in_use -= bytes;
+            if (c.in_use == 0) {
+                ctx->free(c.base, c.size);
+                chunks.erase(chunks.begin() + (ptrdiff_t)k);
+                return;
+            }
+            size_t off = (size_t)(q - c.base), len = bytes;
+            auto next = c.free.lower_bound(off);
+            if (next != c.free.end() && next->first == off + len) {
+                len += next->second;
+                next = c.free.erase(next);
+            }
+            if (next != c.free.begin()) {
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    prev->second += len;
+                    return;
+                }
+            }
+            c.free[off] = len;
+            return;
+        }
+    }
+
+    void release_all(edm_ctx *ctx)
+    {
+        for (Chunk &c : chunks) ctx->free(c.base, c.size);
+        chunks.clear();
     }
 };
 
@@ -233,6 +308,7 @@ struct edm_batch {
     std::map<int64_t, edm_problem *> probs;   // ordered by id: launch order is admission order
     PinnedBuf h_plan, h_coef, h_rot;
     DevArena d_plan, d_coef, d_rot;
+    SlabArena slabs;
     HostPool pool;
     int64_t steps = 0;
     int64_t quantum = 0;   // 0: every running problem takes part in every step
@@ -258,9 +334,9 @@ static int krylov_dim(int64_t D, int nev, int krylovdim)
     return m;
 }
 
-static void release_problem(edm_ctx *ctx, edm_problem *p)
+static void release_problem(edm_batch *bt, edm_problem *p)
 {
-    if (p->slab) ctx->free(p->slab, p->slab_bytes);
+    if (p->slab) bt->slabs.free(bt->ctx, p->slab, p->slab_bytes);
     p->slab = nullptr;
     delete p;
 }
@@ -333,7 +409,7 @@ int batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int nev, int 
     p->want_vectors = want_vectors != 0;
     try {
         p->slab_bytes = problem_slab_bytes(D, m);
-        p->slab = ctx->alloc(p->slab_bytes);
+        p->slab = bt->slabs.alloc(ctx, p->slab_bytes);
         unsigned char *c = (unsigned char *)p->slab;
         size_t vb = (size_t)p->Dpad * sizeof(double2);
         for (int i = 0; i <= m; ++i, c += vb) p->V.push_back((double2 *)c);
@@ -350,14 +426,18 @@ int batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int nev, int 
         c += align_up((size_t)m * m * sizeof(double));
         p->d_vptrs = (double2 **)c;
 
-        // start vector, as run_eigsolve
+        // start vector, as run_eigsolve; a caller's vec0 is checked on the host, the seeded
+        // uniform [0,1)+i[0,1) one cannot have a zero or non-finite norm and is normalised
+        // on the device without a round trip
         hipStream_t st = ctx->stream;
-        if (vec0)
+        if (vec0) {
             EDM_HIP_CHECK(hipMemcpyAsync(p->V[0], vec0, D * sizeof(double2), hipMemcpyHostToDevice, st));
-        else
+            double b0 = single_norm(ctx, p, p->V[0]);
+            EDM_REQUIRE(b0 > 0 && std::isfinite(b0), EDM_EINVAL, "start vector is zero or not finite");
+        } else {
             blas_fill_random(ctx, p->V[0], D, seed, 0);
-        double b0 = single_norm(ctx, p, p->V[0]);
-        EDM_REQUIRE(b0 > 0 && std::isfinite(b0), EDM_EINVAL, "start vector is zero or not finite");
+            blas_norm(ctx, p->V[0], D, p->d_beta);
+        }
         blas_scale_inv(ctx, p->V[0], p->V[0], p->d_beta, D);
         p->T.assign((size_t)m * m, 0.0);
         p->res.assign(m, 0.0);
@@ -365,7 +445,7 @@ int batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int nev, int 
         p->j = 0;
         p->iters = 1;
     } catch (...) {
-        release_problem(ctx, p);
+        release_problem(bt, p);
         throw;
     }
     bt->probs[p->id] = p;
@@ -696,8 +776,9 @@ void batch_destroy(edm_batch *bt)
 {
     edm_ctx *ctx = bt->ctx;
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto &kv : bt->probs) release_problem(ctx, kv.second);
+    for (auto &kv : bt->probs) release_problem(bt, kv.second);
     bt->probs.clear();
+    bt->slabs.release_all(ctx);
     bt->d_plan.release(ctx);
     bt->d_coef.release(ctx);
     bt->d_rot.release(ctx);
@@ -772,7 +853,7 @@ int batch_result(edm_batch *bt, int64_t id, double *vals, double *vecs, edm_solv
     }
     int rc = p->state == kDone ? EDM_OK : EDM_ENOCONV;
     bt->probs.erase(id);
-    release_problem(ctx, p);
+    release_problem(bt, p);
     return rc;
 }
 
@@ -782,7 +863,7 @@ int batch_remove(edm_batch *bt, int64_t id)
     edm_problem *p = find_problem(bt, id);
     EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     bt->probs.erase(id);
-    release_problem(ctx, p);
+    release_problem(bt, p);
     return EDM_OK;
 }
 
