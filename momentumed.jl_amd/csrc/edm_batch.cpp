@@ -164,6 +164,7 @@ struct SlabArena {
 // Host builder for one upload: sections appended at 256-byte alignment.
 struct Plan {
     std::vector<unsigned char> h;
+    void clear() { h.clear(); }
     size_t put(const void *src, size_t n)
     {
         size_t off = align_up(h.size());
@@ -310,6 +311,13 @@ struct edm_batch {
     DevArena d_plan, d_coef, d_rot;
     SlabArena slabs;
     HostPool pool;
+    // host scratch of the per-step plan, kept between steps so that building a plan does
+    // not allocate
+    Plan plan;
+    std::vector<unsigned char> mvdesc;
+    std::vector<int> mvblk, tile2prob, blk2prob;
+    std::vector<size_t> coef_off, vtab_off;
+    std::vector<BatchVecDesc> vd;
     int64_t steps = 0;
     int64_t quantum = 0;   // 0: every running problem takes part in every step
 };
@@ -626,7 +634,8 @@ static int batch_step_once(edm_batch *bt)
     const int n = (int)act.size();
 
     // coefficient arena: per problem h[m+1], h2[m+1], beta (one double2 slot)
-    std::vector<size_t> coef_off(n);
+    std::vector<size_t> &coef_off = bt->coef_off;
+    coef_off.resize(n);
     size_t coef_bytes = 0;
     for (int q = 0; q < n; ++q) {
         coef_off[q] = coef_bytes;
@@ -637,33 +646,42 @@ static int batch_step_once(edm_batch *bt)
 
     // plan: matvec descriptors + 512-row block table, V tables, vector descriptors, tile and
     // 256-row block tables
-    Plan pl;
+    Plan &pl = bt->plan;
+    pl.clear();
     const size_t mvsz = batch_mv_desc_size();
-    std::vector<unsigned char> mvdesc;
-    std::vector<int> mvblk;
+    std::vector<unsigned char> &mvdesc = bt->mvdesc;
+    std::vector<int> &mvblk = bt->mvblk;
+    mvdesc.clear();
+    mvblk.clear();
     std::vector<edm_problem *> single_mv;
     size_t lds = 16;
     for (int q = 0; q < n; ++q) {
         edm_problem *p = act[q];
         size_t l = 0;
-        std::vector<unsigned char> d(mvsz);
+        const size_t at = mvdesc.size();
+        mvdesc.resize(at + mvsz);
         int64_t blk0 = (int64_t)mvblk.size();
-        if (batch_mv_desc(p->op, p->b, p->V[p->j], p->V[p->j + 1], blk0, d.data(), &l)) {
-            int pid = (int)(mvdesc.size() / mvsz);
-            mvdesc.insert(mvdesc.end(), d.begin(), d.end());
+        if (batch_mv_desc(p->op, p->b, p->V[p->j], p->V[p->j + 1], blk0, mvdesc.data() + at,
+                          &l)) {
+            int pid = (int)(at / mvsz);
             mvblk.insert(mvblk.end(), (size_t)batch_mv_blocks(p->D), pid);
             lds = std::max(lds, l);
         } else {
+            mvdesc.resize(at);
             single_mv.push_back(p);
         }
     }
     size_t mv_off = pl.put(mvdesc.data(), mvdesc.size());
     size_t mvblk_off = pl.put(mvblk.data(), mvblk.size() * sizeof(int));
-    std::vector<size_t> vtab_off(n);
+    std::vector<size_t> &vtab_off = bt->vtab_off;
+    vtab_off.resize(n);
     for (int q = 0; q < n; ++q)
         vtab_off[q] = pl.put(act[q]->V.data(), (size_t)(act[q]->j + 1) * sizeof(double2 *));
-    std::vector<BatchVecDesc> vd(n);
-    std::vector<int> tile2prob, blk2prob;
+    std::vector<BatchVecDesc> &vd = bt->vd;
+    vd.resize(n);
+    std::vector<int> &tile2prob = bt->tile2prob, &blk2prob = bt->blk2prob;
+    tile2prob.clear();
+    blk2prob.clear();
     int maxvec = 0;
     unsigned char *dplan_base = nullptr;   // filled after the arena is sized
     unsigned char *dcoef = nullptr;
