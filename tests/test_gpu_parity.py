@@ -318,6 +318,38 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
         print(f"[split] {name}: D={basis.D} bitwise equal for 2..16 lanes", flush=True)
 
 
+@pytest.mark.parametrize("layout", ["mixed_conserved", "three_conserved"])
+def test_component_layouts(gpu_ctx, oracle_mod, layout):
+    """Mixed components inside two conserved ones (with one-body tunnelling), and three
+    conserved components: every sector's basis bit-exact against the oracle, Hx within 1e-13
+    and the lowest eigenvalues within 1e-10 of the oracle's dense matrix, for every sector
+    (the oracle's Hamiltonian for both layouts is pinned against Jordan-Wigner)."""
+    import test_oracle as T
+    if layout == "mixed_conserved":
+        para, N, ops = T._para_mixed_conserved(), (2, 2), True
+    else:
+        para, N, ops = T._para_three_conserved(), (1, 2, 1), False
+    lists = [me.ED_scatterlist_twobody(para)]
+    if ops:
+        lists.insert(0, me.ED_scatterlist_onebody(para))
+    op = me.MBOperator(*lists, upper_hermitian=True)
+    subs, ks = me.ED_momentum_subspaces(para, N)
+    o_lists, o_ks = oracle_mod.enumerate_sectors(para.k_list, para.Gk, para.Nc_mix,
+                                                 para.Nc_conserve, list(N))
+    assert ks == o_ks and len(subs) == len(o_lists)
+    full = _oracle_full(oracle_mod, op)
+    for sp, ob, k in zip(subs, o_lists, ks):
+        np.testing.assert_array_equal(sp.list, ob)
+        H = oracle_mod.dense_matrix(ob, full)
+        x = _rand(sp.length, 5)
+        dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, para.bits)
+        y = _gpu_matvec(gpu_ctx, dop, dev.device_basis_of(sp, gpu_ctx), x)
+        assert np.abs(y - H @ x).max() <= Y_TOL * np.abs(H @ x).max(), k
+        E, _ = me.EDsolve(sp, op, N=3, method="map", method_info=False)
+        np.testing.assert_allclose(E, np.linalg.eigvalsh(H)[:3], rtol=1e-10, atol=1e-12)
+    print(f"[layout] {layout}: {len(subs)} sectors, D {[sp.length for sp in subs]}", flush=True)
+
+
 @pytest.mark.parametrize("passes", ["2", "3"])
 def test_matvec_windowed_passes(gpu_ctx, oracle_mod, monkeypatch, passes):
     """The matvec split into passes over windows of x (accumulating into y)."""

@@ -175,6 +175,42 @@ def test_jw_mixed_components(oracle_mod):
     assert _check_jw(oracle_mod, para, [3], True) < 1e-13
 
 
+def _tunnel(kc, cf, ci):
+    """Hermitian in (cf, ci): band energy on the diagonal, complex tunnelling between the
+    mixed components of a conserved block."""
+    if cf == ci:
+        return complex(0.5 * np.cos(2 * np.pi * float(kc[1])) + 0.1 * cf)
+    return 0.3 + 0.2j if cf < ci else 0.3 - 0.2j
+
+
+def _para_mixed_conserved():
+    """Two mixed components inside each of two conserved ones (4 components, 16 orbitals)."""
+    k = mesh(1, 4)
+    return EDPara(k_list=k, Gk=(1, 4), Nc_mix=2, Nc_conserve=2, H_one=_tunnel,
+                  H_two=CoulombGaussian(k, (1, 4), c=0.3, perturb=0.05, seed=10,
+                                        inter_ratio=0.7),
+                  two_momentum_coordinate=False)
+
+
+def _para_three_conserved():
+    """Three conserved components on a 1x5 mesh (15 orbitals)."""
+    k = mesh(1, 5)
+    return EDPara(k_list=k, Gk=(1, 5), Nc_conserve=3,
+                  H_two=CoulombGaussian(k, (1, 5), c=0.3, perturb=0.05, seed=11,
+                                        inter_ratio=0.6),
+                  two_momentum_coordinate=False)
+
+
+def test_jw_mixed_inside_conserved_components(oracle_mod):
+    """Mixed components inside conserved ones: the one-body tunnelling stays inside each
+    conserved block, and both blocks keep their particle numbers."""
+    assert _check_jw(oracle_mod, _para_mixed_conserved(), [2, 2], True) < 1e-13
+
+
+def test_jw_three_conserved_components(oracle_mod):
+    assert _check_jw(oracle_mod, _para_three_conserved(), [1, 2, 1], False) < 1e-13
+
+
 def test_jw_catches_a_planted_sign_error(oracle_mod):
     """Plant a mistake in the oracle (drop the fermion sign) and check the JW test sees it."""
     O = oracle_mod
