@@ -318,17 +318,20 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
         print(f"[split] {name}: D={basis.D} bitwise equal for 2..16 lanes", flush=True)
 
 
-@pytest.mark.parametrize("layout", ["mixed_conserved", "three_conserved"])
+@pytest.mark.parametrize("layout", ["mixed_conserved", "three_conserved", "three_dimensional"])
 def test_component_layouts(gpu_ctx, oracle_mod, layout):
-    """Mixed components inside two conserved ones (with one-body tunnelling), and three
-    conserved components: every sector's basis bit-exact against the oracle, Hx within 1e-13
-    and the lowest eigenvalues within 1e-10 of the oracle's dense matrix, for every sector
-    (the oracle's Hamiltonian for both layouts is pinned against Jordan-Wigner)."""
+    """Mixed components inside two conserved ones (with one-body tunnelling), three conserved
+    components, and momenta in three dimensions (2x2x3 mesh): every sector's basis bit-exact
+    against the oracle, Hx within 1e-13 and the lowest eigenvalues within 1e-10 of the
+    oracle's dense matrix, for every sector (the oracle's Hamiltonian for all three is pinned
+    against Jordan-Wigner)."""
     import test_oracle as T
     if layout == "mixed_conserved":
         para, N, ops = T._para_mixed_conserved(), (2, 2), True
-    else:
+    elif layout == "three_conserved":
         para, N, ops = T._para_three_conserved(), (1, 2, 1), False
+    else:
+        para, N, ops = T._para_3d(), (4,), False
     lists = [me.ED_scatterlist_twobody(para)]
     if ops:
         lists.insert(0, me.ED_scatterlist_onebody(para))

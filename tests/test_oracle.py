@@ -201,6 +201,43 @@ def _para_three_conserved():
                   two_momentum_coordinate=False)
 
 
+class _Yukawa3D:
+    """A momentum-conserving two-body interaction on a 3-D mesh, index signature
+    H_two(kf1, kf2, ki2, ki1, cf1, cf2, ci2, ci1): V(q) = 1 / (0.5 + |q|^2) on the shortest
+    q of the transfer kf1 - ki1, with a small complex term that depends on all four momenta
+    (Hermitian, symmetric under exchanging the two particles) so no level is accidentally
+    degenerate."""
+
+    def __init__(self, k, dims):
+        self.k, self.dims = np.asarray(k), dims
+
+    def _q2(self, a, b):
+        s = 0.0
+        for d, n in enumerate(self.dims):
+            dq = int(self.k[d, a - 1] - self.k[d, b - 1]) % n
+            dq = dq - n if dq > n // 2 else dq
+            s += dq * dq
+        return s
+
+    def __call__(self, kf1, kf2, ki2, ki1, cf1=1, cf2=1, ci2=1, ci1=1):
+        v = 1.0 / (0.5 + self._q2(kf1, ki1))
+        w = 0.03 * (np.sin(kf1 + 2 * kf2 - ki1 - 2 * ki2) + np.sin(kf2 + 2 * kf1 - ki2 - 2 * ki1))
+        return complex(v, w)
+
+
+def _para_3d():
+    dims = (2, 2, 3)
+    k = np.array([[i % 2 for i in range(12)], [(i // 2) % 2 for i in range(12)],
+                  [i // 4 for i in range(12)]], dtype=np.int64)
+    return EDPara(k_list=k, Gk=dims, H_two=_Yukawa3D(k, dims), two_momentum_coordinate=False)
+
+
+def test_jw_three_dimensional_momenta(oracle_mod):
+    """Momenta in three dimensions (a 2x2x3 mesh, 4 particles): sectors over all three
+    residues, the interaction's Hamiltonian against Jordan-Wigner."""
+    assert _check_jw(oracle_mod, _para_3d(), [4], False) < 1e-13
+
+
 def test_jw_mixed_inside_conserved_components(oracle_mod):
     """Mixed components inside conserved ones: the one-body tunnelling stays inside each
     conserved block, and both blocks keep their particle numbers."""
