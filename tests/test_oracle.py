@@ -214,8 +214,10 @@ class _Yukawa3D:
     def _q2(self, a, b):
         s = 0.0
         for d, n in enumerate(self.dims):
-            dq = int(self.k[d, a - 1] - self.k[d, b - 1]) % n
-            dq = dq - n if dq > n // 2 else dq
+            dq = int(self.k[d, a - 1] - self.k[d, b - 1])
+            if n:   # periodic direction: the shortest representative
+                dq %= n
+                dq = dq - n if dq > n // 2 else dq
             s += dq * dq
         return s
 
@@ -230,6 +232,20 @@ def _para_3d():
     k = np.array([[i % 2 for i in range(12)], [(i // 2) % 2 for i in range(12)],
                   [i // 4 for i in range(12)]], dtype=np.int64)
     return EDPara(k_list=k, Gk=dims, H_two=_Yukawa3D(k, dims), two_momentum_coordinate=False)
+
+
+def _para_open():
+    """A 3x4 mesh periodic in x (Gk = 3) and open in y (Gk = 0: the y momenta add up without
+    a modulus, as on a cylinder)."""
+    k = np.array([[i % 3 for i in range(12)], [i // 3 for i in range(12)]], dtype=np.int64)
+    return EDPara(k_list=k, Gk=(3, 0), H_two=_Yukawa3D(k, (3, 0)),
+                  two_momentum_coordinate=False)
+
+
+def test_jw_open_momentum_direction(oracle_mod):
+    """One periodic and one open momentum direction: sectors over residues in x and exact
+    sums in y, against Jordan-Wigner."""
+    assert _check_jw(oracle_mod, _para_open(), [4], False) < 1e-13
 
 
 def test_jw_three_dimensional_momenta(oracle_mod):
