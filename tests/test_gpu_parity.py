@@ -319,11 +319,12 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
 
 
 @pytest.mark.parametrize("layout", ["mixed_conserved", "three_conserved", "three_dimensional",
-                                    "open_direction"])
+                                    "open_direction", "sixty_four_bits"])
 def test_component_layouts(gpu_ctx, oracle_mod, layout):
     """Mixed components inside two conserved ones (with one-body tunnelling), three conserved
-    components, momenta in three dimensions (2x2x3 mesh), and a mesh periodic in x and open
-    in y (Gk = (3, 0)): every sector's basis bit-exact
+    components, momenta in three dimensions (2x2x3 mesh), a mesh periodic in x and open in y
+    (Gk = (3, 0)), and 64 orbitals (two mixed components on 4x8; partners found by binary
+    search, bit 63 in use): every sector's basis bit-exact
     against the oracle, Hx within 1e-13 and the lowest eigenvalues within 1e-10 of the
     oracle's dense matrix, for every sector (the oracle's Hamiltonian for all three is pinned
     against Jordan-Wigner)."""
@@ -334,8 +335,10 @@ def test_component_layouts(gpu_ctx, oracle_mod, layout):
         para, N, ops = T._para_three_conserved(), (1, 2, 1), False
     elif layout == "three_dimensional":
         para, N, ops = T._para_3d(), (4,), False
-    else:
+    elif layout == "open_direction":
         para, N, ops = T._para_open(), (4,), False
+    else:
+        para, N, ops = T._para_64(), (2,), True
     lists = [me.ED_scatterlist_twobody(para)]
     if ops:
         lists.insert(0, me.ED_scatterlist_onebody(para))

@@ -35,8 +35,8 @@ def test_parity_mask_closed_form(oracle_mod):
     M = (1 << 64) - 1
     for p in range(64):                  # unpaired top bit p -> bits p+1..63
         assert O.parity_mask(1 << p) == (M ^ ((1 << (p + 1)) - 1) if p < 63 else 0)
-    for a in range(20):
-        for b in range(a + 1, 24):       # pair (a<b) -> bits a+1..b-1
+    for a in range(64):
+        for b in range(a + 1, 64):       # pair (a<b) -> bits a+1..b-1
             assert O.parity_mask((1 << a) | (1 << b)) == ((1 << b) - (1 << (a + 1)))
 
 
@@ -246,6 +246,30 @@ def test_jw_open_momentum_direction(oracle_mod):
     """One periodic and one open momentum direction: sectors over residues in x and exact
     sums in y, against Jordan-Wigner."""
     assert _check_jw(oracle_mod, _para_open(), [4], False) < 1e-13
+
+
+def _tunnel_2d(kc, cf, ci):
+    if cf == ci:
+        return complex(0.5 * np.cos(2 * np.pi * float(kc[0]))
+                       + 0.3 * np.cos(2 * np.pi * float(kc[1])) + 0.1 * cf)
+    return 0.25 + 0.15j if cf < ci else 0.25 - 0.15j
+
+
+def _para_64():
+    """Two mixed components on a 4x8 mesh: 64 orbitals, the last one on bit 63."""
+    k = mesh(4, 8)
+    return EDPara(k_list=k, Gk=(4, 8), Nc_mix=2, H_one=_tunnel_2d,
+                  H_two=CoulombGaussian(k, (4, 8), c=0.3, perturb=0.02, seed=12),
+                  two_momentum_coordinate=False)
+
+
+@pytest.mark.skipif(not os.environ.get("EDM_SLOW_TESTS"),
+                    reason="about 140 s; set EDM_SLOW_TESTS=1 to run")
+def test_jw_sixty_four_orbitals(oracle_mod):
+    """All 64 bits of a state in use (2 particles, 32 sectors): the Hamiltonian with
+    one-body tunnelling against Jordan-Wigner. The bit rules at bit 63 are also pinned by
+    the closed forms of every single bit and pair above."""
+    assert _check_jw(oracle_mod, _para_64(), [2], True) < 1e-13
 
 
 def test_jw_three_dimensional_momenta(oracle_mod):
