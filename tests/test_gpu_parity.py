@@ -318,6 +318,31 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
         print(f"[split] {name}: D={basis.D} bitwise equal for 2..16 lanes", flush=True)
 
 
+@pytest.mark.parametrize("N", [0, 1, 2, 13, 14, 15])
+def test_edge_fillings(gpu_ctx, oracle_mod, N):
+    """Empty, single-particle, nearly full and full fillings of the 15-orbital 3x5 system
+    (the reference's colex cases t = 0 and t = n): sectors and states bit-exact against the
+    oracle (one vacuum sector for N = 0, one full state for N = 15), and each sector's
+    lowest energy through EDsolve equal to the oracle's dense matrix within 1e-10, both with
+    the default method (which switches to dense for these tiny sectors, as in the reference)
+    and with the matrix-free map (Krylov spaces down to a single vector)."""
+    w = W.small()
+    op = w.operator()
+    subs, ks = me.ED_momentum_subspaces(w.para, N)
+    o_lists, o_ks = oracle_mod.enumerate_sectors(w.para.k_list, w.para.Gk, 1, 1, [N])
+    assert ks == o_ks and len(subs) == len(o_lists)
+    full = _oracle_full(oracle_mod, op)
+    for sp, ob in zip(subs, o_lists):
+        np.testing.assert_array_equal(sp.list, ob)
+        e_ref = np.linalg.eigvalsh(oracle_mod.dense_matrix(ob, full))[0]
+        for method in ("sparse", "map"):   # sparse switches to dense below 100 states
+            E, V = me.EDsolve(sp, op, N=1, method=method, method_info=False)
+            assert abs(E[0] - e_ref) <= 1e-10 * max(1.0, abs(e_ref)), (method, sp.length)
+            assert abs(np.linalg.norm(V[0].vec) - 1.0) < 1e-12
+    if N == 0:
+        assert [int(s) for s in subs[0].list] == [0] and abs(E[0]) < 1e-15
+
+
 @pytest.mark.parametrize("layout", ["mixed_conserved", "three_conserved", "three_dimensional",
                                     "open_direction", "sixty_four_bits"])
 def test_component_layouts(gpu_ctx, oracle_mod, layout):
