@@ -27,6 +27,28 @@ def _rand(n, seed):
     return v / np.linalg.norm(v)
 
 
+def test_reference_unit_vectors_on_device(gpu_ctx):
+    """The reference's own unit tests, through the device path: EDCore/test/
+    test_multiplication.jl:39-61 (Scatter(1+1im, 1, 2; upper_hermitian) on |01> gives
+    (1-1im)|10>, on |10> gives (1+1im)|01>) with the device matvec, and :63-78
+    (<10| c_2^+ c_1 |01> = 1) with the device bracket, one- and two-operator forms."""
+    space = me.HilbertSubspace([0b01, 0b10], bits=2)
+    s1 = me.Scatter(1.0 + 1.0j, 1, 2, bits=2, upper_hermitian=True)
+    H = me.create_gpu_linearmap(me.LinearMap(me.MBOperator([s1], upper_hermitian=True),
+                                             space), "rocm")
+    y = H(np.array([1, 0], dtype=np.complex128))
+    assert y[0] == 0 and y[1] == 1 - 1j
+    y = H(np.array([0, 1], dtype=np.complex128))
+    assert y[0] == 1 + 1j and y[1] == 0
+    op = me.MBOperator([me.Scatter(1.0, 2, 1, bits=2)], upper_hermitian=False)
+    bra = me.MBS64Vector(np.array([0, 1], dtype=np.complex128), space)   # <10|
+    ket = me.MBS64Vector(np.array([1, 0], dtype=np.complex128), space)   # |01>
+    assert abs(me.ED_bracket(bra, op, ket) - 1.0) < 1e-15
+    ident = me.MBOperator([me.Scatter(1.0, 1, 1, bits=2), me.Scatter(1.0, 2, 2, bits=2)],
+                          upper_hermitian=False)          # the particle number, 1 here
+    assert abs(me.ED_bracket(bra, ident, op, ket) - 1.0) < 1e-15
+
+
 def test_bracket_same_space(gpu_ctx, oracle_mod):
     w = W.small()
     op = w.operator()
