@@ -49,6 +49,30 @@ def test_reference_unit_vectors_on_device(gpu_ctx):
     assert abs(me.ED_bracket(bra, ident, op, ket) - 1.0) < 1e-15
 
 
+def test_reference_integration_example(gpu_ctx, oracle_mod):
+    """The reference's integration test (MomentumED/test/example1_FQH_Laughlin.jl:93-111):
+    ED_momentum_subspaces, ED_scatterlist_twobody and EDsolve(subspace, scat; N=10) in every
+    one of the 12 sectors of the 3x4 mesh with 4 electrons (the scatter-list form switches to
+    the dense method at these sizes, as in the reference). The test asserts nothing; here every
+    sector's 10 energies equal the oracle's dense spectrum within 1e-10 and the vectors are
+    eigenvectors (residual < 1e-9)."""
+    import test_oracle as T
+    para = T._example1_para()
+    subs, ks = me.ED_momentum_subspaces(para, (4,))
+    scat = me.ED_scatterlist_twobody(para)
+    full = _oracle_full(oracle_mod, me.MBOperator(scat, upper_hermitian=True))
+    assert len(subs) == 12
+    for sp, k in zip(subs, ks):
+        E, V = me.EDsolve(sp, scat, N=10, method_info=False)
+        Hd = oracle_mod.dense_matrix(sp.list, full)
+        ref = np.linalg.eigvalsh(Hd)[:10]
+        np.testing.assert_allclose(E, ref, rtol=1e-10, atol=1e-12, err_msg=str(k))
+        for e, v in zip(E, V):
+            assert np.linalg.norm(Hd @ v.vec - e * v.vec) < 1e-9
+    print(f"[example1] 12 sectors, lowest {min(float(np.min(me.EDsolve(sp, scat, N=1, method_info=False)[0])) for sp in subs):.10f}",
+          flush=True)
+
+
 def test_bracket_same_space(gpu_ctx, oracle_mod):
     w = W.small()
     op = w.operator()

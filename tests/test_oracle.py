@@ -272,6 +272,75 @@ def test_jw_sixty_four_orbitals(oracle_mod):
     assert _check_jw(oracle_mod, _para_64(), [2], True) < 1e-13
 
 
+class _Example1Interaction:
+    """The two-body interaction of the reference's integration test (MomentumED/test/
+    example1_FQH_Laughlin.jl:16-88), restated as its input: a screened Coulomb form factor
+    V(q) = W0 / |ql| tanh(|ql| D) exp(-|ql|^2 / 2) on the triangular lattice (W0 D at q = 0),
+    summed over reciprocal shells |g1|, |g2|, |g1 - g2| <= 2 with the magnetic-translation
+    phase and the parity sign ita. Momenta are fractional coordinates k / Gk
+    (two_momentum_coordinate)."""
+    Gl = np.sqrt(2 * np.pi / np.sqrt(0.75))
+    D_l, W0, angle, nshell = 5.0, 1.0, 2 * np.pi / 3, 2
+
+    def __init__(self, exchange_symmetric: bool = False):
+        self.exchange_symmetric = exchange_symmetric
+
+    def vff(self, q1, q2):
+        ql = np.sqrt(q1 * q1 + q2 * q2 + 2 * np.cos(self.angle) * q1 * q2) * self.Gl
+        if ql == 0.0:
+            return self.W0 * self.D_l
+        return self.W0 / ql * np.tanh(ql * self.D_l) * np.exp(-0.5 * ql * ql)
+
+    @staticmethod
+    def ita(g1, g2):
+        return 1 if g1 % 2 == 0 and g2 % 2 == 0 else -1
+
+    def __call__(self, kf1, kf2, ki2, ki1, cf1=1, cf2=1, ci2=1, ci1=1):
+        if self.exchange_symmetric:
+            return 0.5 * (self._v(kf1, kf2, ki2, ki1) + self._v(kf2, kf1, ki1, ki2))
+        return self._v(kf1, kf2, ki2, ki1)
+
+    def _v(self, kf1, kf2, ki2, ki1):
+        d = [float(ki1[i]) - float(kf1[i]) for i in range(2)]
+        q = [x - round(x) for x in d]                      # nearest, ties to even
+        G1 = [round(float(ki1[i]) - float(kf1[i]) - q[i]) for i in range(2)]
+        G2 = [round(float(kf2[i]) - float(ki2[i]) - q[i]) for i in range(2)]
+
+        def cross(a1, a2, b1, b2):
+            return a1 * b2 - a2 * b1
+        total = 0j
+        n = self.nshell
+        for g1 in range(-n, n + 1):
+            for g2 in range(-n, n + 1):
+                if abs(g1 - g2) > n:
+                    continue
+                qq1, qq2 = q[0] + g1, q[1] + g2
+                ph = 0.5 * cross(float(ki1[0]), float(ki1[1]), float(kf1[0]), float(kf1[1]))
+                ph += 0.5 * cross(float(ki1[0] + kf1[0]), float(ki1[1] + kf1[1]), qq1, qq2)
+                ph += 0.5 * cross(float(ki2[0]), float(ki2[1]), float(kf2[0]), float(kf2[1]))
+                ph += 0.5 * cross(float(ki2[0] + kf2[0]), float(ki2[1] + kf2[1]), -qq1, -qq2)
+                sign = self.ita(g1 + G1[0], g2 + G1[1]) * self.ita(g1 + G2[0], g2 + G2[1])
+                total += sign * np.exp(2j * np.pi * ph) * self.vff(qq1, qq2)
+        return complex(total)
+
+
+def _example1_para(exchange_symmetric: bool = False):
+    """example1_FQH_Laughlin.jl:1-10,93: 3x4 mesh, Ne = 4, the interaction above."""
+    return EDPara(k_list=mesh(3, 4), Gk=(3, 4),
+                  H_two=_Example1Interaction(exchange_symmetric))
+
+
+def test_jw_reference_integration_example(oracle_mod):
+    """The reference's integration-test system (example1_FQH_Laughlin.jl, which asserts
+    nothing itself). Its interaction is Hermitian to rounding but, through the truncated
+    reciprocal-shell sum, not symmetric under exchanging the two particles (4e-7); the term
+    builder antisymmetrises as if it were, a raw Jordan-Wigner operator does not. With the
+    exchange-symmetrised interaction the two agree to rounding in every sector; with the raw
+    one they differ by the symmetry defect only."""
+    assert _check_jw(oracle_mod, _example1_para(True), [4], False) < 1e-13
+    assert _check_jw(oracle_mod, _example1_para(False), [4], False) < 1e-6
+
+
 def test_jw_three_dimensional_momenta(oracle_mod):
     """Momenta in three dimensions (a 2x2x3 mesh, 4 particles): sectors over all three
     residues, the interaction's Hamiltonian against Jordan-Wigner."""
