@@ -13,7 +13,7 @@ from typing import Callable, Sequence
 
 import numpy as np
 
-from .core import HilbertSubspace, MBS64, Scatter, sort_merge_scatlist
+from .core import ColexMBS64, ColexMBS64Mask, HilbertSubspace, MBS64, Scatter, sort_merge_scatlist
 
 __all__ = [
     "EDPara", "momentum_residue", "MBS_totalmomentum", "ED_momentum_subspaces",
@@ -109,6 +109,66 @@ def _momentum_list(para: EDPara, N_each) -> list[tuple]:
     return [tuple(reversed(t)) for t in itertools.product(*reversed(ranges))]
 
 
+def _wanted_momenta(para: EDPara, N_each, momentum_restriction: bool, momentum_list):
+    if momentum_restriction:
+        wanted = []
+        for k in (momentum_list or []):
+            k = momentum_residue(k, para.Gk)
+            if k not in wanted:
+                wanted.append(k)
+        return wanted
+    return _momentum_list(para, N_each)
+
+
+def _host_momentum_subspaces(para: EDPara, N_each, momentum_restriction: bool, momentum_list,
+                             mask, selection_rule):
+    """momentum_decomposition.jl:101-152,339-413 with a mask, holes or a selection rule (the
+    forms the reference's entanglement analysis uses), enumerated on the host: component c
+    (the last one in the highest bits) holds |N_c| particles, or |N_c| holes when N_c < 0,
+    on its orbitals (those of `mask` only, when given: 1-based orbital indices of the whole
+    state); its momentum is the residue of its orbitals' momenta, negated for holes; a state
+    is kept when selection_rule(state) holds, in the sector of its total momentum residue.
+    Every list comes out ascending (outer components vary slowest, each one in colex order)."""
+    width = para.Nk * para.Nc_mix
+    if mask is not None:
+        mask = sorted(int(o) for o in mask)
+    wanted = _wanted_momenta(para, N_each, momentum_restriction, momentum_list)
+    pats, moms = [], []
+    for c, n in enumerate(N_each):
+        if not abs(n) <= width:
+            raise AssertionError("Invalid number of electrons in one component")
+        if mask is None:
+            it = ColexMBS64(width, abs(n))
+        else:
+            local = [o - c * width for o in mask if c * width < o <= (c + 1) * width]
+            it = ColexMBS64Mask(width, abs(n), local)
+        p = np.array([st.n for st in it], dtype=np.uint64)
+        sgn = (n > 0) - (n < 0)
+        m = np.array([[sgn * v for v in MBS_totalmomentum(para, int(x))] for x in p],
+                     dtype=np.int64).reshape(len(p), para.dim)
+        pats.append(p)
+        moms.append(m)
+    # outer products, the last component outermost: ascending states
+    states, ksum = pats[0], moms[0]
+    for c in range(1, len(N_each)):
+        hi = pats[c] << np.uint64(c * width)
+        states = (hi[:, None] | states[None, :]).reshape(-1)
+        ksum = (moms[c][:, None, :] + ksum[None, :, :]).reshape(-1, para.dim)
+    res = np.stack([ksum[:, d] % para.Gk[d] if para.Gk[d] else ksum[:, d]
+                    for d in range(para.dim)], axis=1) if len(states) else ksum
+    keep = np.ones(len(states), dtype=bool)
+    if selection_rule is not None:
+        keep = np.array([bool(selection_rule(MBS64(para.bits, n=int(x)))) for x in states],
+                        dtype=bool)
+    subspaces, ks = [], []
+    for k in wanted:
+        sel = keep & np.all(res == np.asarray(k, dtype=np.int64), axis=1)
+        if sel.any():
+            subspaces.append(HilbertSubspace(states[sel], bits=para.bits))
+            ks.append(k)
+    return subspaces, ks
+
+
 def ED_momentum_subspaces(para: EDPara, N_each_component, *, momentum_restriction: bool = False,
                           momentum_list: list | None = None, device: str = "rocm",
                           mask=None, selection_rule=None):
@@ -116,20 +176,20 @@ def ED_momentum_subspaces(para: EDPara, N_each_component, *, momentum_restrictio
     momentum_list order with empty sectors dropped. Returns (subspaces, subspace_k).
 
     The states are enumerated on the device; each returned HilbertSubspace keeps its device
-    copy and fetches its host list lazily. `mask` / `selection_rule` (used by the reference's
-    entanglement analysis) are not supported by the device enumerator."""
+    copy and fetches its host list lazily. The forms with `mask`, `selection_rule` or holes
+    (negative N), which the reference uses to build entanglement-analysis blocks, are
+    enumerated on the host (as the reference does for every form) and uploaded when a device
+    operation first needs them."""
     from . import device as dev
-    if mask is not None or selection_rule is not None:
-        raise NotImplementedError("mask / selection_rule are analysis features outside the "
-                                  "device enumerator")
     if isinstance(N_each_component, (int, np.integer)):
         N_each_component = (int(N_each_component),)
     N_each = [int(n) for n in N_each_component]
     if len(N_each) != para.Nc_conserve:
         raise AssertionError("The length of number_list must be equal to the number of "
                              f"conserved components {para.Nc_conserve}")
-    if any(n < 0 for n in N_each):
-        raise NotImplementedError("hole (negative N) sectors are an analysis feature")
+    if mask is not None or selection_rule is not None or any(n < 0 for n in N_each):
+        return _host_momentum_subspaces(para, N_each, momentum_restriction, momentum_list,
+                                        mask, selection_rule)
     if device not in ("rocm", "gpu"):
         raise ValueError("the basis is enumerated on the device (device='rocm')")
     dev.throw_gpu_unavailable("rocm")
@@ -137,11 +197,7 @@ def ED_momentum_subspaces(para: EDPara, N_each_component, *, momentum_restrictio
     system = dev.system_struct(para.k_list, para.Gk, para.Nc_mix, para.Nc_conserve, N_each)
     mlist = _momentum_list(para, N_each)
     if momentum_restriction:
-        wanted = []
-        for k in (momentum_list or []):
-            k = momentum_residue(k, para.Gk)
-            if k not in wanted:
-                wanted.append(k)
+        wanted = _wanted_momenta(para, N_each, True, momentum_list)
         sectors = [mlist.index(k) for k in wanted if k in mlist]
     else:
         sectors = list(range(len(mlist)))

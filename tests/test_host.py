@@ -468,3 +468,72 @@ def test_gauge_fixing_closed_forms():
     ED_connection_gaugefixing([a, b])                # det [[1j, 0], [0, 1]] = 1j
     np.testing.assert_allclose(a.vec, [1.0, 0, 0], atol=1e-15)
     np.testing.assert_allclose(b.vec, [0, 1.0, 0], atol=1e-15)
+
+
+def _brute_sectors(para, N_each, mask=None, rule=None):
+    """All states of the whole bit space, filtered component by component: |N_c| set bits
+    inside component c (inside the mask, when given), momentum residue of the components'
+    residues with holes negated, then the rule; grouped by momentum, ascending."""
+    from momentumed.preparation import _momentum_list, momentum_residue
+    width = para.Nk * para.Nc_mix
+    out = {}
+    for s in range(1 << para.bits):
+        ok, K = True, [0] * para.dim
+        for c, n in enumerate(N_each):
+            field = (s >> (c * width)) & ((1 << width) - 1)
+            if bin(field).count("1") != abs(n):
+                ok = False
+                break
+            if mask is not None and any(((field >> b) & 1) and (c * width + b + 1) not in mask
+                                        for b in range(width)):
+                ok = False
+                break
+            kc = [0] * para.dim
+            for b in range(width):
+                if field >> b & 1:
+                    for d in range(para.dim):
+                        kc[d] += int(para.k_list[d, b % para.Nk])
+            kc = momentum_residue(kc, para.Gk)
+            sgn = (n > 0) - (n < 0)
+            K = [K[d] + sgn * kc[d] for d in range(para.dim)]
+        if not ok or (rule is not None and not rule(MBS64(para.bits, n=s))):
+            continue
+        out.setdefault(momentum_residue(K, para.Gk), []).append(s)
+    order = _momentum_list(para, N_each)
+    return [(k, out[k]) for k in order if k in out]
+
+
+def test_momentum_subspaces_mask_holes_selection():
+    """ED_momentum_subspaces with mask, holes (negative N) and selection_rule
+    (momentum_decomposition.jl:101-152,339-413) against brute force over every state: one
+    component with a mask, two holes, a selection rule; two conserved components with a
+    particle and a hole component under a mask that spans both."""
+    from momentumed.workloads import mesh
+    p1 = EDPara(k_list=mesh(3, 4), Gk=(3, 4))
+    p2 = EDPara(k_list=mesh(2, 3), Gk=(2, 3), Nc_conserve=2)
+    cases = [(p1, [3], [1, 2, 4, 7, 8, 11, 12], None),
+             (p1, [-2], None, None),
+             (p1, [3], None, lambda m: not (m.n & 0b101)),
+             (p2, [2, -1], [1, 2, 3, 5, 8, 9, 10, 12], None),
+             (p2, [1, 2], None, lambda m: bin(m.n).count("1") == 3 and m.n % 3 != 0)]
+    for para, N, mask, rule in cases:
+        subs, ks = me.ED_momentum_subspaces(para, tuple(N), mask=mask, selection_rule=rule)
+        want = _brute_sectors(para, N, mask, rule)
+        assert ks == [k for k, _ in want], (N, mask)
+        for sp, (_, states) in zip(subs, want):
+            assert [int(x) for x in sp.list] == states
+        assert sum(len(s) for _, s in want) > 0
+
+
+def test_host_enumeration_equals_the_oracle(oracle_mod):
+    """The host enumeration (forced by an always-true selection rule) reproduces the
+    oracle's pinned sector lists bit for bit: two conserved components, and a mixed one."""
+    from momentumed.workloads import mesh
+    for para, N in ((EDPara(k_list=mesh(2, 3), Gk=(2, 3), Nc_conserve=2), [2, 3]),
+                    (EDPara(k_list=mesh(1, 5), Gk=(1, 5), Nc_mix=2), [4])):
+        subs, ks = me.ED_momentum_subspaces(para, tuple(N), selection_rule=lambda m: True)
+        lists, oks = oracle_mod.enumerate_sectors(para.k_list, para.Gk, para.Nc_mix,
+                                                  para.Nc_conserve, N)
+        assert ks == oks
+        for sp, ob in zip(subs, lists):
+            np.testing.assert_array_equal(sp.list, ob)
