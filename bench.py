@@ -257,7 +257,7 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (sectors and "
+        print("bench.py measures one GPU: the package has no multi-GPU path (sectors and "
               "parameter points would shard one problem set per GPU). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)

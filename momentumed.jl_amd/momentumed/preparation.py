@@ -344,7 +344,7 @@ def ED_scatterlist_twobody(para: EDPara, *, kshift=None, **H_two_kw) -> list[Sca
     return sort_merge_scatlist(terms)
 
 
-# The north-star names (commented-out aliases in scat_list.jl:574-575), exported for scripts
-# written against the README.
+# The older names of the two term builders (aliases left commented out in
+# scat_list.jl:574-575), exported for scripts that still use them.
 ED_sortedScatterList_onebody = ED_scatterlist_onebody
 ED_sortedScatterList_twobody = ED_scatterlist_twobody

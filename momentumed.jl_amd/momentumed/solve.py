@@ -98,7 +98,7 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
             raise ValueError("No GPU available. Please load a supported GPU extension and "
                              "verify availability.")
     if device != "rocm":
-        raise ValueError(f"device=:{device} is not available in this build (use 'rocm').")
+        raise ValueError(f"device=:{device} is not available in this package (use 'rocm').")
     dev.throw_gpu_unavailable("rocm")
 
     t0 = time.perf_counter()
