@@ -436,9 +436,38 @@ class MBOperator:
     def bits(self) -> int:
         return self.scats[0].bits if self.scats else 0
 
+    def __mul__(self, other):
+        """op * vec (EDCore/src/multiplication.jl:123-129): the operator applied to an
+        MBS64Vector, as a new vector on the same subspace (targets outside it are dropped; an
+        upper-Hermitian list acts with its adjoint half), computed by the device matvec.
+        op2 * op1 is the product that applies op1 and then op2 to a vector (:131-140)."""
+        if isinstance(other, MBOperator):
+            return _OperatorProduct(self, other)
+        if isinstance(other, MBS64Vector):
+            if not self.scats:
+                return MBS64Vector(np.zeros(len(other.vec), dtype=np.complex128), other.space)
+            from .device import create_gpu_linearmap
+            H = create_gpu_linearmap(self, other.space)
+            return MBS64Vector(H(np.ascontiguousarray(other.vec, dtype=np.complex128)),
+                               other.space)
+        return NotImplemented
+
     def __repr__(self):
         h = "(Hermitian, upper triangle entries only) " if self.upper_hermitian else ""
         return f"Many Body Operator {h}:\n\t{len(self.scats)} scattering terms"
+
+
+class _OperatorProduct:
+    """op2 * op1 waiting for its vector: (op2 * op1) * vec applies op1, then op2, each on the
+    vector's subspace (multiplication.jl:131-140)."""
+
+    def __init__(self, op2: MBOperator, op1: MBOperator):
+        self.op2, self.op1 = op2, op1
+
+    def __mul__(self, vec):
+        if not isinstance(vec, MBS64Vector):
+            return NotImplemented
+        return self.op2 * (self.op1 * vec)
 
 
 # ---------------------------------------------------------------------------- spaces

@@ -49,6 +49,28 @@ def test_reference_unit_vectors_on_device(gpu_ctx):
     assert abs(me.ED_bracket(bra, ident, op, ket) - 1.0) < 1e-15
 
 
+def test_operator_times_vector(gpu_ctx, oracle_mod):
+    """op * vec and op2 * op1 * vec (multiplication.jl:123-140) on the device: an
+    upper-Hermitian operator, a non-Hermitian momentum-conserving one, and the product of
+    the two, each against the oracle's dense matrices on the vector's subspace within 1e-13
+    of the scale."""
+    w = W.small()
+    H = w.operator()
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][2]
+    rng = np.random.default_rng(31)
+    # the operator's first 200 terms with random complex amplitudes and no adjoints
+    G = me.MBOperator([me.Scatter(None, bits=15,
+                                  _raw=(complex(rng.normal(), rng.normal()), t.out, t.in_))
+                       for t in H.scats[:200]], upper_hermitian=False)
+    x = me.MBS64Vector(_rand(sp.length, 32), sp)
+    Hd = oracle_mod.dense_matrix(sp.list, _oracle_full(oracle_mod, H))
+    Gd = oracle_mod.dense_matrix(sp.list, _oracle_full(oracle_mod, G))
+    assert np.abs(Gd - Gd.conj().T).max() > 1e-3          # not Hermitian
+    for got, want in ((H * x, Hd @ x.vec), (G * x, Gd @ x.vec), (G * H * x, Gd @ (Hd @ x.vec))):
+        assert got.space is sp
+        assert np.abs(got.vec - want).max() <= 1e-13 * np.abs(want).max()
+
+
 def test_reference_integration_example(gpu_ctx, oracle_mod):
     """The reference's integration test (MomentumED/test/example1_FQH_Laughlin.jl:93-111):
     ED_momentum_subspaces, ED_scatterlist_twobody and EDsolve(subspace, scat; N=10) in every
