@@ -532,6 +532,15 @@ class MBS64Vector:
     def __len__(self):
         return self.vec.size
 
+    def size(self) -> tuple:
+        """manybodystate_vector.jl:70."""
+        return (self.vec.size,)
+
+    def similar(self) -> "MBS64Vector":
+        """manybodystate_vector.jl:77-79: a vector on the same subspace (zeros here; the
+        reference leaves the values undefined)."""
+        return MBS64Vector(np.zeros_like(self.vec), self.space)
+
     def dot(self, other: "MBS64Vector") -> complex:
         """manybodystate_vector.jl:100-103 (conjugates the first argument)."""
         if self.space is not other.space:

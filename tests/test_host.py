@@ -101,6 +101,23 @@ def test_mbs64_reference_vectors():
         MBS64(4, n=16)
 
 
+def test_mbs64vector_operations():
+    """manybodystate_vector.jl:63-117: length, size, similar (same subspace), dot with the
+    first argument conjugated, + and - on one subspace only."""
+    sp = HilbertSubspace([1, 2, 4], bits=3)
+    a = me.MBS64Vector(np.array([1 + 1j, 2, 3j]), sp)
+    b = me.MBS64Vector(np.array([1, 1j, 1]), sp)
+    assert len(a) == 3 and a.size() == (3,)
+    z = a.similar()
+    assert z.space is sp and z.size() == (3,)
+    assert a.dot(b) == np.vdot(a.vec, b.vec)
+    np.testing.assert_array_equal((a + b).vec, a.vec + b.vec)
+    np.testing.assert_array_equal((a - b).vec, a.vec - b.vec)
+    other = me.MBS64Vector(np.zeros(3), HilbertSubspace([1, 2, 4], bits=3))
+    with pytest.raises(AssertionError):
+        a + other
+
+
 def test_mbs64_manipulation_reference_vectors():
     """EDCore/test/test_manybodystate_basis.jl:1-46: bits, physicality, union, equality and
     order, occupation tests and updates, the sign count, flips."""
