@@ -295,8 +295,10 @@ int edm_csr_destroy(edm_csr *m);
 int edm_batch_create(edm_ctx *ctx, edm_batch **out);
 int edm_batch_destroy(edm_batch *bt);
 /* Admit a problem (arguments as edm_eigsolve; want_vectors keeps the nev eigenvectors on the
- * device for edm_batch_result). Allocates the problem's workspace (edm_batch_problem_bytes)
- * from the context: EDM_ENOMEM when the context's HBM budget does not hold it. *id names it. */
+ * device for edm_batch_result). Takes the problem's workspace (edm_batch_problem_bytes) from
+ * the context: with an HBM budget as one allocation of its own (EDM_ENOMEM when the budget
+ * does not hold it), without one as a range of a 1 GiB chunk the batch shares between its
+ * problems (a chunk returns to the device once no problem uses it). *id names it. */
 int edm_batch_add(edm_batch *bt, const edm_op *op, const edm_basis *b, int32_t nev,
                   int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
                   const double *vec0, int32_t want_vectors, int64_t *id);
@@ -322,7 +324,7 @@ int edm_batch_count(edm_batch *bt, int32_t *n_total, int32_t *n_running);
  * problems step often while large ones step about once per D / rows steps. Only the pace
  * changes; each problem's iteration and result do not. */
 int edm_batch_set_quantum(edm_batch *bt, int64_t rows);
-/* Device bytes edm_batch_add allocates for a problem of dimension D. */
+/* Device bytes of the workspace edm_batch_add takes for a problem of dimension D. */
 int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t *bytes);
 
 /* Host eigen-decomposition of a real symmetric n x n matrix (row-major A, not modified):
