@@ -281,7 +281,13 @@ struct Workspace {
     double2 **d_vptrs = nullptr, **d_tmpptrs = nullptr;
     double2 *d_h = nullptr, *d_h2 = nullptr;
     double *d_beta = nullptr, *d_Y = nullptr;
+    // two pinned landing slots for a step's h, h2 and beta, each with the event recorded
+    // after its copy, so the host can wait for one step while the next is already queued
+    double2 *h_pin[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
     size_t vbytes = 0;
+
+    size_t coef_count() const { return 2 * (size_t)(m + 1) + 1; }
 
     Workspace(edm_ctx *c, int64_t D_, int m_, int ntmp) : ctx(c), D(D_), m(m_)
     {
@@ -295,6 +301,11 @@ struct Workspace {
             d_h2 = d_h + (m + 1);
             d_beta = (double *)(d_h2 + (m + 1));
             d_Y = (double *)ctx->alloc((size_t)m * m * sizeof(double));
+            for (int s = 0; s < 2; ++s) {
+                EDM_HIP_CHECK(hipHostMalloc((void **)&h_pin[s], coef_count() * sizeof(double2),
+                                            hipHostMallocDefault));
+                EDM_HIP_CHECK(hipEventCreateWithFlags(&ev[s], hipEventDisableTiming));
+            }
         } catch (...) {
             release();
             throw;
@@ -320,6 +331,14 @@ struct Workspace {
         ctx->free(d_tmpptrs, sizeof(double2 *));
         ctx->free(d_h, (2 * (m + 1) + 1) * sizeof(double2));
         ctx->free(d_Y, (size_t)m * m * sizeof(double));
+        // a step queued ahead may still be copying into a slot when an error unwinds
+        if (h_pin[0] || h_pin[1]) (void)hipStreamSynchronize(ctx->stream);
+        for (int s = 0; s < 2; ++s) {
+            if (h_pin[s]) (void)hipHostFree(h_pin[s]);
+            if (ev[s]) (void)hipEventDestroy(ev[s]);
+            h_pin[s] = nullptr;
+            ev[s] = nullptr;
+        }
         d_vptrs = d_tmpptrs = nullptr;
         d_h = d_h2 = nullptr;
         d_beta = d_Y = nullptr;
@@ -350,9 +369,9 @@ void cgs2(Workspace &ws, int n, double2 *target, std::vector<double2> &h_out)
     }
 }
 
-// CGS2 of w against V[0..n), then beta = ||w|| and w /= beta, with one device-to-host copy
-// and one synchronisation; h_out = first-pass + second-pass coefficients.
-void cgs2_norm_scale(Workspace &ws, int n, double2 *w, std::vector<double2> &h_out, double *beta)
+// Queue CGS2 of w against V[0..n), then beta = ||w|| and w /= beta, then one copy of h, h2
+// and beta into pinned slot `slot`, whose event marks the copy's completion.
+void queue_cgs2_norm_scale(Workspace &ws, int n, double2 *w, int slot)
 {
     edm_ctx *ctx = ws.ctx;
     blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.d_h);
@@ -361,15 +380,24 @@ void cgs2_norm_scale(Workspace &ws, int n, double2 *w, std::vector<double2> &h_o
     blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h2, w, ws.D);
     blas_norm(ctx, w, ws.D, ws.d_beta);
     blas_scale_inv(ctx, w, w, ws.d_beta, ws.D);
-    // h, h2 and beta are adjacent on the device (d_h2 = d_h + m + 1, d_beta follows)
-    std::vector<double2> buf(2 * (size_t)(ws.m + 1) + 1);
-    EDM_HIP_CHECK(hipMemcpyAsync(buf.data(), ws.d_h, buf.size() * sizeof(double2),
+    // h, h2 and beta are adjacent on the device (d_h2 = d_h + m + 1, d_beta follows). Work
+    // queued after this copy may overwrite them: the stream runs the copy first.
+    EDM_HIP_CHECK(hipMemcpyAsync(ws.h_pin[slot], ws.d_h, ws.coef_count() * sizeof(double2),
                                  hipMemcpyDeviceToHost, ctx->stream));
-    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    EDM_HIP_CHECK(hipEventRecord(ws.ev[slot], ctx->stream));
+}
+
+// Wait for slot `slot`'s copy only (work queued behind it keeps running); h_out = first-pass
+// + second-pass coefficients of the n basis vectors.
+void collect_cgs2_norm_scale(Workspace &ws, int n, int slot, std::vector<double2> &h_out,
+                             double *beta)
+{
+    EDM_HIP_CHECK(hipEventSynchronize(ws.ev[slot]));
+    const double2 *buf = ws.h_pin[slot];
+    const double2 *h2 = buf + (ws.m + 1);
     h_out.resize(n);
-    const double2 *h2 = buf.data() + (ws.m + 1);
     for (int i = 0; i < n; ++i) h_out[i] = make_double2(buf[i].x + h2[i].x, buf[i].y + h2[i].y);
-    *beta = *(const double *)(buf.data() + 2 * (ws.m + 1));
+    *beta = *(const double *)(buf + 2 * (ws.m + 1));
 }
 
 double norm_host(Workspace &ws, const double2 *v)
@@ -440,20 +468,36 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
     std::vector<double> res(m, 0.0);
 
     for (iters = 1; iters <= maxiter; ++iters) {
+        // A step queues the matvec, both Gram-Schmidt passes, the norm, the scaling and the
+        // copy of h, h2 and beta. Its successor only needs the scaled vector, which is on the
+        // device, so it is queued before the host waits for this step's coefficients: the
+        // device does not idle during the host's round trip. When a step breaks down, the
+        // step queued behind it is discarded (its vector and coefficients are written again).
+        bool queued = false;   // step j is already on the stream
+        int slot = 0;          // pinned slot of step j's coefficients
         for (int j = k; j < m; ++j) {
-            // one host round trip per step: matvec, both Gram-Schmidt passes, the norm and
-            // the scaling are queued, then h, h2 and beta come back together (a breakdown
-            // overwrites the scaled vector below)
-            mv(ws.V[j], ws.V[j + 1]);
+            if (!queued) {
+                mv(ws.V[j], ws.V[j + 1]);
+                queue_cgs2_norm_scale(ws, j + 1, ws.V[j + 1], slot);
+            }
+            queued = j + 1 < m;
+            if (queued) {
+                mv(ws.V[j + 1], ws.V[j + 2]);
+                queue_cgs2_norm_scale(ws, j + 2, ws.V[j + 2], slot ^ 1);
+            }
             ++matvecs;
             double beta = 0.0;
-            cgs2_norm_scale(ws, j + 1, ws.V[j + 1], h, &beta);
+            collect_cgs2_norm_scale(ws, j + 1, slot, h, &beta);
+            slot ^= 1;
             t(j, j) = h[j].x;
             hnorm = std::max(hnorm, std::fabs(h[j].x) + beta + (j > 0 ? std::fabs(t(j, j - 1)) : 0.0));
             bool breakdown = !(beta > 1e-14 * std::max(hnorm, 1e-300));
             if (breakdown) {
                 // invariant subspace: continue from a fresh random direction orthogonal to
-                // the current basis (coupling zero)
+                // the current basis (coupling zero); the step queued behind this one ran on
+                // the unusable vector and is run again
+                EDM_HIP_CHECK(hipStreamSynchronize(st));
+                queued = false;
                 beta = 0.0;
                 for (int attempt = 0; attempt < 4; ++attempt) {
                     blas_fill_random(ctx, ws.V[j + 1], D, seed ^ (0xA5A5A5A5ULL + ++rng_stream), 0);
