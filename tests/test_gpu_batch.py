@@ -137,6 +137,43 @@ def test_batch_not_converged_and_vec0_match_alone(gpu_ctx):
     assert res[b].vectors is None
 
 
+def test_breakdown_mid_cycle_lone_and_batched(gpu_ctx):
+    """A start vector inside a three-dimensional invariant subspace (three basis states of a
+    diagonal one-body operator with distinct diagonal entries): the Krylov space runs out at
+    the third step of the first cycle, exactly (every vector stays on those three states), so
+    the lone solve discards the step it had already queued behind that one and continues from
+    a random direction; the lowest four eigenvalues are then the four smallest diagonal
+    entries, and the batched solve (whose host step handles the same breakdown) is bitwise
+    the lone one."""
+    print("[batch] breakdown at step 3: diagonal 3x5 N=5 operator, vec0 on three states",
+          flush=True)
+    eps = [0.37 * np.sqrt(k + 2.0) + 0.011 * k * k for k in range(15)]
+    para = me.EDPara(k_list=W.mesh(3, 5), Gk=(3, 5),
+                     H_one=lambda k, cf, ci: eps[k - 1] if cf == ci else 0.0,
+                     one_momentum_coordinate=False)
+    op = me.MBOperator(me.ED_scatterlist_onebody(para), upper_hermitian=True)
+    sp = me.ED_momentum_subspaces(para, 5)[0][0]
+    states = np.asarray(sp.list, dtype=np.uint64)
+    occ = (states[:, None] >> np.arange(15, dtype=np.uint64)) & np.uint64(1)
+    diag = occ.astype(float) @ np.asarray(eps)
+    order = np.argsort(diag)
+    assert sp.length > 100 and np.min(np.diff(diag[order])) > 1e-9
+    v0 = np.zeros(sp.length, complex)
+    v0[order[-3:]] = 1.0            # the three largest entries: the answer needs the restart
+    H = dev.create_gpu_linearmap(op, sp)
+    vals, vecs, info = _alone(H, 4, vec0=v0)
+    assert info["converged"] == 4
+    np.testing.assert_allclose(vals, diag[order][:4], rtol=1e-10, atol=1e-12)
+    for i in range(4):
+        assert abs(abs(vecs[i][order[i]]) - 1.0) < 1e-9
+    with B.BatchSolver(gpu_ctx) as bs:
+        pid = bs.add(H, 4, vec0=v0, want_vectors=True)
+        res = bs.run()
+    np.testing.assert_array_equal(res[pid].values, vals)
+    np.testing.assert_array_equal(np.array(res[pid].vectors), vecs)
+    assert res[pid].info["matvecs"] == info["matvecs"]
+
+
 def _map_on(ctx, op, sp):
     system, sector = sp.sector_info
     basis = dev.DeviceBasis.upload(ctx, sp.list, sp.bits, system, sector)
