@@ -53,6 +53,25 @@ def _kernel_name(rank_mode: int, block: int = 512) -> str:
     return f"k_matvec_ldsu<{rank}, true, {u}, {block if u == 1 else 512}>"
 
 
+def cpu_share():
+    """Host threads the CPU baseline uses, and what it knows of the host: the threads this
+    process may run on (sched_getaffinity), capped by OMP_NUM_THREADS when that is set (a
+    GPU box of this pool gives one GPU's job a share of 16), nproc and the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, {"threads": threads, "affinity_cpus": aff, "nproc": os.cpu_count(),
+                     "omp_num_threads": omp, "cpu_model": model}
+
+
 def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
     """The oracle's row-gather matvec (restatement of linear_map.jl:100-118: every term tried
     on every row, binary search for partners) on a random row sample, scaled to matvecs/s."""
@@ -131,6 +150,117 @@ def block_section(ctx, dop, basis, x, y, stream, args, nnz_off, S, ms_single):
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
                          "bytes_model": "D*(8+4*32) + 4*16*nnz_off + 40*S_full",
                          "algorithmic_bytes": B}}
+
+
+def config_section(name, args):
+    """The small or medium configuration solved on the GPU and on the host.
+
+    small: all 15 sectors of the 3x5 mesh, N = 5, lowest 8 (krylovdim 30, tol 1e-12);
+    medium: sector (0,0) of the 3x9 mesh, N = 9, lowest 6, krylovdim 40, tol 1e-12.
+    GPU: krylov_map_solve per sector (operator and bases resident), sector-solves/s, and the
+    matvec of the first sector timed with HIP events (GB/s of the bytes model). CPU: the
+    oracle's restatement of the reference's map (linear_map.jl:100-118, OpenMP over rows)
+    inside a thick-restart Lanczos (oracle/lanczos.py) on the host's CPU share; small: every
+    sector solved; medium: one Krylov cycle (krylovdim steps) timed and scaled to the GPU
+    solve's matvec count, labelled extrapolated."""
+    import numpy as np
+    import torch
+    import momentumed as me
+    from momentumed import device as dev
+    from momentumed import workloads as W
+    from momentumed._lib import check
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lanczos
+    import oracle
+    w = W.get(name)
+    op = w.operator()
+    if name == "small":
+        spaces, ks = me.ED_momentum_subspaces(w.para, w.N_each)
+    else:
+        spaces, ks = me.ED_momentum_subspaces(w.para, w.N_each, momentum_restriction=True,
+                                              momentum_list=[w.sector])
+    ctx = dev.get_context(0)
+    maps = [dev.create_gpu_linearmap(op, sp) for sp in spaces]
+    kw = dict(krylovdim=w.krylovdim, tol=w.tol, want_vectors=False)
+    dev.krylov_map_solve(maps[0], w.nev, **kw)   # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    gpu_vals, mvs = [], 0
+    for H in maps:
+        v, _, info = dev.krylov_map_solve(H, w.nev, **kw)
+        gpu_vals.append(v)
+        mvs += info["matvecs"]
+    t_gpu = time.perf_counter() - t0
+    H = maps[0]
+    D = H.basis.D
+    nnz, nnz_off = ctypes.c_int64(), ctypes.c_int64()
+    check(ctx.L.edm_connectivity_totals(ctx.h, H.op.h, H.basis.h, ctypes.byref(nnz),
+                                        ctypes.byref(nnz_off)), ctx.h)
+    x = torch.empty(D, dtype=torch.complex128, device="cuda:0")
+    y = torch.empty_like(x)
+    check(ctx.L.edm_fill_random(ctx.h, x.data_ptr(), D, 12345, 1), ctx.h)
+    ctx.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream, device="cuda:0")
+    reps = 200 if name == "small" else 50
+    for _ in range(5):
+        H.matvec_device(y.data_ptr(), x.data_ptr())
+    ctx.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        H.matvec_device(y.data_ptr(), x.data_ptr())
+    ev1.record(stream)
+    ev1.synchronize()
+    ms_mv = ev0.elapsed_time(ev1) / reps
+    B = D * 40 + nnz_off.value * 16 + H.op.S_full * 40
+    out = {"workload": w.name, "sectors": len(spaces), "D": [sp.length for sp in spaces],
+           "nev": w.nev, "krylovdim": w.krylovdim, "tol": w.tol,
+           "gpu_seconds": round(t_gpu, 4), "gpu_matvecs": mvs,
+           "sector_solves_per_s": round(len(spaces) / t_gpu, 2),
+           "matvec": {"sector": list(ks[0]), "ms": round(ms_mv, 5),
+                      "algorithmic_bytes": B, "GB_per_s": round(B / (ms_mv * 1e-3) / 1e9, 1),
+                      "nnz_offdiag": nnz_off.value,
+                      "bytes_model": "D*40 + nnz_off*16 + S_full*40"}}
+    log(f"[{name}] GPU: {len(spaces)} solves in {t_gpu:.3f} s, matvec {ms_mv:.4f} ms")
+    if args.no_cpu_baseline:
+        return out
+    threads, host = cpu_share()
+    amp, o, i = op.arrays()
+    full = oracle.full_list(oracle.TermList(amp, o, i), True)
+    cpu = {"kind": "port", "cores": threads, "host": host,
+           "algorithm": "oracle restatement of linear_map.jl:100-118 (every full-list term "
+                        "tried on every row, binary search) inside oracle/lanczos.py"}
+    if name == "small":
+        t0 = time.perf_counter()
+        err = 0.0
+        for sp, gv in zip(spaces, gpu_vals):
+            b = np.ascontiguousarray(sp.list)
+            vals, _, _ = lanczos.eigsolve(
+                lambda v, b=b: oracle.matvec(b, full, v, nthreads=threads), len(b), w.nev,
+                krylovdim=w.krylovdim, tol=w.tol)
+            err = max(err, float(np.abs(vals - gv).max() / np.abs(gv).max()))
+        t_cpu = time.perf_counter() - t0
+        cpu.update({"seconds": round(t_cpu, 3), "sample": f"all {len(spaces)} sectors solved",
+                    "sector_solves_per_s": round(len(spaces) / t_cpu, 3),
+                    "eigenvalues_max_rel_diff_vs_gpu": err})
+    else:
+        b = np.ascontiguousarray(spaces[0].list)
+        t0 = time.perf_counter()
+        _, _, info = lanczos.eigsolve(lambda v: oracle.matvec(b, full, v, nthreads=threads),
+                                      len(b), w.nev, krylovdim=w.krylovdim, tol=w.tol,
+                                      maxiter=1)
+        t_cyc = time.perf_counter() - t0
+        per_step = t_cyc / info["matvecs"]
+        t_cpu = per_step * mvs
+        cpu.update({"seconds": round(t_cpu, 2), "extrapolated": True,
+                    "sample": f"one {info['matvecs']}-step Lanczos cycle timed "
+                              f"({t_cyc:.1f} s), scaled by the GPU solve's {mvs} matvecs",
+                    "sector_solves_per_s": round(1.0 / t_cpu, 5)})
+    cpu["gpu_speedup"] = round(t_cpu / t_gpu, 1)
+    log(f"[{name}] CPU ({threads} threads): {cpu['seconds']} s")
+    out["cpu"] = cpu
+    return out
 
 
 def batched_section(args):
@@ -212,20 +342,24 @@ def served_section(args):
     one-at-a-time server, same requests, same HBM budget and queue bound."""
     from momentumed import serve as S
     reqs = S.served_stream(rate=50.0, duration=args.serve_seconds, seed=7)
+    prefix = [r for r in reqs if r.arrival < args.serve_plain_seconds]
     out = {"stream": f"Poisson 50 req/s for {args.serve_seconds:g} s (seed 7), meshes "
                      f"{['%dx%d' % m for m in S.MESHES]}, fillings 1/3, 2/5 (non-integer N "
                      f"redrawn), one random sector, screening ~U(1,4), nev 4",
-           "hbm_budget_bytes": 200 * 10 ** 9, "max_queue": 256, "drain_s": args.serve_drain}
-    for mode in ("batched", "plain"):
+           "hbm_budget_bytes": 200 * 10 ** 9, "max_queue": 256, "drain_s": args.serve_drain,
+           "plain_stream": f"the first {args.serve_plain_seconds:g} s of the same stream "
+                           f"({len(prefix)} requests), same budget and queue bound"}
+    for mode, rq in (("batched", reqs), ("plain", prefix)):
         srv = S.SolveServer(hbm_budget_bytes=200 * 10 ** 9, max_queue=256, mode=mode)
         try:
-            rep = srv.run(reqs, drain_s=args.serve_drain)
+            rep = srv.run(rq, drain_s=args.serve_drain)
         finally:
             srv.close()
         summ = rep.summary()
         log(f"[served] {mode}: {summ['completed']}/{summ['offered']} done, "
             f"{summ['rejected']} rejected, {summ['unfinished']} unfinished, "
-            f"{summ['solves_per_s']}/s, p50 {summ['p50_s']}, p99 {summ['p99_s']}")
+            f"{summ['solves_per_s']}/s, p50 {summ['p50_s']}, p99 {summ['p99_s']}, "
+            f"p99 with unfinished {summ['p99_all_s']}")
         out[mode] = summ
     b, p = out["batched"]["solves_per_s"], out["plain"]["solves_per_s"]
     out["speedup_vs_plain"] = round(b / p, 2) if b and p else None
@@ -249,8 +383,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batch-ratios", type=int, default=64)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
-    ap.add_argument("--serve-drain", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=120.0)
+    ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
+                    help="the plain (one solve at a time) baseline serves this prefix")
+    ap.add_argument("--serve-drain", type=float, default=30.0)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the small and medium configuration lines")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, run 3 plain device copies of one vector "
                          "(16*D bytes read + written each) to calibrate rocprof byte counters")
@@ -390,14 +528,17 @@ def main():
         basis_h = basis.download()
         x_h = x.cpu().numpy()
         y_h = y.cpu().numpy()
-        nthreads = min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline on {nthreads} threads (~{args.cpu_seconds:.0f} s)")
+        nthreads, host = cpu_share()
+        log(f"[bench] cpu baseline on {nthreads} threads (~{args.cpu_seconds:.0f} s): {host}")
         cb, err = cpu_baseline(w, basis_h, full, x_h, y_h, args.cpu_seconds, nthreads)
+        cb["host"] = host
         out["cpu_baseline"] = cb
         out["parity_sampled_rows_max_rel_err"] = err
         log(f"[bench] cpu baseline {cb['value']:.3e} matvec/s; GPU/CPU rows max rel err {err:.2e}")
     else:
         out["cpu_baseline"] = None
+    if not args.no_configs:
+        out["configs"] = {n: config_section(n, args) for n in ("small", "medium")}
     if not args.no_batched:
         out["batched"] = batched_section(args)
     if not args.no_served:

@@ -43,7 +43,7 @@ typedef struct edm_csr edm_csr;
 
 /* Description of a momentum-conserved system and particle numbers
  * (EDPara, MomentumED/src/preparation/init_parameter.jl:62-146, and the N_each_component
- * argument of ED_momentum_subspaces, momentum_decomposition.jl:339-343). */
+ * argument of ED_momentum_subspaces, momentum_decomposition.jl:188-192). */
 typedef struct {
     int32_t dim;             /* spatial dimension of momenta, 1..3 */
     int32_t nk;              /* Nk = number of momenta (columns of k_list) */
@@ -83,8 +83,8 @@ int edm_ctx_stream(edm_ctx *ctx, void **stream);
 int edm_ctx_memory(edm_ctx *ctx, uint64_t *used_bytes, uint64_t *budget_bytes);
 
 /* ----------------------------------------------------------------- basis (a1) */
-/* ED_momentum_subspaces (momentum_decomposition.jl:339-413): number of momentum sectors in
- * momentum_list order (first coordinate fastest, :360-369), including empty ones. */
+/* ED_momentum_subspaces (momentum_decomposition.jl:188-262): number of momentum sectors in
+ * momentum_list order (first coordinate fastest, :209-219), including empty ones. */
 int edm_num_sectors(const edm_system *sys, int64_t *nsec);
 /* Sector sizes of every sector, enumerated on the device (counts[nsec]). */
 int edm_sector_sizes(edm_ctx *ctx, const edm_system *sys, int64_t *counts);
@@ -93,7 +93,7 @@ int edm_sector_sizes(edm_ctx *ctx, const edm_system *sys, int64_t *counts);
  * momentum_list. The handle carries rank tables for partner lookup. */
 int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm_basis **out);
 /* Every sector's basis at once, for ED_momentum_subspaces without momentum_restriction
- * (momentum_decomposition.jl:339-413, which walks all states once on the host): one device
+ * (momentum_decomposition.jl:188-262, which walks all states once on the host): one device
  * counting pass per 64 sectors and one writing pass per 16 sectors over the occupation rank
  * space, instead of a counting and a writing pass per sector. out[k] (k < the number of
  * sectors in momentum_list, edm_sector_sizes order) receives sector k's basis, as
@@ -101,16 +101,28 @@ int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm
 int edm_basis_enumerate_all(edm_ctx *ctx, const edm_system *sys, edm_basis **out);
 /* HilbertSubspace(list) (EDCore/src/types/hilbert_subspace.jl:19-30) uploaded as-is
  * (must be sorted ascending, unique). `sys` may be NULL; when given and the list is the
- * complete sector `sector`, rank tables replace the binary search
- * (hilbert_subspace.jl:67-72 semantics are kept either way). */
+ * complete sector `sector`, that sector's rank tables replace the binary search; otherwise
+ * rank tables are derived from the list when it splits into equal blocks (rank mode 2,
+ * edm_basis_rank_mode). hilbert_subspace.jl:67-72 semantics are kept either way. */
 int edm_basis_upload(edm_ctx *ctx, const uint64_t *states, int64_t D, int32_t nbits,
                      const edm_system *sys, int64_t sector, edm_basis **out);
 int edm_basis_size(const edm_basis *b, int64_t *D);
 int edm_basis_download(const edm_basis *b, uint64_t *states_host);
 /* Device pointer to the D sorted states (valid until edm_basis_destroy). */
 int edm_basis_device_ptr(const edm_basis *b, const uint64_t **states_dev);
-/* 1 if partner indices come from rank tables, 0 if from binary search. */
+/* How partner indices are found (searchsortedfirst semantics, hilbert_subspace.jl:67-72, in
+ * every mode): 0 = binary search over the sorted states (gpu_interface.jl:98-113);
+ * 1 = rank tables of the complete sector `sector` of the system given at upload or
+ * enumeration; 2 = rank tables derived from the state list alone (edm_basis_upload without a
+ * system, or with one whose sector the list is not), which an operator uses only after a
+ * device check that every target of every state under it lies in the basis, and binary
+ * search otherwise. The reference's device extension uploads bare state lists
+ * (MomentumEDCUDAExt.jl:94-113), so mode 2 is what its call sequence gets. */
 int edm_basis_rank_mode(const edm_basis *b, int32_t *mode);
+/* Release the rank tables (mode 0 from then on). No reference counterpart: it lets a caller
+ * compare the two lookups or give back the tables' memory. Not while a batch holds a
+ * problem on this basis (its step plans point at the tables). */
+int edm_basis_drop_rank(edm_basis *b);
 int edm_basis_destroy(edm_basis *b);
 
 /* ----------------------------------------------------------------- operator (a2-a5) */
@@ -324,6 +336,11 @@ int edm_batch_count(edm_batch *bt, int32_t *n_total, int32_t *n_running);
  * problems step often while large ones step about once per D / rows steps. Only the pace
  * changes; each problem's iteration and result do not. */
 int edm_batch_set_quantum(edm_batch *bt, int64_t rows);
+/* Weight of one problem in that round robin (default 1): its credit per step is
+ * weight * rows, so it steps weight times as often as a problem of the same dimension and
+ * weight 1. A server uses it to reserve device time for a large problem that would
+ * otherwise step only once every D / rows steps. Pace only; the iteration is unchanged. */
+int edm_batch_set_weight(edm_batch *bt, int64_t id, double weight);
 /* Device bytes of the workspace edm_batch_add takes for a problem of dimension D. */
 int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t *bytes);
 

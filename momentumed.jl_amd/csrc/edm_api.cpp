@@ -41,18 +41,24 @@ void free_basis(edm_basis *b)
     delete b;
 }
 
+void drop_tables(edm_basis *b)
+{
+    if (b->d_jlo) b->ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
+    if (b->d_jhi) b->ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
+    if (b->d_jlo16) b->ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
+    b->d_jlo = b->d_jhi = nullptr;
+    b->d_jlo16 = nullptr;
+    b->rank_mode = 0;
+    b->rank_derived = false;
+}
+
 // keep freshly built rank tables only when they reproduce every state's position
 void verify_or_drop(edm_basis *b)
 {
-    if (verify_rank_tables(b)) {
+    if (verify_rank_tables(b))
         b->rank_mode = 1;
-    } else {
-        b->ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
-        b->ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
-        if (b->d_jlo16) b->ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
-        b->d_jlo = b->d_jhi = nullptr;
-        b->d_jlo16 = nullptr;
-    }
+    else
+        drop_tables(b);
 }
 
 void attach_rank(edm_basis *b)
@@ -272,6 +278,23 @@ int edm_basis_upload(edm_ctx *ctx, const uint64_t *states, int64_t D, int32_t nb
                                              hipMemcpyHostToDevice, ctx->stream));
             EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
             if (sys && sector >= 0) attach_rank(b);
+            if (b->rank_mode == 0) {
+                // derived tables are an optimisation: an upload that cannot hold them (HBM
+                // budget) still succeeds, with binary search
+                bool got = false;
+                try {
+                    got = derive_rank_tables(b, states);
+                } catch (const EdmError &) {
+                    got = false;
+                }
+                if (got) {
+                    b->rank_derived = true;
+                    verify_or_drop(b);
+                    if (b->rank_mode == 0) b->rank_derived = false;
+                } else {
+                    drop_tables(b);
+                }
+            }
         } catch (...) {
             free_basis(b);
             throw;
@@ -311,8 +334,18 @@ int edm_basis_device_ptr(const edm_basis *b, const uint64_t **states_dev)
 int edm_basis_rank_mode(const edm_basis *b, int32_t *mode)
 {
     if (!b || !mode) return EDM_EINVAL;
-    *mode = b->rank_mode;
+    *mode = b->rank_mode == 1 && b->rank_derived ? 2 : b->rank_mode;
     return EDM_OK;
+}
+
+int edm_basis_drop_rank(edm_basis *b)
+{
+    if (!b) return EDM_EINVAL;
+    return guarded(b->ctx, [&]() -> int {
+        EDM_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
+        drop_tables(b);
+        return EDM_OK;
+    });
 }
 
 int edm_basis_destroy(edm_basis *b)
@@ -696,7 +729,7 @@ int edm_rdm1(edm_ctx *ctx, const edm_basis *b, const void *psi_dev, int32_t npai
         if (b->D == 0 || npairs == 0) return EDM_OK;
         // a complete sector's rank tables hold every target of a sector-conserving c_i^+ c_j;
         // every target of any other pair lies outside the sector (entry exactly 0)
-        const bool lin = b->rank_mode == 1 && b->has_spec;
+        const bool lin = b->rank_mode == 1 && b->has_spec && !b->rank_derived;
         std::vector<int2> use;
         std::vector<int> where;
         for (int p = 0; p < npairs; ++p) {
@@ -968,6 +1001,12 @@ int edm_batch_set_quantum(edm_batch *bt, int64_t rows)
 {
     EDM_NULL_BATCH(bt);
     return guarded(batch_ctx(bt), [&]() -> int { return batch_set_quantum(bt, rows); });
+}
+
+int edm_batch_set_weight(edm_batch *bt, int64_t id, double weight)
+{
+    EDM_NULL_BATCH(bt);
+    return guarded(batch_ctx(bt), [&]() -> int { return batch_set_weight(bt, id, weight); });
 }
 
 int edm_batch_problem_bytes(int64_t D, int32_t nev, int32_t krylovdim, uint64_t *bytes)

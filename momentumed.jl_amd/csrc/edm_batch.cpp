@@ -298,6 +298,7 @@ struct edm_problem {
     uint64_t rng_stream = 0;
     int state = kRunning;
     int64_t credit = 0;              // rows of step quantum accumulated (set_quantum)
+    double weight = 1.0;             // credit per step = weight * quantum (set_weight)
     std::vector<double> vals;
     std::chrono::steady_clock::time_point t0;
     double seconds = 0.0;
@@ -615,15 +616,20 @@ static int batch_step_once(edm_batch *bt)
     if (act.empty() && parked.empty()) return 0;
     if (bt->quantum > 0 && !act.empty()) {
         // deficit round robin over rows: each step credits every running problem with
-        // `quantum` rows; a problem takes part when its credit covers its dimension, so
-        // a problem of D rows steps about once every D / quantum steps and every problem
-        // receives about the same device work per step
+        // weight * `quantum` rows; a problem takes part when its credit covers its
+        // dimension, so a problem of D rows steps about once every D / (weight * quantum)
+        // steps and every problem of weight 1 receives about the same device work per step
+        auto per_step = [&](const edm_problem *p) {
+            return std::max<int64_t>(1, (int64_t)std::llround(p->weight * (double)bt->quantum));
+        };
         int64_t rounds = INT64_MAX;
-        for (edm_problem *p : act)
-            rounds = std::min(rounds, std::max<int64_t>(1, (p->D - p->credit + bt->quantum - 1) / bt->quantum));
+        for (edm_problem *p : act) {
+            const int64_t q = per_step(p);
+            rounds = std::min(rounds, std::max<int64_t>(1, (p->D - p->credit + q - 1) / q));
+        }
         std::vector<edm_problem *> sel;
         for (edm_problem *p : act) {
-            p->credit += rounds * bt->quantum;
+            p->credit += rounds * per_step(p);
             if (p->credit >= p->D) {
                 p->credit = std::min(p->credit - p->D, p->D);
                 sel.push_back(p);
@@ -891,6 +897,14 @@ int batch_count(edm_batch *bt, int *n_total, int *n_running)
     for (auto &kv : bt->probs) run += kv.second->state == kRunning;
     if (n_total) *n_total = (int)bt->probs.size();
     if (n_running) *n_running = run;
+    return EDM_OK;
+}
+
+int batch_set_weight(edm_batch *bt, int64_t id, double weight)
+{
+    EDM_REQUIRE(weight > 0 && std::isfinite(weight), EDM_EINVAL, "weight must be positive");
+    edm_problem *p = find_problem(bt, id);
+    p->weight = weight;
     return EDM_OK;
 }
 

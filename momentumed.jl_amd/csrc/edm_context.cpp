@@ -53,7 +53,7 @@ SystemSpec SystemSpec::from(const edm_system *s)
     sp.kmin.assign(sp.dim, 0);
     sp.krange.assign(sp.dim, 0);
     sp.nsec = 1;
-    for (int d = 0; d < sp.dim; ++d) {   // momentum_decomposition.jl:360-369
+    for (int d = 0; d < sp.dim; ++d) {   // momentum_decomposition.jl:209-219
         if (sp.gk[d] != 0) {
             sp.kmin[d] = 0;
             sp.krange[d] = sp.gk[d];

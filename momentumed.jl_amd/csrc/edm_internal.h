@@ -102,6 +102,10 @@ struct edm_basis {
     bool has_spec = false;
     SystemSpec spec;
     int64_t sector = -1;
+    // rank tables derived from the state list alone (no system given): valid for the states
+    // themselves; an operator uses them only after a device check that every target of every
+    // state is in the basis (op_closed)
+    bool rank_derived = false;
 };
 
 // ------------------------------------------------------------------ operator
@@ -208,6 +212,10 @@ bool build_rank_tables(edm_basis *b);
 // the same for many complete sectors of one system (J_lo shared, J_hi on host threads);
 // entry k tells whether bs[k] got tables
 std::vector<bool> build_rank_tables_many(const std::vector<edm_basis *> &bs);
+// rank tables from the sorted state list alone (states_host[D]), when it splits into equal
+// blocks: J_hi[hi] = first index with that high half, J_lo[lo] = the position of lo in its
+// block, the same in every block that holds it. Uploads them and returns true, or false.
+bool derive_rank_tables(edm_basis *b, const uint64_t *states_host);
 // matvec (edm_matvec.hip)
 enum class ApplyMode { Forward, Adjoint };
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
@@ -222,6 +230,9 @@ void launch_connectivity(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int
                          int64_t nrows, int64_t *d_counts, const int64_t *d_rowptr,
                          int64_t *d_cols, int32_t *d_terms, int8_t *d_negs);
 bool op_conserves(const edm_op *op, const edm_basis *b);
+// every target of every basis state under the operator's full list, in both directions, is
+// in the basis (checked on the device by binary search, cached per basis)
+bool op_closed(const edm_op *op, const edm_basis *b);
 void launch_conn_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int64_t *nnz,
                         int64_t *nnz_off);
 edm_op *build_op(edm_ctx *ctx, int nbits, const double *amp, const uint64_t *out,
@@ -312,6 +323,7 @@ int batch_remove(edm_batch *bt, int64_t id);
 int batch_count(edm_batch *bt, int *n_total, int *n_running);
 uint64_t batch_problem_bytes(int64_t D, int nev, int krylovdim);
 int batch_set_quantum(edm_batch *bt, int64_t rows);
+int batch_set_weight(edm_batch *bt, int64_t id, double weight);
 // vector kernels (edm_blas.hip)
 void blas_fill_random(edm_ctx *ctx, double2 *v, int64_t D, uint64_t seed, int mode);
 // h[i] = <V_i, w> for i < nvec (V given by a device pointer table); result on device

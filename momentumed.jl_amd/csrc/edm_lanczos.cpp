@@ -323,6 +323,9 @@ struct Workspace {
     }
     void release()
     {
+        // A step queued ahead may still be running when an error unwinds: it writes V[j+2],
+        // d_h and a pinned slot. Wait for the stream before any of them is freed.
+        (void)hipStreamSynchronize(ctx->stream);
         for (auto p : V) ctx->free(p, vbytes);
         for (auto p : tmp) ctx->free(p, vbytes);
         V.clear();
@@ -331,8 +334,6 @@ struct Workspace {
         ctx->free(d_tmpptrs, sizeof(double2 *));
         ctx->free(d_h, (2 * (m + 1) + 1) * sizeof(double2));
         ctx->free(d_Y, (size_t)m * m * sizeof(double));
-        // a step queued ahead may still be copying into a slot when an error unwinds
-        if (h_pin[0] || h_pin[1]) (void)hipStreamSynchronize(ctx->stream);
         for (int s = 0; s < 2; ++s) {
             if (h_pin[s]) (void)hipHostFree(h_pin[s]);
             if (ev[s]) (void)hipEventDestroy(ev[s]);

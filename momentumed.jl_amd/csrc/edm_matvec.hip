@@ -111,6 +111,42 @@ __device__ __forceinline__ void visit_row(uint64_t s, const OpDev &op, const Ran
     }
 }
 
+// Flags (*miss = 1) a basis state with an off-diagonal target outside the basis: the same
+// candidates as visit_row, each looked up by binary search.
+__global__ void __launch_bounds__(kRowBlock)
+k_closure_miss(int64_t D, const uint64_t *__restrict__ states, OpDev op, RankSearch rank,
+               unsigned int *miss)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint64_t s = states[i];
+    bool bad = false;
+    auto bucket = [&](int key, uint64_t outm) {
+        const uint32_t e0 = op.keyptr[key], e1 = op.keyptr[key + 1];
+        const uint64_t mid = s & ~outm;
+        for (uint32_t e = e0; e < e1 && !bad; ++e) {
+            const uint64_t in = op.tin[e];
+            if (in == outm || (mid & in)) continue;
+            if (rank(mid | in) < 0) bad = true;
+        }
+    };
+    uint64_t rem = s;
+    while (rem && !bad) {
+        int b = __builtin_ctzll(rem);
+        rem &= rem - 1;
+        uint64_t ob = 1ULL << b;
+        if (op.has_onebody) bucket(b, ob);
+        uint64_t low = s & (ob - 1);
+        int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low && !bad) {
+            int a = __builtin_ctzll(low);
+            low &= low - 1;
+            bucket(kb + a, ob | (1ULL << a));
+        }
+    }
+    if (bad) atomicOr(miss, 1u);
+}
+
 template <class Rank>
 __global__ void __launch_bounds__(kRowBlock)
 k_matvec_fast(int64_t D, const uint64_t *__restrict__ states, OpDev op, Rank rank,
@@ -1232,7 +1268,8 @@ OpDev opdev(const edm_op *op, const OpTable &t)
 
 bool use_lin(const edm_op *op, const edm_basis *b)
 {
-    return b->rank_mode == 1 && op_conserves(op, b);
+    if (b->rank_mode != 1) return false;
+    return b->rank_derived ? op_closed(op, b) : op_conserves(op, b);
 }
 
 RankLin make_lin(const edm_basis *b)
@@ -1256,6 +1293,49 @@ bool op_conserves(const edm_op *op, const edm_basis *b)
     bool r = op_conserves_uncached(op, b);
     op->conserve_cache.emplace_back(b->id, r);
     return r;
+}
+
+static bool op_closed_uncached(const edm_op *op, const edm_basis *b);
+
+bool op_closed(const edm_op *op, const edm_basis *b)
+{
+    for (auto &kv : op->conserve_cache)
+        if (kv.first == b->id) return kv.second;
+    bool r = op_closed_uncached(op, b);
+    op->conserve_cache.emplace_back(b->id, r);
+    return r;
+}
+
+// One- and two-body number-conserving operators only (the bucketed tables); the forward
+// table and, for a list that is not upper-Hermitian, the adjoint one.
+static bool op_closed_uncached(const edm_op *op, const edm_basis *b)
+{
+    if (!op->fast || op->nbits != b->nbits) return false;
+    if (b->D == 0) return true;
+    edm_ctx *ctx = b->ctx;
+    unsigned int *d_miss = (unsigned int *)ctx->alloc(sizeof(unsigned int));
+    unsigned int miss = 0;
+    try {
+        EDM_HIP_CHECK(hipMemsetAsync(d_miss, 0, sizeof(unsigned int), ctx->stream));
+        hipLaunchKernelGGL(k_closure_miss, dim3(nblk(b->D)), dim3(kRowBlock), 0, ctx->stream,
+                           b->D, b->d_states, opdev(op, op->fwd), make_search(b), d_miss);
+        EDM_HIP_CHECK(hipGetLastError());
+        if (!op->upper) {
+            hipLaunchKernelGGL(k_closure_miss, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                               ctx->stream, b->D, b->d_states, opdev(op, op->adj),
+                               make_search(b), d_miss);
+            EDM_HIP_CHECK(hipGetLastError());
+        }
+        EDM_HIP_CHECK(hipMemcpyAsync(&miss, d_miss, sizeof(miss), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->free(d_miss, sizeof(unsigned int));
+        throw;
+    }
+    ctx->free(d_miss, sizeof(unsigned int));
+    return miss == 0;
 }
 
 static bool op_conserves_uncached(const edm_op *op, const edm_basis *b)

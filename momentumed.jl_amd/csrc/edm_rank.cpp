@@ -124,23 +124,24 @@ bool high_table(const SystemSpec &sp, const LowTables &lo, int64_t sector, int64
     return (int64_t)run == D;
 }
 
-void upload_tables(edm_basis *b, const LowTables &lo, const std::vector<uint32_t> &jhi)
+void upload_tables(edm_basis *b, int nlo, int nhi, const std::vector<uint32_t> &jlo,
+                   const std::vector<uint32_t> &jhi)
 {
     edm_ctx *ctx = b->ctx;
-    const uint64_t nlo_vals = 1ULL << lo.nlo, nhi_vals = 1ULL << lo.nhi;
-    b->nlo = lo.nlo;
-    b->nhi = lo.nhi;
+    const uint64_t nlo_vals = 1ULL << nlo, nhi_vals = 1ULL << nhi;
+    b->nlo = nlo;
+    b->nhi = nhi;
     b->d_jlo = (uint32_t *)ctx->alloc(nlo_vals * sizeof(uint32_t));
     b->d_jhi = (uint32_t *)ctx->alloc(nhi_vals * sizeof(uint32_t));
-    EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo, lo.jlo.data(), nlo_vals * sizeof(uint32_t),
+    EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo, jlo.data(), nlo_vals * sizeof(uint32_t),
                                  hipMemcpyHostToDevice, ctx->stream));
     EDM_HIP_CHECK(hipMemcpyAsync(b->d_jhi, jhi.data(), nhi_vals * sizeof(uint32_t),
                                  hipMemcpyHostToDevice, ctx->stream));
     uint32_t mx = 0;
-    for (uint32_t v : lo.jlo) mx = std::max(mx, v);
+    for (uint32_t v : jlo) mx = std::max(mx, v);
     std::vector<uint16_t> j16;   // must outlive the asynchronous copy below
     if (mx <= 0xFFFF) {
-        j16.assign(lo.jlo.begin(), lo.jlo.end());
+        j16.assign(jlo.begin(), jlo.end());
         b->d_jlo16 = (uint16_t *)ctx->alloc(nlo_vals * sizeof(uint16_t));
         EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo16, j16.data(), nlo_vals * sizeof(uint16_t),
                                      hipMemcpyHostToDevice, ctx->stream));
@@ -156,7 +157,7 @@ bool build_rank_tables(edm_basis *b)
     LowTables lo = low_tables(b->spec);
     std::vector<uint32_t> jhi;
     if (!high_table(b->spec, lo, b->sector, b->D, jhi)) return false;
-    upload_tables(b, lo, jhi);
+    upload_tables(b, lo.nlo, lo.nhi, lo.jlo, jhi);
     return true;
 }
 
@@ -184,8 +185,31 @@ std::vector<bool> build_rank_tables_many(const std::vector<edm_basis *> &bs)
     for (auto &t : th) t.join();
     for (size_t k = 0; k < bs.size(); ++k)
         if (good[k]) {
-            upload_tables(bs[k], lo, jhi[k]);
+            upload_tables(bs[k], lo.nlo, lo.nhi, lo.jlo, jhi[k]);
             ok[k] = true;
         }
     return ok;
+}
+
+bool derive_rank_tables(edm_basis *b, const uint64_t *states)
+{
+    const int nbits = b->nbits;
+    const int64_t D = b->D;
+    if (nbits < 2 || nbits > 44 || D == 0 || D >= (int64_t)0xFFFFFFFFLL) return false;
+    const int nlo = nbits / 2, nhi = nbits - nlo;
+    const uint64_t lomask = (1ULL << nlo) - 1ULL;
+    constexpr uint32_t kAbsent = 0xFFFFFFFFu;
+    std::vector<uint32_t> jhi(1ULL << nhi, kAbsent), jlo(1ULL << nlo, kAbsent);
+    for (int64_t i = 0; i < D; ++i) {
+        const uint64_t h = states[i] >> nlo, l = states[i] & lomask;
+        if (jhi[h] == kAbsent) jhi[h] = (uint32_t)i;   // sorted: the block's first state
+        const uint32_t pos = (uint32_t)(i - jhi[h]);
+        if (jlo[l] == kAbsent) jlo[l] = pos;
+        else if (jlo[l] != pos) return false;         // blocks differ: no two-table rank
+    }
+    // entries no state uses are never read for a target inside the basis
+    for (uint32_t &v : jhi) if (v == kAbsent) v = 0;
+    for (uint32_t &v : jlo) if (v == kAbsent) v = 0;
+    upload_tables(b, nlo, nhi, jlo, jhi);
+    return true;
 }

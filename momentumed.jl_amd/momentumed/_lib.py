@@ -49,6 +49,7 @@ _SIGS = [
     ("edm_basis_download", c_i32, [c_vp, c_vp]),
     ("edm_basis_device_ptr", c_i32, [c_vp, P(c_vp)]),
     ("edm_basis_rank_mode", c_i32, [c_vp, P(c_i32)]),
+    ("edm_basis_drop_rank", c_i32, [c_vp]),
     ("edm_basis_destroy", c_i32, [c_vp]),
     ("edm_op_upload", c_i32, [c_vp, c_i32, c_vp, c_vp, c_vp, c_i64, c_i32, P(c_vp)]),
     ("edm_op_size", c_i32, [c_vp, P(c_i64)]),
@@ -76,6 +77,7 @@ _SIGS = [
     ("edm_batch_remove", c_i32, [c_vp, c_i64]),
     ("edm_batch_count", c_i32, [c_vp, P(c_i32), P(c_i32)]),
     ("edm_batch_set_quantum", c_i32, [c_vp, c_i64]),
+    ("edm_batch_set_weight", c_i32, [c_vp, c_i64, ctypes.c_double]),
     ("edm_batch_problem_bytes", c_i32, [c_i64, c_i32, c_i32, P(c_u64)]),
     ("edm_matvec_tune", c_i32, [c_vp, c_vp, c_vp, P(c_i32), c_vp]),
     ("edm_matvec_between", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

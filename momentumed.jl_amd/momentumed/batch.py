@@ -148,6 +148,10 @@ class BatchSolver:
         """Deficit-round-robin step pacing (0: every problem every step); see edm.h."""
         check(self.L.edm_batch_set_quantum(self.h, int(rows)), self.ctx.h)
 
+    def set_weight(self, pid: int, weight: float):
+        """Round-robin weight of one problem (credit weight * quantum rows per step)."""
+        check(self.L.edm_batch_set_weight(self.h, int(pid), float(weight)), self.ctx.h)
+
     def count(self) -> tuple[int, int]:
         t, r = ctypes.c_int32(), ctypes.c_int32()
         check(self.L.edm_batch_count(self.h, ctypes.byref(t), ctypes.byref(r)), self.ctx.h)

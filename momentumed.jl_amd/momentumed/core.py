@@ -523,7 +523,9 @@ class MBS64Vector:
     """EDCore/src/types/manybodystate_vector.jl:21-29: coefficients bound to a subspace."""
 
     def __init__(self, vec, space: HilbertSubspace):
-        vec = np.asarray(vec, dtype=np.complex128)
+        vec = np.asarray(vec)
+        if vec.dtype != np.complex64:   # ComplexF32 coefficients stay single precision
+            vec = vec.astype(np.complex128, copy=False)
         if vec.shape != (space.length,):
             raise AssertionError("vector length mismatches Hilbert space dimension.")
         self.vec = vec

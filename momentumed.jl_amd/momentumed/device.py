@@ -181,9 +181,16 @@ class DeviceBasis:
 
     @property
     def rank_mode(self) -> int:
+        """0 binary search, 1 rank tables of a complete sector of the given system, 2 rank
+        tables derived from the state list (used for an operator that keeps every target in
+        the basis)."""
         m = ctypes.c_int32()
         check(self.ctx.L.edm_basis_rank_mode(self.h, ctypes.byref(m)), self.ctx.h)
         return m.value
+
+    def drop_rank(self):
+        """Release the rank tables: partner indices by binary search from then on."""
+        check(self.ctx.L.edm_basis_drop_rank(self.h), self.ctx.h)
 
     @property
     def device_ptr(self) -> int:

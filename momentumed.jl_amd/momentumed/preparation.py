@@ -97,7 +97,7 @@ def MBS_totalmomentum(para: EDPara, mbs) -> tuple:
 
 
 def _momentum_list(para: EDPara, N_each) -> list[tuple]:
-    """momentum_decomposition.jl:360-369: product of per-dim ranges, first coordinate fastest."""
+    """momentum_decomposition.jl:209-219: product of per-dim ranges, first coordinate fastest."""
     ntot = int(sum(N_each))
     ranges = []
     for d in range(para.dim):
@@ -122,7 +122,7 @@ def _wanted_momenta(para: EDPara, N_each, momentum_restriction: bool, momentum_l
 
 def _host_momentum_subspaces(para: EDPara, N_each, momentum_restriction: bool, momentum_list,
                              mask, selection_rule):
-    """momentum_decomposition.jl:101-152,339-413 with a mask, holes or a selection rule (the
+    """momentum_decomposition.jl:101-152,188-262 with a mask, holes or a selection rule (the
     forms the reference's entanglement analysis uses), enumerated on the host: component c
     (the last one in the highest bits) holds |N_c| particles, or |N_c| holes when N_c < 0,
     on its orbitals (those of `mask` only, when given: 1-based orbital indices of the whole
@@ -172,7 +172,7 @@ def _host_momentum_subspaces(para: EDPara, N_each, momentum_restriction: bool, m
 def ED_momentum_subspaces(para: EDPara, N_each_component, *, momentum_restriction: bool = False,
                           momentum_list: list | None = None, device: str = "rocm",
                           mask=None, selection_rule=None):
-    """momentum_decomposition.jl:339-413: the Hilbert subspace of every momentum sector, in
+    """momentum_decomposition.jl:188-262: the Hilbert subspace of every momentum sector, in
     momentum_list order with empty sectors dropped. Returns (subspaces, subspace_k).
 
     The states are enumerated on the device; each returned HilbertSubspace keeps its device

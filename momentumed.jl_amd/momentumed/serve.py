@@ -113,23 +113,43 @@ class ServeReport:
     results: dict = field(default_factory=dict)       # rid -> eigenvalues
     peak_hbm_bytes: int = 0
     steps: int = 0
+    rejected_labels: list = field(default_factory=list)
+    # (label, seconds since arrival when the run ended) of requests still queued or running
+    unfinished_ages: list = field(default_factory=list)
 
     def summary(self) -> dict:
+        """Counts, throughput and latency percentiles, overall and per request class.
+        p50_s / p99_s are over completed requests; p50_all_s / p99_all_s count every
+        unfinished request too, at the time it had waited when the run ended (a lower bound
+        on its latency), so an over-capacity class cannot hide behind its completions.
+        Rejected requests (queue full on arrival, or larger than the budget) are counted
+        apart; they have no latency."""
         lat = np.array([s for _, s in self.latencies]) if self.latencies else np.zeros(0)
+        cens = np.array([s for _, s in self.unfinished_ages]) if self.unfinished_ages else \
+            np.zeros(0)
+        both = np.concatenate([lat, cens])
 
         def pct(a, q):
-            return float(np.percentile(a, q)) if a.size else None
+            return round(float(np.percentile(a, q)), 4) if a.size else None
         by = {}
-        for lab in sorted({lab for lab, _ in self.latencies}):
+        labels = {lab for lab, _ in self.latencies} | {lab for lab, _ in self.unfinished_ages} | \
+            set(self.rejected_labels)
+        for lab in sorted(labels):
             a = np.array([s for l2, s in self.latencies if l2 == lab])
-            by[lab] = {"n": int(a.size), "p50_s": round(pct(a, 50), 4), "p99_s": round(pct(a, 99), 4)}
+            u = np.array([s for l2, s in self.unfinished_ages if l2 == lab])
+            ab = np.concatenate([a, u])
+            by[lab] = {"completed": int(a.size), "unfinished": int(u.size),
+                       "rejected": sum(1 for l2 in self.rejected_labels if l2 == lab),
+                       "p50_s": pct(a, 50), "p99_s": pct(a, 99),
+                       "p50_all_s": pct(ab, 50), "p99_all_s": pct(ab, 99)}
         return {
             "mode": self.mode, "offered": self.offered, "admitted": self.admitted,
             "completed": self.completed, "rejected": self.rejected,
             "unfinished": self.unfinished, "elapsed_s": round(self.elapsed_s, 3),
             "solves_per_s": round(self.completed / self.elapsed_s, 3) if self.elapsed_s else None,
             "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
-            "max_s": float(lat.max()) if lat.size else None,
+            "p50_all_s": pct(both, 50), "p99_all_s": pct(both, 99),
+            "max_s": round(float(lat.max()), 4) if lat.size else None,
             "by_class": by, "peak_hbm_bytes": self.peak_hbm_bytes, "steps": self.steps,
         }
 
@@ -140,10 +160,24 @@ class SolveServer:
     def __init__(self, hbm_budget_bytes: int = 200 * 10 ** 9, max_queue: int = 256,
                  krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100, seed: int = 0x5EED,
                  mode: str = "batched", prepare_ahead: int = 64, device: int = 0,
-                 quantum_rows: int = 1 << 15):
+                 quantum_rows: int = 1 << 15, big_share: float = 0.5,
+                 head_wait_s: float = 10.0):
+        """quantum_rows: the step quantum of the batch's round robin (a problem of D rows
+        steps about once every D / quantum_rows steps). big_share: the share of the rows
+        stepped that the oldest running problem larger than the quantum receives (its
+        round-robin weight is set so), so a large request keeps moving however many small
+        ones run beside it (0 turns this off). head_wait_s: once the head of the queue has
+        waited this long, requests behind it are no longer admitted ahead of it, so the
+        budget frees up for it."""
         if mode not in ("batched", "plain"):
             raise ValueError("mode must be 'batched' or 'plain'")
+        if not 0.0 <= big_share < 1.0:
+            raise ValueError("big_share must be in [0, 1)")
         self.mode = mode
+        self.quantum = int(quantum_rows)
+        self.big_share = float(big_share)
+        self.head_wait_s = float(head_wait_s)
+        self._weights: dict = {}
         self.ctx = Context(device, hbm_budget_bytes)
         self.budget = int(hbm_budget_bytes)
         self.max_queue = max_queue
@@ -232,6 +266,7 @@ class SolveServer:
                 while i < len(reqs) and reqs[i].arrival <= now:
                     if len(waiting) >= self.max_queue:
                         rep.rejected += 1
+                        rep.rejected_labels.append(reqs[i].label)
                     else:
                         waiting.append(_Job(reqs[i]))
                     i += 1
@@ -241,8 +276,9 @@ class SolveServer:
                         queued_for_prep.add(job.req.rid)
                         prep_in.put(job)
                 if self.mode == "batched":
-                    self._admit_batched(waiting, running, rep, prep_lock)
+                    self._admit_batched(waiting, running, rep, prep_lock, now)
                     if running:
+                        self._rebalance(running)
                         _, nfin = self.batch.step(1)
                         rep.steps += 1
                         if nfin:
@@ -271,6 +307,8 @@ class SolveServer:
             worker.join()
         rep.elapsed_s = time.perf_counter() - t0
         rep.unfinished = len(waiting) + len(running)
+        for job in list(waiting) + list(running.values()):
+            rep.unfinished_ages.append((job.req.label, rep.elapsed_s - job.req.arrival))
         for pid, job in list(running.items()):
             self.batch.remove(pid)
             self._release(job)
@@ -280,9 +318,10 @@ class SolveServer:
         with lock:
             return job.arrays is not None
 
-    def _admit_batched(self, waiting, running, rep, lock):
+    def _admit_batched(self, waiting, running, rep, lock, now=0.0):
         """FIFO admission with backfilling: when the head of the queue does not fit the free
-        budget, prepared requests behind it that do fit (so are smaller) are admitted."""
+        budget, prepared requests behind it that do fit (so are smaller) are admitted, until
+        the head has waited head_wait_s; from then on nothing passes it."""
         used, _ = self.ctx.memory()
         free = self.budget - used
         k = 0
@@ -294,8 +333,11 @@ class SolveServer:
             if need > self.budget:
                 waiting.pop(k)                 # can never fit
                 rep.rejected += 1
+                rep.rejected_labels.append(job.req.label)
                 continue
             if need > free:
+                if k == 0 and now - job.req.arrival > self.head_wait_s:
+                    break                      # the head is owed the budget that frees up
                 k += 1
                 continue
             try:
@@ -314,9 +356,27 @@ class SolveServer:
             used, _ = self.ctx.memory()
             free = self.budget - used
 
+    def _rebalance(self, running):
+        """Round-robin weights: the oldest running problem larger than the quantum gets the
+        weight that gives it big_share of the rows stepped (the rows the others step per
+        step are sum(min(D, quantum))); every other problem weight 1."""
+        want = {}
+        if self.big_share > 0 and running:
+            big = next((p for p, j in running.items() if j.basis.D > self.quantum), None)
+            if big is not None:
+                rest = sum(min(j.basis.D, self.quantum) for p, j in running.items() if p != big)
+                f = self.big_share
+                want[big] = max(1.0, f * rest / ((1.0 - f) * self.quantum))
+        for pid in running:
+            w = want.get(pid, 1.0)
+            if self._weights.get(pid, 1.0) != w:
+                self.batch.set_weight(pid, w)
+                self._weights[pid] = w
+
     def _collect(self, running, rep, t0, on_result):
         for pid in self.batch.finished():
             job = running.pop(pid)
+            self._weights.pop(pid, None)
             r = self.batch.result(pid)
             t = time.perf_counter() - t0
             self._release(job)
@@ -348,18 +408,21 @@ class SolveServer:
 
 
 def solve_alone(req: Request, ctx: Context, krylovdim=30, tol=1e-12, maxiter=100,
-                seed=0x5EED) -> np.ndarray:
-    """The request solved by itself (the parity reference for the served results)."""
+                seed=0x5EED, want_vectors: bool = False):
+    """The request solved by itself (the parity reference for the served results): its
+    eigenvalues, or with want_vectors (eigenvalues, eigenvectors, basis states, info)."""
     sysd = system_struct(mesh(*req.mesh), req.mesh, 1, 1, (req.N,))
     sector = req.sector[0] + req.mesh[0] * req.sector[1]
     basis = DeviceBasis.enumerate(ctx, sysd, sector)
     amp, out, inn = request_operator(req).arrays()
     op = DeviceOperator(ctx, req.mesh[0] * req.mesh[1], amp, out, inn, True)
     try:
-        vals, _, _ = krylov_map_solve(ROCmLinearMap(op, basis), req.nev, want_vectors=False,
-                                      krylovdim=krylovdim, tol=tol, maxiter=maxiter, seed=seed)
+        vals, vecs, info = krylov_map_solve(ROCmLinearMap(op, basis), req.nev,
+                                            want_vectors=want_vectors, krylovdim=krylovdim,
+                                            tol=tol, maxiter=maxiter, seed=seed)
+        states = basis.download() if want_vectors else None
     finally:
         op.close()
         basis.close()
-    return vals
+    return (vals, vecs, states, info) if want_vectors else vals
 

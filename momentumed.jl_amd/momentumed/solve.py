@@ -33,12 +33,16 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
     eigensolver (the reference runs LAPACK on the host, MomentumED.jl:340-369). As in the
     reference, "sparse" switches to "dense" below min_sparse_dim and "dense" to "sparse"
     above max_dense_dim (default 200 for an MBOperator, 500 for scatter lists,
-    MomentumED.jl:221,379). KrylovKit keywords krylovdim, tol, maxiter, vec0 are honoured,
+    MomentumED.jl:221,379). element_type Float64 or Float32 (sparse and dense methods; the
+    map ignores it, as the reference's does): the device computes in complex FP64 and
+    Float32 results are rounded to Float32 / ComplexF32. KrylovKit keywords krylovdim, tol,
+    maxiter, vec0 are honoured,
     plus `seed` for the start vector and, for method="map", `blocksize` (block Lanczos with
     that many vectors per step; 1, the default, is the single-vector iteration KrylovKit
     runs)."""
     if N < 1:
         raise AssertionError("N should be at least 1.")
+    single = _single_precision(element_type)
     if method not in ("sparse", "dense", "map"):
         raise AssertionError(f"Unknown method: {method}. Use :sparse, :dense, :map.")
     if device not in _DEVICES:
@@ -113,7 +117,8 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
             print(f"  {t1 - t0:.6f} seconds (device matrix, {H_gpu.nnz} stored entries)")
             print(f"  {time.perf_counter() - t1:.6f} seconds (device dense eigensolver)")
         n = min(N, dim)
-        return np.asarray(vals[:n]), [MBS64Vector(vecs[i], subspace) for i in range(n)]
+        return _as_requested(np.asarray(vals[:n]), [vecs[i] for i in range(n)], subspace,
+                             single)
     if method == "sparse":
         H_gpu = dev.create_gpu_matrix(hamiltonian, subspace, "rocm")
         what = f"device matrix, {H_gpu.nnz} stored entries"
@@ -135,6 +140,26 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
         warnings.warn(f"Lanczos eigsolve stopped after {info['iterations']} restarts with "
                       f"{info['converged']} of {N} eigenpairs converged "
                       f"(max residual {info['max_residual']:.2e})")
-    energies = np.asarray(vals[:N])
-    vectors = [MBS64Vector(vecs[i], subspace) for i in range(N)]
-    return energies, vectors
+    if method == "map":
+        single = False   # the map ignores element_type (MomentumED.jl:389 builds LinearMap
+        #                  with its Float64 default)
+    return _as_requested(np.asarray(vals[:N]), vecs[:N], subspace, single)
+
+
+def _single_precision(element_type) -> bool:
+    """element_type as the reference accepts it (sparse_matrix.jl:72): Float64 or Float32
+    (also given as numpy's float64 / float32 or Python's float)."""
+    if element_type in (float, np.float64, "Float64"):
+        return False
+    if element_type in (np.float32, "Float32"):
+        return True
+    raise AssertionError(f"element_type={element_type}. Use element_type Float64, Float32.")
+
+
+def _as_requested(energies, vecs, subspace, single):
+    """The device computes in complex FP64; element_type=Float32 returns the results rounded
+    to Float32 / ComplexF32, the types the reference's single-precision matrix gives."""
+    if single:
+        energies = energies.astype(np.float32)
+        vecs = [np.asarray(v).astype(np.complex64) for v in vecs]
+    return energies, [MBS64Vector(v, subspace) for v in vecs]

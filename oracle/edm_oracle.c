@@ -228,7 +228,7 @@ ORC_EXPORT void orc_connectivity(int64_t D, const uint64_t *basis, int64_t S, co
  * system: dim, Nk, Nc_mix, Nc_conserve, k_list (dim x Nk column-major: k_list[d + dim*k]),
  *         Gk[dim], N_each[Nc_conserve] (>= 0).
  * Sectors are momentum_list = product of per-dim ranges with the first dim fastest
- * (momentum_decomposition.jl:360-369): 0..Gk-1, or [min*ΣN, max*ΣN] when Gk == 0.
+ * (momentum_decomposition.jl:209-219): 0..Gk-1, or [min*ΣN, max*ΣN] when Gk == 0.
  * counts[nsec] receives the per-sector sizes; when states != NULL, sector s's states are
  * written at states[offsets[s] ...] in ascending order (offsets = exclusive prefix of counts).
  * Returns the number of sectors (including empty ones), or -1 on bad input.

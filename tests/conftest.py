@@ -12,6 +12,9 @@ for p in (os.path.join(ROOT, "momentumed.jl_amd"), os.path.join(ROOT, "oracle"),
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs libedm.so kernels)")
     config.addinivalue_line("markers", "slow: large-size GPU checks")
+    # a solve that stops with unconverged pairs fails the test that ran it, unless the test
+    # expects it (pytest.warns)
+    config.addinivalue_line("filterwarnings", "error:Lanczos eigsolve stopped")
 
 
 @pytest.fixture(scope="session")

@@ -252,3 +252,67 @@ def test_served_results_equal_alone(gpu_ctx):
     assert rep.steps > 0
     for r in reqs:
         np.testing.assert_array_equal(rep.results[r.rid], S.solve_alone(r, gpu_ctx))
+
+
+@pytest.mark.timeout(900)
+def test_served_stream_with_large_requests(gpu_ctx, oracle_mod):
+    """A served stream holding 5x6 requests and a 6x6@1/3 request (34.8 M states) among small
+    ones, on the server's default scheduling: every request completes (the large one is
+    not starved by the small ones), each result is bitwise the request solved alone, and
+    the values are exact eigenvalues: small sectors against the oracle's matrix (dense or
+    ARPACK), the 5x6 and 6x6 ones through oracle-sampled residuals of their eigenvectors."""
+    from fractions import Fraction
+
+    import scipy.sparse as sps2
+    import scipy.sparse.linalg as spl
+    from momentumed import serve as S
+    print("[serve] mixed stream with 5x6 and 6x6 requests", flush=True)
+    small = S.served_stream(rate=60.0, duration=0.25, seed=21,
+                            meshes=((3, 5), (4, 5), (4, 6)))
+    n0 = len(small)
+    big = [S.Request(n0, 0.01, (6, 6), Fraction(1, 3), 12, (1, 2), 2.5),
+           S.Request(n0 + 1, 0.03, (5, 6), Fraction(1, 3), 10, (2, 3), 1.7),
+           S.Request(n0 + 2, 0.06, (5, 6), Fraction(2, 5), 12, (4, 1), 3.1)]
+    reqs = small + big
+    srv = S.SolveServer(hbm_budget_bytes=200 * 10 ** 9, max_queue=1000)
+    try:
+        rep = srv.run(reqs, drain_s=600.0)
+    finally:
+        srv.close()
+    summ = rep.summary()
+    print({k: summ[k] for k in ("completed", "unfinished", "elapsed_s", "steps")},
+          summ["by_class"], flush=True)
+    assert rep.completed == len(reqs) and rep.rejected == 0 and rep.unfinished == 0
+    for r in reqs:
+        want_vec = r.mesh in ((5, 6), (6, 6))
+        print(f"[serve] alone: request {r.rid} {r.label} sector {r.sector}", flush=True)
+        got = S.solve_alone(r, gpu_ctx, want_vectors=want_vec)
+        vals = got[0] if want_vec else got
+        np.testing.assert_array_equal(rep.results[r.rid], vals)
+        amp, out, inn = S.request_operator(r).arrays()
+        full = oracle_mod.full_list(oracle_mod.TermList(amp, out, inn), True)
+        if want_vec:
+            _, vecs, states, info = got
+            assert info["converged"] == r.nev
+            rng = np.random.default_rng(r.rid)
+            rows = rng.integers(0, len(states), 2048)
+            for i, v in enumerate(vecs):
+                yo = oracle_mod.matvec(states, full, v, rows=rows)
+                est = np.linalg.norm(yo - vals[i] * v[rows]) * np.sqrt(len(states) / len(rows))
+                assert est <= 1e-9, (r.label, i, est)
+            continue
+        lists, ks = oracle_mod.enumerate_sectors(S.mesh(*r.mesh), r.mesh, 1, 1, [r.N])
+        basis = lists[ks.index(tuple(r.sector))]
+        if len(basis) <= 1500:
+            ev = np.linalg.eigvalsh(oracle_mod.dense_matrix(basis, full))[: r.nev + 4]
+        else:
+            rp, cols, terms, negs = oracle_mod.connectivity(basis, full)
+            Hs = sps2.csr_matrix((full.amp[terms] * np.where(negs, -1.0, 1.0), cols, rp),
+                                 shape=(len(basis), len(basis)))
+            ev = np.sort(spl.eigsh(Hs, k=r.nev + 4, which="SA", tol=1e-14, ncv=60)[0])
+        # single-vector Lanczos may return one copy of a degenerate level: every value must
+        # be an exact eigenvalue, and the lowest must be the ground state
+        scale = np.abs(ev).max()
+        for e in vals:
+            assert np.abs(ev - e).min() <= 1e-10 * scale, (r.label, e, ev)
+        assert abs(vals[0] - ev[0]) <= 1e-10 * scale

@@ -149,8 +149,13 @@ def test_rank_tables_modes(gpu_ctx, oracle_mod):
     assert b.D == 173613 and b.rank_mode == 1
     full = b.download()
     assert dev.DeviceBasis.upload(gpu_ctx, full, 27, sysd, 0).rank_mode == 1
-    assert dev.DeviceBasis.upload(gpu_ctx, full, 27).rank_mode == 0
-    assert dev.DeviceBasis.upload(gpu_ctx, full[::2], 27, sysd, 0).rank_mode == 0
+    # without the system the tables are derived from the states (mode 2), also for a partial
+    # list that splits into equal blocks (every other state); drop_rank returns to search
+    b2 = dev.DeviceBasis.upload(gpu_ctx, full, 27)
+    assert b2.rank_mode == 2
+    assert dev.DeviceBasis.upload(gpu_ctx, full[::2], 27, sysd, 0).rank_mode == 2
+    b2.drop_rank()
+    assert b2.rank_mode == 0
 
 
 # ------------------------------------------------------------------ connectivity (a6-a8)
@@ -219,8 +224,12 @@ def test_matvec_matches_oracle(gpu_ctx, oracle_mod, name):
         lm = dev.ROCmLinearMap(dop, b).adjoint()
         ya = lm(x)
         assert np.abs(ya - yo).max() <= Y_TOL * np.abs(yo).max()
-        # binary-search partner lookup gives the identical result bit for bit
+        # rank tables derived from the state list (the reference's call sequence: a bare
+        # list, no system) and binary-search lookup give the identical result bit for bit
         b2 = dev.DeviceBasis.upload(gpu_ctx, basis, w.para.bits)
+        assert b2.rank_mode == 2
+        np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b2, x), y)
+        b2.drop_rank()
         assert b2.rank_mode == 0
         np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b2, x), y)
 
@@ -408,7 +417,9 @@ def test_matvec_partial_basis_drops_targets(gpu_ctx, oracle_mod):
     lists, _ = oracle_mod.enumerate_sectors(w.para.k_list, w.para.Gk, 1, 1, [5])
     part = lists[0][::3].copy()
     b = dev.DeviceBasis.upload(gpu_ctx, part, w.para.bits, _system(w.para, w.N_each), 0)
-    assert b.rank_mode == 0
+    # the every-third list splits into equal blocks, so it gets derived rank tables, but
+    # targets fall outside it: the closure check sends its lookups to binary search
+    assert b.rank_mode == 2
     x = _rand(len(part), 7)
     yo = oracle_mod.matvec(part, full, x)
     assert np.abs(_gpu_matvec(gpu_ctx, dop, b, x) - yo).max() <= Y_TOL * np.abs(yo).max()

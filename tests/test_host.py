@@ -522,7 +522,7 @@ def _brute_sectors(para, N_each, mask=None, rule=None):
 
 def test_momentum_subspaces_mask_holes_selection():
     """ED_momentum_subspaces with mask, holes (negative N) and selection_rule
-    (momentum_decomposition.jl:101-152,339-413) against brute force over every state: one
+    (momentum_decomposition.jl:101-152,188-262) against brute force over every state: one
     component with a mask, two holes, a selection rule; two conserved components with a
     particle and a hole component under a mask that spans both."""
     from momentumed.workloads import mesh

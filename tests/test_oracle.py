@@ -830,3 +830,34 @@ def test_pes_block_rdm_particle_hole(oracle_mod):
         assert np.linalg.eigvalsh(got).min() > -1e-14
         trace += float(np.trace(got).real)
     assert abs(trace - 6.0) <= 1e-13
+
+
+# ------------------------------------------------------------------ CPU Lanczos (baseline)
+def test_cpu_lanczos_against_lapack(oracle_mod):
+    """oracle/lanczos.py (the CPU solve the baseline times) on the small workload's sectors
+    with the oracle matvec: the lowest 8 eigenvalues to 1e-10 of LAPACK on the oracle's dense
+    matrix, residuals within tol, orthonormal vectors; a start vector inside a 3-dimensional
+    invariant subspace (a diagonal map) breaks the iteration down and it still finds the
+    four smallest entries."""
+    import lanczos
+    from momentumed import workloads as W
+    w = W.small()
+    amp, out, inn = w.operator().arrays()
+    full = oracle_mod.full_list(oracle_mod.TermList(amp, out, inn), True)
+    lists, _ = oracle_mod.enumerate_sectors(w.para.k_list, w.para.Gk, 1, 1, [5])
+    for basis in lists[:3]:
+        H = oracle_mod.dense_matrix(basis, full)
+        vals, vecs, info = lanczos.eigsolve(lambda x: oracle_mod.matvec(basis, full, x),
+                                            len(basis), 8, krylovdim=30, tol=1e-12)
+        ref = np.linalg.eigvalsh(H)[:8]
+        assert info["converged"] == 8
+        np.testing.assert_allclose(vals, ref, rtol=1e-10, atol=0)
+        for i in range(8):
+            assert np.linalg.norm(H @ vecs[i] - vals[i] * vecs[i]) <= 1e-10
+        G = vecs.conj() @ vecs.T
+        assert np.abs(G - np.eye(8)).max() < 1e-12
+    d = np.arange(1.0, 41.0)
+    v0 = np.zeros(40, complex)
+    v0[[5, 9, 20]] = 1.0
+    vals, _, info = lanczos.eigsolve(lambda x: d * x, 40, 4, krylovdim=10, v0=v0)
+    np.testing.assert_allclose(vals, [1, 2, 3, 4], atol=1e-12)
