@@ -349,8 +349,13 @@ def served_section(args):
            "hbm_budget_bytes": 200 * 10 ** 9, "max_queue": 256, "drain_s": args.serve_drain,
            "plain_stream": f"the first {args.serve_plain_seconds:g} s of the same stream "
                            f"({len(prefix)} requests), same budget and queue bound"}
+    out["policy"] = {"quantum_rows": 1 << 15, "big_share": args.serve_big_share,
+                     "head_wait_s": args.serve_head_wait}
     for mode, rq in (("batched", reqs), ("plain", prefix)):
-        srv = S.SolveServer(hbm_budget_bytes=200 * 10 ** 9, max_queue=256, mode=mode)
+        if mode == "plain" and args.serve_plain_seconds <= 0:
+            continue
+        srv = S.SolveServer(hbm_budget_bytes=200 * 10 ** 9, max_queue=256, mode=mode,
+                            big_share=args.serve_big_share, head_wait_s=args.serve_head_wait)
         try:
             rep = srv.run(rq, drain_s=args.serve_drain)
         finally:
@@ -361,7 +366,8 @@ def served_section(args):
             f"{summ['solves_per_s']}/s, p50 {summ['p50_s']}, p99 {summ['p99_s']}, "
             f"p99 with unfinished {summ['p99_all_s']}")
         out[mode] = summ
-    b, p = out["batched"]["solves_per_s"], out["plain"]["solves_per_s"]
+    b = out["batched"]["solves_per_s"]
+    p = out["plain"]["solves_per_s"] if "plain" in out else None
     out["speedup_vs_plain"] = round(b / p, 2) if b and p else None
     return out
 
@@ -387,6 +393,10 @@ def main():
     ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
                     help="the plain (one solve at a time) baseline serves this prefix")
     ap.add_argument("--serve-drain", type=float, default=30.0)
+    ap.add_argument("--serve-big-share", type=float, default=0.5,
+                    help="share of the stepped rows the oldest large running problem gets")
+    ap.add_argument("--serve-head-wait", type=float, default=60.0,
+                    help="seconds after which the queue's head is no longer passed")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the small and medium configuration lines")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -211,6 +211,17 @@ int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev
                  int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
                  const double *vec0, double *vals, double *vecs, edm_solve_info *info);
 
+/* krylov_map_solve(H, N; ishermitian=false) (linear_map.jl:179-191; MomentumEDCUDAExt.jl:
+ * 159-179 -> KrylovKit.eigsolve(..., :SR; ishermitian=false)): the nev eigenvalues of smallest
+ * real part of a general (non-Hermitian) operator's map, by Krylov-Schur (Arnoldi with full
+ * re-orthogonalisation and restarts that keep the leading Schur vectors of the projected
+ * matrix; Stewart 2001). vals[2*nev] complex (re, im), ascending real part; vecs (host,
+ * nev*D complex, unit-norm right eigenvectors) may be NULL. Convergence, arguments and
+ * returns as edm_eigsolve (absolute residual |b . y_i| <= tol). */
+int edm_eigsolve_general(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
+                         int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
+                         const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+
 /* The same eigenpairs by thick-restart block Lanczos with block size `block` (1..8): each
  * step applies the map to `block` vectors at once (edm_matvec_block) and orthonormalises
  * them against the whole basis (CGS2, column by column); the projected matrix is complex
@@ -287,6 +298,11 @@ int edm_csr_matvec(edm_ctx *ctx, const edm_csr *m, const void *x_dev, void *y_de
 int edm_csr_eigsolve(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylovdim, double tol,
                      int32_t maxiter, uint64_t seed, const double *vec0, double *vals,
                      double *vecs, edm_solve_info *info);
+/* krylov_matrix_solve(H_gpu, N; ishermitian=false) (sparse_matrix.jl:248-255): the general
+ * eigensolver of edm_eigsolve_general on the stored matrix. */
+int edm_csr_eigsolve_general(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylovdim,
+                             double tol, int32_t maxiter, uint64_t seed, const double *vec0,
+                             double *vals, double *vecs, edm_solve_info *info);
 /* Matrix(H) of the reference's :dense method (MomentumED.jl:340-369, which densifies the
  * sparse matrix and calls LAPACK on the host): the stored matrix written as a dense D x D
  * complex row-major array at a_dev (device, 16 * D * D bytes, entry (i, j) = <i|H|j>), for a
@@ -354,6 +370,12 @@ int edm_projected_eigh(int32_t n, const double *A, double *w, double *Z);
  * edm_eigsolve_block): A and Z hold n*n interleaved (re, im) pairs, A row-major, Z with
  * eigenvector i in Z[2*i*n .. 2*i*n + 2*n). */
 int edm_projected_heigh(int32_t n, const double *A, double *w, double *Z);
+
+/* Host complex Schur decomposition A = Z T Z^H of a general n x n matrix (A, T and Z hold
+ * n*n interleaved (re, im) pairs, row-major; column c of Z is Schur vector c; the diagonal
+ * of T ascending by real part, then imaginary part): the Rayleigh-Ritz step of
+ * edm_eigsolve_general, where KrylovKit uses LAPACK's Schur routines. No device work. */
+int edm_projected_schur(int32_t n, const double *A, double *T, double *Z);
 
 /* ----------------------------------------------------------------- device vector helpers */
 /* Fill D complex values at a device pointer: mode 0 = uniform [0,1) in re and im

@@ -38,6 +38,7 @@ void free_basis(edm_basis *b)
     if (b->d_jlo) ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
     if (b->d_jhi) ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
     if (b->d_jlo16) ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
+    if (b->d_start) ctx->free(b->d_start, (b->nbuckets + 1) * sizeof(uint32_t));
     delete b;
 }
 
@@ -59,6 +60,18 @@ void verify_or_drop(edm_basis *b)
         b->rank_mode = 1;
     else
         drop_tables(b);
+}
+
+// the binary search's bucket index for a basis whose lookups may search (no system rank
+// tables); an optimisation, so a budget that cannot hold it leaves the plain search
+void maybe_index(edm_basis *b)
+{
+    if (b->rank_mode == 1 && !b->rank_derived) return;
+    try {
+        build_search_index(b);
+    } catch (const EdmError &e) {
+        if (e.code != EDM_ENOMEM) throw;
+    }
 }
 
 void attach_rank(edm_basis *b)
@@ -196,6 +209,7 @@ int edm_basis_enumerate(edm_ctx *ctx, const edm_system *sys, int64_t sector, edm
         try {
             b->d_states = enum_sector_states(ctx, sp, sector, &b->D);
             attach_rank(b);
+            maybe_index(b);
         } catch (...) {
             free_basis(b);
             throw;
@@ -233,6 +247,7 @@ int edm_basis_enumerate_all(edm_ctx *ctx, const edm_system *sys, edm_basis **out
             for (size_t k = 0; k < bs.size(); ++k) {
                 bs[k]->rank_mode = 0;
                 if (built[k]) verify_or_drop(bs[k]);
+                maybe_index(bs[k]);
             }
         } catch (...) {
             for (int64_t q = 0; q < sp.nsec; ++q) {
@@ -295,6 +310,7 @@ int edm_basis_upload(edm_ctx *ctx, const uint64_t *states, int64_t D, int32_t nb
                     drop_tables(b);
                 }
             }
+            maybe_index(b);
         } catch (...) {
             free_basis(b);
             throw;
@@ -344,6 +360,7 @@ int edm_basis_drop_rank(edm_basis *b)
     return guarded(b->ctx, [&]() -> int {
         EDM_HIP_CHECK(hipStreamSynchronize(b->ctx->stream));
         drop_tables(b);
+        maybe_index(b);
         return EDM_OK;
     });
 }
@@ -574,6 +591,27 @@ int edm_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev
     });
 }
 
+int edm_eigsolve_general(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
+                         int32_t krylovdim, double tol, int32_t maxiter, uint64_t seed,
+                         const double *vec0, double *vals, double *vecs, edm_solve_info *info)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && vals, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(op->nbits == b->nbits, EDM_EDIM,
+                    "Dimension of Hamiltonian linear map mismatches vector length.");
+        int rc = run_eigsolve_general_mv(
+            ctx, b->D,
+            [&](const double2 *x, double2 *y) { launch_matvec(ctx, op, b, x, y, ApplyMode::Forward); },
+            nev, krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+        if (rc == EDM_ENOCONV) {
+            ctx->last_error = "Krylov method fails. Cannot find " + std::to_string(nev) +
+                              " eigenvectors within maxiter.";
+        }
+        return rc;
+    });
+}
+
 int edm_eigsolve_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t nev,
                        int32_t block, int32_t krylovdim, double tol, int32_t maxiter,
                        uint64_t seed, const double *vec0, double *vals, double *vecs,
@@ -662,6 +700,21 @@ int edm_csr_eigsolve(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylov
         EDM_REQUIRE(ctx && m && vals, EDM_EINVAL, "NULL argument");
         EDM_REQUIRE(m->ctx == ctx, EDM_EINVAL, "matrix belongs to another context");
         int rc = run_eigsolve_mv(
+            ctx, m->D, [&](const double2 *x, double2 *y) { csr_matvec(ctx, m, x, y); }, nev,
+            krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
+        if (rc == EDM_ENOCONV) ctx->last_error = "Krylov method fails. Cannot find N eigenvectors.";
+        return rc;
+    });
+}
+
+int edm_csr_eigsolve_general(edm_ctx *ctx, const edm_csr *m, int32_t nev, int32_t krylovdim,
+                             double tol, int32_t maxiter, uint64_t seed, const double *vec0,
+                             double *vals, double *vecs, edm_solve_info *info)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && m && vals, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(m->ctx == ctx, EDM_EINVAL, "matrix belongs to another context");
+        int rc = run_eigsolve_general_mv(
             ctx, m->D, [&](const double2 *x, double2 *y) { csr_matvec(ctx, m, x, y); }, nev,
             krylovdim, tol, maxiter, seed, vec0, vals, vecs, info);
         if (rc == EDM_ENOCONV) ctx->last_error = "Krylov method fails. Cannot find N eigenvectors.";
@@ -925,6 +978,14 @@ int edm_projected_heigh(int32_t n, const double *A, double *w, double *Z)
             Z[2 * q + 1] = zz[q].imag();
         }
         return EDM_OK;
+    });
+}
+
+int edm_projected_schur(int32_t n, const double *A, double *T, double *Z)
+{
+    return guarded(nullptr, [&]() -> int {
+        EDM_REQUIRE(n >= 1 && A && T && Z, EDM_EINVAL, "bad argument");
+        return projected_schur(n, A, T, Z);
     });
 }
 

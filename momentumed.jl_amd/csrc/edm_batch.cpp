@@ -317,6 +317,9 @@ struct edm_batch {
     Plan plan;
     std::vector<unsigned char> mvdesc;
     std::vector<int> mvblk, tile2prob, blk2prob;
+    // shared-walk matvec descriptors and block tables by group size (index 2..max group)
+    std::vector<unsigned char> gdesc[8];
+    std::vector<int> gblk[8];
     std::vector<size_t> coef_off, vtab_off;
     std::vector<BatchVecDesc> vd;
     int64_t steps = 0;
@@ -661,8 +664,58 @@ static int batch_step_once(edm_batch *bt)
     mvblk.clear();
     std::vector<edm_problem *> single_mv;
     size_t lds = 16;
-    for (int q = 0; q < n; ++q) {
-        edm_problem *p = act[q];
+    // Problems on one basis whose operators share the term structure (the sweep's parameter
+    // points of one sector) go through one row walk, up to batch_mv_max_group() at a time;
+    // every other problem gets its own descriptor.
+    const int maxg = std::min(batch_mv_max_group(), 7);
+    const size_t gsz = batch_mv_group_desc_size();
+    for (int g = 2; g <= maxg; ++g) {
+        bt->gdesc[g].clear();
+        bt->gblk[g].clear();
+    }
+    std::vector<edm_problem *> solo;
+    {
+        std::map<std::pair<uint64_t, uint64_t>, std::vector<edm_problem *>> byshape;
+        for (edm_problem *p : act) {
+            if (p->op->fast && p->op->cfwd.ok)
+                byshape[{p->b->id, p->op->cfwd.shape_hash}].push_back(p);
+            else
+                solo.push_back(p);
+        }
+        const edm_op *ops[8];
+        const double2 *xs[8];
+        double2 *ys[8];
+        for (auto &kv : byshape) {
+            std::vector<edm_problem *> &v = kv.second;
+            for (size_t k = 0; k < v.size();) {
+                const int G = (int)std::min<size_t>((size_t)maxg, v.size() - k);
+                if (G == 1) {
+                    solo.push_back(v[k++]);
+                    continue;
+                }
+                for (int g = 0; g < G; ++g) {
+                    ops[g] = v[k + g]->op;
+                    xs[g] = v[k + g]->V[v[k + g]->j];
+                    ys[g] = v[k + g]->V[v[k + g]->j + 1];
+                }
+                std::vector<unsigned char> &dv = bt->gdesc[G];
+                std::vector<int> &bv = bt->gblk[G];
+                const size_t at = dv.size();
+                dv.resize(at + gsz);
+                size_t l = 0;
+                if (batch_mv_group_desc(ops, v[k]->b, xs, ys, G, (int64_t)bv.size(),
+                                        dv.data() + at, &l)) {
+                    bv.insert(bv.end(), (size_t)batch_mv_blocks(v[k]->D), (int)(at / gsz));
+                    lds = std::max(lds, l);
+                } else {
+                    dv.resize(at);
+                    for (int g = 0; g < G; ++g) solo.push_back(v[k + g]);
+                }
+                k += (size_t)G;
+            }
+        }
+    }
+    for (edm_problem *p : solo) {
         size_t l = 0;
         const size_t at = mvdesc.size();
         mvdesc.resize(at + mvsz);
@@ -679,6 +732,11 @@ static int batch_step_once(edm_batch *bt)
     }
     size_t mv_off = pl.put(mvdesc.data(), mvdesc.size());
     size_t mvblk_off = pl.put(mvblk.data(), mvblk.size() * sizeof(int));
+    size_t gdesc_off[8] = {0}, gblk_off[8] = {0};
+    for (int g = 2; g <= maxg; ++g) {
+        gdesc_off[g] = pl.put(bt->gdesc[g].data(), bt->gdesc[g].size());
+        gblk_off[g] = pl.put(bt->gblk[g].data(), bt->gblk[g].size() * sizeof(int));
+    }
     std::vector<size_t> &vtab_off = bt->vtab_off;
     vtab_off.resize(n);
     for (int q = 0; q < n; ++q)
@@ -729,6 +787,10 @@ static int batch_step_once(edm_batch *bt)
             launch_matvec(ctx, p->op, p->b, p->V[p->j], p->V[p->j + 1], ApplyMode::Forward);
         launch_batch_mv(ctx, dplan_base + mv_off, (const int *)(dplan_base + mvblk_off),
                         (int64_t)mvblk.size(), lds);
+        for (int g = 2; g <= maxg; ++g)
+            launch_batch_mv_group(ctx, g, dplan_base + gdesc_off[g],
+                                  (const int *)(dplan_base + gblk_off[g]),
+                                  (int64_t)bt->gblk[g].size(), lds);
         const BatchVecDesc *dvd = (const BatchVecDesc *)(dplan_base + vd_off);
         bblas_cgs2_norm(ctx, dvd, n, maxvec, (const int *)(dplan_base + t2p_off),
                         (int64_t)tile2prob.size(), (const int *)(dplan_base + b2p_off),

@@ -106,6 +106,11 @@ struct edm_basis {
     // themselves; an operator uses them only after a device check that every target of every
     // state is in the basis (op_closed)
     bool rank_derived = false;
+    // bucket index of the binary search: start[h] = first state whose bits above start_shift
+    // are >= h, for h < nbuckets, and start[nbuckets] = D (build_search_index)
+    uint32_t *d_start = nullptr;
+    int start_shift = 0;
+    uint64_t nbuckets = 0;
 };
 
 // ------------------------------------------------------------------ operator
@@ -141,6 +146,10 @@ struct CompactTable {
     // amplitudes (bit 15: use the complex conjugate), so the amplitudes can be staged in LDS
     // too; lds_full = bytes of image + tail (0 when the tail does not fit in LDS)
     size_t off_ampidx = 0, off_amps = 0, lds_full = 0;
+    // host copy of the table's structure (bucket offsets, then codes) and its hash: two
+    // operators whose tables match here differ only in amplitudes
+    std::vector<uint16_t> shape;
+    uint64_t shape_hash = 0;
 };
 
 struct edm_op {
@@ -239,6 +248,9 @@ edm_op *build_op(edm_ctx *ctx, int nbits, const double *amp, const uint64_t *out
                  const uint64_t *in, int64_t S, bool upper);
 void destroy_op(edm_op *op);
 bool verify_rank_tables(edm_basis *b);
+// the bucket index of the binary search (a no-op for small bases or when it exists); every
+// basis gets one when it is created (edm_matvec.hip)
+void build_search_index(edm_basis *b);
 int run_eigsolve(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nev, int krylovdim,
                  double tol, int maxiter, uint64_t seed, const double *vec0, double *vals,
                  double *vecs, edm_solve_info *info);
@@ -248,6 +260,15 @@ int run_eigsolve_mv(edm_ctx *ctx, int64_t D, const std::function<void(const doub
                     const double *vec0, double *vals, double *vecs, edm_solve_info *info);
 // block Lanczos (edm_lanczos.cpp): mv(nv, x, y) queues y_v = A x_v for v < nv
 using BlockMatvec = std::function<void(int, const double2 *const *, double2 *const *)>;
+// Krylov-Schur for the nev eigenvalues of smallest real part of a general map; vals holds
+// 2 * nev doubles (complex, ascending real part) (edm_lanczos.cpp)
+int run_eigsolve_general_mv(edm_ctx *ctx, int64_t D,
+                            const std::function<void(const double2 *, double2 *)> &mv, int nev,
+                            int krylovdim, double tol, int maxiter, uint64_t seed,
+                            const double *vec0, double *vals, double *vecs, edm_solve_info *info);
+// complex Schur form A = Z T Z^H, diagonal ascending by real part (interleaved complex,
+// row-major n x n; Z column c = Schur vector c)
+int projected_schur(int n, const double *A, double *T, double *Z);
 int run_eigsolve_block(edm_ctx *ctx, int64_t D, int bs, const BlockMatvec &mv, int nev,
                        int krylovdim, double tol, int maxiter, uint64_t seed, const double *vec0,
                        double *vals, double *vecs, edm_solve_info *info);
@@ -308,6 +329,16 @@ bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, doubl
                    int64_t blk0, void *out, size_t *lds_bytes);
 void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, int64_t nblocks,
                      size_t lds_bytes);
+// the shared-walk form for G (2..batch_mv_max_group()) problems on one basis whose operators
+// have the same term structure (batch_mv_same_shape): one descriptor covers all G, x[g] and
+// y[g] per problem; false when the group cannot share (then run them one by one)
+int batch_mv_max_group();
+size_t batch_mv_group_desc_size();
+bool batch_mv_same_shape(const edm_op *a, const edm_op *b);
+bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const double2 *const *x,
+                         double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes);
+void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,
+                           int64_t nblocks, size_t lds_bytes);
 // multi-problem Lanczos engine (edm_batch.cpp)
 edm_batch *batch_create(edm_ctx *ctx);
 void batch_destroy(edm_batch *bt);

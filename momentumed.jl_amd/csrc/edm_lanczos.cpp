@@ -20,6 +20,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <limits>
 #include <numeric>
 
 namespace {
@@ -907,4 +908,362 @@ int run_eigsolve_block_op(edm_ctx *ctx, const edm_op *op, const edm_basis *b, in
     EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     ctx->free(xi, xb);
     return rc;
+}
+
+// ---------------------------------------------------------------- general (non-Hermitian)
+// Complex Schur form of a general n x n matrix and the Krylov-Schur iteration built on it,
+// for KrylovKit.eigsolve(A, x0, howmany, :SR; ishermitian=false) as krylov_map_solve and
+// krylov_matrix_solve call it (MomentumED/src/method/linear_map.jl:179-191,
+// sparse_matrix.jl:248-255). The Schur form: Householder reduction to upper Hessenberg form,
+// then implicit single-shift QR sweeps with Wilkinson shifts (an exceptional shift after
+// every ten sweeps without deflation) until every subdiagonal entry is negligible (Golub &
+// Van Loan, 4th ed., Sec. 7.4-7.5, in complex arithmetic); the diagonal is then ordered by
+// ascending real part (then imaginary part) by swapping neighbouring diagonal entries with
+// one plane rotation each (Sec. 7.6.2). The iteration: Arnoldi with full re-orthogonalisation
+// (CGS2) to krylovdim vectors, then a restart that keeps the leading Schur vectors of the
+// projected matrix (Stewart, SIAM J. Matrix Anal. Appl. 23, 2001), which keeps the Arnoldi
+// relation A V_k = V_k T_k + v b^H exact without implicit shifts.
+namespace {
+
+using cplx = std::complex<double>;
+
+// G = [[c, s], [-conj(s), c]], c real, with G [f; g] = [r; 0]
+void plane_rotation(cplx f, cplx g, double &c, cplx &s)
+{
+    const double af = std::abs(f), ag = std::abs(g);
+    if (ag == 0.0) { c = 1.0; s = 0.0; return; }
+    if (af == 0.0) { c = 0.0; s = std::conj(g) / ag; return; }
+    const double nu = std::hypot(af, ag);
+    c = af / nu;
+    s = (f / af) * std::conj(g) / nu;
+}
+
+struct Mat {
+    int n;
+    std::vector<cplx> a;   // row-major
+    cplx &operator()(int i, int j) { return a[(size_t)i * n + j]; }
+};
+
+// rows k, k+1 of M <- G [rows], columns [j0, j1)
+void rot_rows(Mat &M, int k, double c, cplx s, int j0, int j1)
+{
+    for (int j = j0; j < j1; ++j) {
+        const cplx x = M(k, j), y = M(k + 1, j);
+        M(k, j) = c * x + s * y;
+        M(k + 1, j) = -std::conj(s) * x + c * y;
+    }
+}
+// columns k, k+1 of M <- [columns] G^H, rows [i0, i1)
+void rot_cols(Mat &M, int k, double c, cplx s, int i0, int i1)
+{
+    for (int i = i0; i < i1; ++i) {
+        const cplx x = M(i, k), y = M(i, k + 1);
+        M(i, k) = c * x + std::conj(s) * y;
+        M(i, k + 1) = -s * x + c * y;
+    }
+}
+
+bool schur_less(cplx x, cplx y)
+{
+    return x.real() < y.real() || (x.real() == y.real() && x.imag() < y.imag());
+}
+
+// A = Z T Z^H: on return A holds T (upper triangular, diagonal ascending by real part) and Z
+// the unitary Schur basis (column c = Schur vector c). False when the QR sweeps do not
+// converge.
+bool complex_schur(Mat &A, Mat &Z)
+{
+    const int n = A.n;
+    Z.n = n;
+    Z.a.assign((size_t)n * n, cplx(0.0, 0.0));
+    for (int i = 0; i < n; ++i) Z(i, i) = 1.0;
+    double anorm = 0.0;
+    for (const cplx &x : A.a) anorm = std::max(anorm, std::abs(x));
+    if (anorm == 0.0) return true;
+    // work at unit scale (a power of two, exact), so squared norms neither under- nor overflow
+    const int ex = scale_exponent(anorm);
+    const double down = std::ldexp(1.0, -ex), up = std::ldexp(1.0, ex);
+    for (cplx &x : A.a) x *= down;
+    anorm *= down;
+    struct Rescale {
+        Mat &A; double up;
+        ~Rescale() { for (cplx &x : A.a) x *= up; }
+    } rescale{A, up};
+    // Householder reduction to upper Hessenberg form
+    std::vector<cplx> v(n), w(n);
+    for (int k = 0; k + 2 < n; ++k) {
+        double xn = 0.0, tail = 0.0;
+        for (int i = k + 1; i < n; ++i) xn += std::norm(A(i, k));
+        for (int i = k + 2; i < n; ++i) tail += std::norm(A(i, k));
+        if (tail == 0.0) continue;
+        xn = std::sqrt(xn);
+        const cplx x0 = A(k + 1, k);
+        const double ax0 = std::abs(x0);
+        const cplx alpha = -(ax0 > 0.0 ? x0 / ax0 : cplx(1.0, 0.0)) * xn;
+        for (int i = k + 1; i < n; ++i) v[i] = A(i, k);
+        v[k + 1] -= alpha;
+        double vv = 0.0;
+        for (int i = k + 1; i < n; ++i) vv += std::norm(v[i]);
+        const double beta = 2.0 / vv;
+        // A <- H A on rows k+1.., columns k..: A -= beta v (v^H A)
+        for (int j = k; j < n; ++j) {
+            cplx s = 0.0;
+            for (int i = k + 1; i < n; ++i) s += std::conj(v[i]) * A(i, j);
+            s *= beta;
+            for (int i = k + 1; i < n; ++i) A(i, j) -= v[i] * s;
+        }
+        // A <- A H and Z <- Z H on columns k+1..: M -= beta (M v) v^H
+        for (Mat *M : {&A, &Z})
+            for (int i = 0; i < n; ++i) {
+                cplx s = 0.0;
+                for (int j = k + 1; j < n; ++j) s += (*M)(i, j) * v[j];
+                s *= beta;
+                for (int j = k + 1; j < n; ++j) (*M)(i, j) -= s * std::conj(v[j]);
+            }
+        A(k + 1, k) = alpha;
+        for (int i = k + 2; i < n; ++i) A(i, k) = 0.0;
+    }
+    // implicit single-shift QR sweeps on the active block [l, hi]
+    const double eps = std::numeric_limits<double>::epsilon();
+    int hi = n - 1, its = 0, total = 0;
+    while (hi > 0) {
+        int l = hi;
+        for (; l > 0; --l) {
+            double sc = std::abs(A(l - 1, l - 1)) + std::abs(A(l, l));
+            if (sc == 0.0) sc = anorm;
+            if (std::abs(A(l, l - 1)) <= eps * sc) {
+                A(l, l - 1) = 0.0;
+                break;
+            }
+        }
+        if (l == hi) {
+            --hi;
+            its = 0;
+            continue;
+        }
+        if (++total > 60 * n) return false;
+        cplx mu;
+        if (its > 0 && its % 10 == 0) {
+            mu = A(hi, hi) + 0.75 * std::abs(A(hi, hi - 1));   // exceptional shift
+        } else {
+            const cplx a = A(hi - 1, hi - 1), b = A(hi - 1, hi), c = A(hi, hi - 1), d = A(hi, hi);
+            const cplx half = 0.5 * (a - d);
+            const cplx disc = std::sqrt(half * half + b * c);
+            const cplx m1 = 0.5 * (a + d) + disc, m2 = 0.5 * (a + d) - disc;
+            mu = std::abs(m1 - d) < std::abs(m2 - d) ? m1 : m2;
+        }
+        ++its;
+        for (int k = l; k < hi; ++k) {
+            double c;
+            cplx s;
+            if (k == l) plane_rotation(A(l, l) - mu, A(l + 1, l), c, s);
+            else plane_rotation(A(k, k - 1), A(k + 1, k - 1), c, s);
+            rot_rows(A, k, c, s, k == l ? l : k - 1, n);
+            rot_cols(A, k, c, s, 0, std::min(k + 3, hi + 1));
+            rot_cols(Z, k, c, s, 0, n);
+            if (k > l) A(k + 1, k - 1) = 0.0;
+        }
+    }
+    for (int i = 1; i < n; ++i)
+        for (int j = 0; j < i; ++j) A(i, j) = 0.0;
+    // order the diagonal: move the smallest remaining entry to position p by neighbour swaps
+    for (int p = 0; p < n; ++p) {
+        int q = p;
+        for (int r = p + 1; r < n; ++r)
+            if (schur_less(A(r, r), A(q, q))) q = r;
+        for (int k = q - 1; k >= p; --k) {
+            const cplx t11 = A(k, k), t22 = A(k + 1, k + 1);
+            double c;
+            cplx s;
+            plane_rotation(A(k, k + 1), t22 - t11, c, s);
+            rot_rows(A, k, c, s, k, n);
+            rot_cols(A, k, c, s, 0, k + 2);
+            rot_cols(Z, k, c, s, 0, n);
+            A(k + 1, k) = 0.0;
+            A(k, k) = t22;
+            A(k + 1, k + 1) = t11;
+        }
+    }
+    return true;
+}
+
+// eigenvector y of the upper triangular T for its i-th diagonal entry (y[j] = 0 for j > i),
+// by back substitution; unit norm
+std::vector<cplx> tri_eigvec(Mat &T, int i)
+{
+    const int n = T.n;
+    double tnorm = 0.0;
+    for (const cplx &x : T.a) tnorm = std::max(tnorm, std::abs(x));
+    const double small = std::max(tnorm, 1e-300) * std::numeric_limits<double>::epsilon();
+    std::vector<cplx> y(n, cplx(0.0, 0.0));
+    y[i] = 1.0;
+    const cplx lam = T(i, i);
+    for (int j = i - 1; j >= 0; --j) {
+        cplx s = 0.0;
+        for (int l = j + 1; l <= i; ++l) s += T(j, l) * y[l];
+        cplx den = T(j, j) - lam;
+        if (std::abs(den) < small) den = small;
+        y[j] = -s / den;
+    }
+    double nrm = 0.0;
+    for (const cplx &x : y) nrm += std::norm(x);
+    nrm = std::sqrt(nrm);
+    for (cplx &x : y) x /= nrm;
+    return y;
+}
+
+}  // namespace
+
+int projected_schur(int n, const double *A, double *T, double *Z)
+{
+    Mat a{n, std::vector<cplx>((size_t)n * n)}, z{n, {}};
+    for (size_t q = 0; q < a.a.size(); ++q) a.a[q] = cplx(A[2 * q], A[2 * q + 1]);
+    EDM_REQUIRE(complex_schur(a, z), EDM_ENOCONV, "Schur decomposition did not converge");
+    for (size_t q = 0; q < a.a.size(); ++q) {
+        T[2 * q] = a.a[q].real();
+        T[2 * q + 1] = a.a[q].imag();
+        Z[2 * q] = z.a[q].real();
+        Z[2 * q + 1] = z.a[q].imag();
+    }
+    return EDM_OK;
+}
+
+int run_eigsolve_general_mv(edm_ctx *ctx, int64_t D,
+                            const std::function<void(const double2 *, double2 *)> &mv, int nev,
+                            int krylovdim, double tol, int maxiter, uint64_t seed,
+                            const double *vec0, double *vals, double *vecs, edm_solve_info *info)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    EDM_REQUIRE(D > 0, EDM_EINVAL, "empty Hilbert subspace");
+    EDM_REQUIRE(nev >= 1, EDM_EINVAL, "N should be at least 1.");
+    EDM_REQUIRE(nev <= D, EDM_EINVAL, "more eigenpairs requested than the subspace dimension");
+    EDM_REQUIRE(tol > 0, EDM_EINVAL, "tol must be positive");
+    EDM_REQUIRE(maxiter >= 1, EDM_EINVAL, "maxiter must be at least 1");
+    int m = std::max(krylovdim, nev + 2);
+    if ((int64_t)m > D) m = (int)D;
+    EDM_REQUIRE(m <= 255, EDM_EINVAL, "krylovdim must be at most 255");
+    Workspace ws(ctx, D, m, m);
+    {
+        int64_t nb = (D + 1023) / 1024;
+        ctx->partials((size_t)(nb + 1) * (m + 1) * sizeof(double2));
+    }
+    const size_t ybytes = (size_t)m * m * sizeof(double2);
+    double2 *d_Yc = (double2 *)ctx->alloc(ybytes);
+    struct YGuard {
+        edm_ctx *c; double2 *p; size_t b;
+        ~YGuard() { (void)hipStreamSynchronize(c->stream); c->free(p, b); }
+    } yguard{ctx, d_Yc, ybytes};
+    hipStream_t st = ctx->stream;
+    int matvecs = 0;
+    uint64_t rng_stream = 0;
+    if (vec0) EDM_HIP_CHECK(hipMemcpyAsync(ws.V[0], vec0, D * sizeof(double2), hipMemcpyHostToDevice, st));
+    else blas_fill_random(ctx, ws.V[0], D, seed, 0);
+    const double b0 = norm_host(ws, ws.V[0]);
+    EDM_REQUIRE(b0 > 0 && std::isfinite(b0), EDM_EINVAL, "start vector is zero or not finite");
+    blas_scale_inv(ctx, ws.V[0], ws.V[0], ws.d_beta, D);
+
+    Mat H{m, std::vector<cplx>((size_t)m * m, cplx(0.0, 0.0))};   // the projected matrix
+    Mat T{m, {}}, Z{m, {}};
+    std::vector<cplx> bres(m);    // residual row: A V Z = V Z T + v_m bres^T
+    std::vector<double> res(m, 0.0);
+    std::vector<double2> h;
+    std::vector<double2> yc((size_t)m * m);
+    int k = 0, iters = 0, nconv = 0;
+    double beta_last = 0.0, hnorm = 0.0;
+    for (iters = 1; iters <= maxiter; ++iters) {
+        for (int j = k; j < m; ++j) {
+            mv(ws.V[j], ws.V[j + 1]);
+            queue_cgs2_norm_scale(ws, j + 1, ws.V[j + 1], 0);
+            ++matvecs;
+            double beta = 0.0;
+            collect_cgs2_norm_scale(ws, j + 1, 0, h, &beta);
+            double col = beta;
+            for (int r = 0; r <= j; ++r) {
+                H(r, j) = cplx(h[r].x, h[r].y);
+                col += std::abs(H(r, j));
+            }
+            hnorm = std::max(hnorm, col);
+            if (!(beta > 1e-14 * std::max(hnorm, 1e-300))) {
+                // invariant subspace: a fresh random direction orthogonal to the basis
+                beta = 0.0;
+                for (int attempt = 0; attempt < 4; ++attempt) {
+                    blas_fill_random(ctx, ws.V[j + 1], D, seed ^ (0xC3C3C3C3ULL + ++rng_stream), 0);
+                    std::vector<double2> hh;
+                    cgs2(ws, j + 1, ws.V[j + 1], hh);
+                    if (norm_host(ws, ws.V[j + 1]) > 1e-8) break;
+                }
+                blas_scale_inv(ctx, ws.V[j + 1], ws.V[j + 1], ws.d_beta, D);
+            }
+            if (j + 1 < m) H(j + 1, j) = beta;
+            beta_last = beta;
+        }
+        T = H;
+        EDM_REQUIRE(complex_schur(T, Z), EDM_ENOCONV,
+                    "Schur decomposition of the projected matrix did not converge");
+        for (int c = 0; c < m; ++c) bres[c] = beta_last * Z(m - 1, c);
+        // residual of Ritz pair i: |bres . y_i| for the unit eigenvector y_i of T
+        nconv = 0;
+        for (int i = 0; i < m; ++i) {
+            const std::vector<cplx> y = tri_eigvec(T, i);
+            cplx s = 0.0;
+            for (int c = 0; c <= i; ++c) s += bres[c] * y[c];
+            res[i] = std::abs(s);
+        }
+        while (nconv < nev && res[nconv] <= tol) ++nconv;
+        if (nconv >= nev || iters == maxiter || (int64_t)m == D) break;
+        // Krylov-Schur restart: keep the leading Schur vectors
+        int keep = (3 * m + 2 * nconv) / 5;
+        keep = std::max(keep, nev);
+        keep = std::min(keep, m - 1);
+        for (int c = 0; c < keep; ++c)
+            for (int r = 0; r < m; ++r) yc[(size_t)c * m + r] = make_double2(Z(r, c).real(), Z(r, c).imag());
+        EDM_HIP_CHECK(hipMemcpyAsync(d_Yc, yc.data(), (size_t)m * keep * sizeof(double2),
+                                     hipMemcpyHostToDevice, st));
+        blas_rotate_c(ctx, (const double2 *const *)ws.d_vptrs, m, d_Yc, m, keep,
+                      (double2 *const *)ws.d_tmpptrs, D);
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int c = 0; c < keep; ++c) std::swap(ws.V[c], ws.tmp[c]);
+        std::swap(ws.V[keep], ws.V[m]);
+        ws.upload_ptrs();
+        std::fill(H.a.begin(), H.a.end(), cplx(0.0, 0.0));
+        for (int r = 0; r < keep; ++r)
+            for (int c = r; c < keep; ++c) H(r, c) = T(r, c);
+        for (int c = 0; c < keep; ++c) H(keep, c) = bres[c];
+        k = keep;
+    }
+    if (iters > maxiter) iters = maxiter;
+    double maxres = 0.0;
+    for (int i = 0; i < nev; ++i) {
+        vals[2 * i] = T(i, i).real();
+        vals[2 * i + 1] = T(i, i).imag();
+        maxres = std::max(maxres, res[i]);
+    }
+    if (vecs) {
+        // Ritz vectors V Z y_i
+        for (int i = 0; i < nev; ++i) {
+            const std::vector<cplx> y = tri_eigvec(T, i);
+            for (int r = 0; r < m; ++r) {
+                cplx s = 0.0;
+                for (int c = 0; c <= i; ++c) s += Z(r, c) * y[c];
+                yc[(size_t)i * m + r] = make_double2(s.real(), s.imag());
+            }
+        }
+        EDM_HIP_CHECK(hipMemcpyAsync(d_Yc, yc.data(), (size_t)m * nev * sizeof(double2),
+                                     hipMemcpyHostToDevice, st));
+        blas_rotate_c(ctx, (const double2 *const *)ws.d_vptrs, m, d_Yc, m, nev,
+                      (double2 *const *)ws.d_tmpptrs, D);
+        for (int i = 0; i < nev; ++i)
+            EDM_HIP_CHECK(hipMemcpyAsync(vecs + 2 * (size_t)i * D, ws.tmp[i], D * sizeof(double2),
+                                         hipMemcpyDeviceToHost, st));
+        EDM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+    if (info) {
+        info->iterations = iters;
+        info->matvecs = matvecs;
+        info->converged = nconv;
+        info->krylovdim = m;
+        info->max_residual = maxres;
+        info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return nconv >= nev ? EDM_OK : EDM_ENOCONV;
 }

@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <unordered_map>
 #include <utility>
@@ -60,19 +61,54 @@ struct RankLin {
     }
 };
 
+// searchsortedfirst over the sorted states, returning -1 for a state outside the basis. With
+// a bucket index (start != null) the search is confined to the states whose bits above
+// `shift` equal the target's: start[h] is the first index with high part >= h, so the bucket
+// of h is [start[h], start[h+1]) and holds t if the basis does.
 struct RankSearch {
     const uint64_t *states;
     int64_t D;
+    const uint32_t *start;
+    int shift;
+    uint64_t nbuckets;
     __device__ __forceinline__ int64_t operator()(uint64_t t) const
     {
+        int64_t lo = 0, hi = D;
+        if (start) {
+            const uint64_t h = t >> shift;
+            if (h >= nbuckets) return -1;   // above every state of the basis
+            lo = start[h];
+            hi = start[h + 1];
+        }
+        const int64_t end = hi;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (states[mid] < t) lo = mid + 1; else hi = mid;
+        }
+        return (lo < end && states[lo] == t) ? lo : -1;
+    }
+};
+
+// start[h] = lower_bound(states, h << shift) for h < nbuckets, start[nbuckets] = D
+__global__ void __launch_bounds__(256)
+k_bucket_starts(int64_t D, const uint64_t *__restrict__ states, int shift, uint64_t nbuckets,
+                uint32_t *__restrict__ start)
+{
+    for (uint64_t h = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; h <= nbuckets;
+         h += (uint64_t)gridDim.x * blockDim.x) {
+        if (h == nbuckets) {
+            start[h] = (uint32_t)D;
+            continue;
+        }
+        const uint64_t t = h << shift;
         int64_t lo = 0, hi = D;
         while (lo < hi) {
             int64_t mid = (lo + hi) >> 1;
             if (states[mid] < t) lo = mid + 1; else hi = mid;
         }
-        return (lo < D && states[lo] == t) ? lo : -1;
+        start[h] = (uint32_t)lo;
     }
-};
+}
 
 // Visit every (term entry, partner) pair of row state s. f(e, j, neg) for off-diagonal
 // contributions, fd(e) for diagonal ones (target = s itself).
@@ -1017,6 +1053,116 @@ k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ bl
     P.y[i] = v;
 }
 
+// Up to kMaxGroup problems with the same basis and the same term structure (bucket offsets
+// and annihilation codes; their amplitudes differ) in one descriptor: each row is walked
+// once, each partner looked up once, and every problem's amplitude and x entry applied to
+// it. Per problem the arithmetic and its order are row_one's, with the diagonal read from
+// that problem's own table, so each y is bitwise that problem's single matvec.
+constexpr int kMaxGroup = 4;
+struct BatchMvGroupDesc {
+    int64_t D;
+    const uint64_t *states;
+    CompactDev t;   // the shared structure (the first problem's image)
+    RankLin lin;
+    RankSearch search;
+    int use_lin;
+    int64_t blk0;
+    const double2 *amp[kMaxGroup];
+    const double2 *diag[kMaxGroup];
+    const double2 *x[kMaxGroup];
+    double2 *y[kMaxGroup];
+};
+
+template <class Rank, int G>
+__device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned char *base,
+                                          const BatchMvGroupDesc &P, const Rank &rank)
+{
+    const uint16_t *s_keyptr = (const uint16_t *)base;
+    const uint16_t *s_code = (const uint16_t *)(base + P.t.off_code);
+    double yr[G], yi[G], dr[G], di[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) yr[g] = yi[g] = dr[g] = di[g] = 0.0;
+    const auto rrow = rank_row32(rank, s);
+    auto take = [&](int e, uint64_t target, uint32_t neg) {
+        const int64_t j = rank_find(rrow, target);
+        if (RankMayMiss<Rank>::value && j < 0) return;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const double2 a = P.amp[g][e];
+            const double2 xv = P.x[g][j];
+            cmac(yr[g], yi[g], flip_sign(a.x, neg), flip_sign(a.y, neg), xv);
+        }
+    };
+    auto enter = [&](int key, int &e0, int &e1) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const double2 dg = P.diag[g][key];
+            dr[g] += dg.x;
+            di[g] += dg.y;
+        }
+        e0 = s_keyptr[key];
+        e1 = s_keyptr[key + 1];
+    };
+    uint64_t rem = s;
+    while (rem) {
+        const int b = 63 - __builtin_clzll(rem);
+        rem ^= 1ULL << b;
+        const uint64_t ob = 1ULL << b;
+        if (b < P.t.nkeys) {   // one-body entries: in = {c}
+            int e0, e1;
+            enter(b, e0, e1);
+            const uint64_t mid = s & ~ob;
+            const uint32_t pout = __popcll(mid & pm_one(b)) & 1u;
+            for (int e = e0; e < e1; ++e) {
+                const int c = s_code[e] & 0xFF;
+                const uint64_t inm = 1ULL << c;
+                if (mid & inm) continue;
+                take(e, mid | inm, (pout + (uint32_t)__popcll(mid & pm_one(c))) & 1u);
+            }
+        }
+        uint64_t low = s & (ob - 1ULL);
+        const int kb = 64 + ((b * (b - 1)) >> 1);
+        while (low) {
+            const int a = 63 - __builtin_clzll(low);
+            low ^= 1ULL << a;
+            const int key = kb + a;
+            if (key >= P.t.nkeys) continue;
+            int e0, e1;
+            enter(key, e0, e1);
+            const uint64_t mid = s & ~(ob | (1ULL << a));
+            const uint32_t pout = __popcll(mid & pm_pair(a, b)) & 1u;
+            for (int e = e0; e < e1; ++e) {   // two-body entries: in = {c, d}, c < d
+                const unsigned code = s_code[e];
+                const int c = code & 0xFF, d = code >> 8;
+                const uint64_t bd = 1ULL << d;
+                const uint64_t inm = bd | (1ULL << c);
+                if (mid & inm) continue;
+                take(e, mid | inm, (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        cmac(yr[g], yi[g], dr[g], di[g], P.x[g][i]);
+        P.y[g][i] = make_double2(yr[g], yi[g]);
+    }
+}
+
+template <int G>
+__global__ void __launch_bounds__(kLdsBlock)
+k_batch_mv_group(const BatchMvGroupDesc *__restrict__ descs, const int *__restrict__ blk2grp)
+{
+    extern __shared__ uint4 lds_u4[];
+    const BatchMvGroupDesc &P = descs[blk2grp[blockIdx.x]];
+    stage_image(P.t, lds_u4);
+    __syncthreads();
+    const int64_t i = ((int64_t)blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
+    if (i >= P.D) return;
+    const unsigned char *base = (const unsigned char *)lds_u4;
+    if (P.use_lin) row_group<RankLin, G>(i, P.states[i], base, P, P.lin);
+    else row_group<RankSearch, G>(i, P.states[i], base, P, P.search);
+}
+
 // ---------------------------------------------------------------- eigenvector consumers
 // <bra basis | O | ket basis>: rows are bra states, partners are looked up in the ket basis
 // (binary search: the bases may be different sectors); a diagonal term contributes where the
@@ -1277,7 +1423,10 @@ RankLin make_lin(const edm_basis *b)
     return RankLin{b->d_jlo, b->d_jhi, b->nlo, (1ULL << b->nlo) - 1ULL, b->d_jlo16};
 }
 
-RankSearch make_search(const edm_basis *b) { return RankSearch{b->d_states, b->D}; }
+RankSearch make_search(const edm_basis *b)
+{
+    return RankSearch{b->d_states, b->D, b->d_start, b->start_shift, b->nbuckets};
+}
 
 }  // namespace
 
@@ -1367,6 +1516,30 @@ static bool op_conserves_uncached(const edm_op *op, const edm_basis *b)
         }
     }
     return true;
+}
+
+void build_search_index(edm_basis *b)
+{
+    // buckets of about half a state each on average, at most 2^26 (256 MB) of them
+    if (b->d_start || b->D < 1024 || b->D >= (int64_t)0xFFFFFFFFLL || b->nbits < 1) return;
+    int nhi = 1;
+    while (nhi < 26 && ((int64_t)1 << nhi) < 2 * b->D) ++nhi;
+    nhi = std::min(nhi, b->nbits);
+    const int shift = b->nbits - nhi;
+    const uint64_t nb = 1ULL << nhi;
+    edm_ctx *ctx = b->ctx;
+    uint32_t *d = (uint32_t *)ctx->alloc((nb + 1) * sizeof(uint32_t));
+    const unsigned grid = (unsigned)std::min<uint64_t>((nb + 1 + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_bucket_starts, dim3(grid), dim3(256), 0, ctx->stream, b->D,
+                       b->d_states, shift, nb, d);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        ctx->free(d, (nb + 1) * sizeof(uint32_t));
+        EDM_HIP_CHECK(e);
+    }
+    b->d_start = d;
+    b->start_shift = shift;
+    b->nbuckets = nb;
 }
 
 bool verify_rank_tables(edm_basis *b)
@@ -1692,6 +1865,71 @@ void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, i
     if (nblocks == 0) return;
     hipLaunchKernelGGL(k_batch_matvec, dim3((unsigned)nblocks), dim3(kLdsBlock), lds_bytes,
                        ctx->stream, (const BatchMvDesc *)d_descs, d_blk2prob);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+int batch_mv_max_group() { return kMaxGroup; }
+
+size_t batch_mv_group_desc_size() { return sizeof(BatchMvGroupDesc); }
+
+bool batch_mv_same_shape(const edm_op *a, const edm_op *b)
+{
+    const CompactTable &ta = a->cfwd, &tb = b->cfwd;
+    return a->fast && b->fast && ta.ok && tb.ok && a->nbits == b->nbits &&
+           ta.shape_hash == tb.shape_hash && ta.nkeys == tb.nkeys && ta.noff == tb.noff &&
+           ta.lds_bytes == tb.lds_bytes && ta.shape == tb.shape;
+}
+
+bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const double2 *const *x,
+                         double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes)
+{
+    EDM_REQUIRE(G >= 2 && G <= kMaxGroup, EDM_EINVAL, "group size out of range");
+    if (matvec_variant() != 11 || matvec_passes() != 1) return false;
+    const int lin = use_lin(ops[0], b) ? 1 : 0;
+    for (int g = 0; g < G; ++g) {
+        EDM_REQUIRE(ops[g]->nbits == b->nbits, EDM_EINVAL,
+                    "operator and basis have different bits");
+        if (!batch_mv_same_shape(ops[0], ops[g]) || (use_lin(ops[g], b) ? 1 : 0) != lin)
+            return false;
+    }
+    BatchMvGroupDesc d{};
+    d.D = b->D;
+    d.states = b->d_states;
+    d.t = compact_dev(ops[0]->cfwd);
+    d.use_lin = lin;
+    if (lin) d.lin = make_lin(b);
+    d.search = make_search(b);
+    d.blk0 = blk0;
+    for (int g = 0; g < G; ++g) {
+        d.amp[g] = ops[g]->cfwd.d_amp;
+        d.diag[g] = ops[g]->cfwd.d_diag;
+        d.x[g] = x[g];
+        d.y[g] = y[g];
+    }
+    memcpy(out, &d, sizeof(d));
+    *lds_bytes = ops[0]->cfwd.lds_bytes;
+    return true;
+}
+
+void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,
+                           int64_t nblocks, size_t lds_bytes)
+{
+    if (nblocks == 0) return;
+    const auto *dd = (const BatchMvGroupDesc *)d_descs;
+    switch (G) {
+    case 2:
+        hipLaunchKernelGGL(k_batch_mv_group<2>, dim3((unsigned)nblocks), dim3(kLdsBlock),
+                           lds_bytes, ctx->stream, dd, d_blk2grp);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_batch_mv_group<3>, dim3((unsigned)nblocks), dim3(kLdsBlock),
+                           lds_bytes, ctx->stream, dd, d_blk2grp);
+        break;
+    default:
+        hipLaunchKernelGGL(k_batch_mv_group<4>, dim3((unsigned)nblocks), dim3(kLdsBlock),
+                           lds_bytes, ctx->stream, dd, d_blk2grp);
+        break;
+    }
     EDM_HIP_CHECK(hipGetLastError());
 }
 
@@ -2073,6 +2311,13 @@ void upload_compact(edm_ctx *ctx, edm_op *op, CompactTable &t, bool adjoint)
         }
     }
     kp[nkeys] = (uint16_t)p;
+    t.shape.assign(kp, kp + nkeys + 1);
+    t.shape.insert(t.shape.end(), cd, cd + ents.size());
+    {
+        uint64_t h = 0x9E3779B97F4A7C15ULL ^ (uint64_t)t.shape.size();
+        for (uint16_t v : t.shape) h = (h ^ v) * 0x100000001B3ULL;
+        t.shape_hash = h;
+    }
     t.nkeys = nkeys;
     t.noff = (int64_t)ents.size();
     t.off_diag = off_diag;

@@ -20,6 +20,8 @@
 
 namespace {
 
+constexpr int kMaxRankBits = 52;
+
 struct QKey {
     uint64_t counts;      // 8 bits per conserved component
     int64_t m[3];
@@ -72,7 +74,9 @@ bool rank_applicable(const edm_basis *b)
     const SystemSpec &sp = b->spec;
     if (sp.ncc > 8) return false;
     if (b->D >= (int64_t)0xFFFFFFFFLL) return false;
-    return sp.nbits <= 44 && sp.nbits >= 2;
+    // two tables of 2^(nbits/2) entries each: at 52 bits 2 x 256 MB; wider systems keep the
+    // bucketed binary search
+    return sp.nbits <= kMaxRankBits && sp.nbits >= 2;
 }
 
 LowTables low_tables(const SystemSpec &sp)
@@ -195,7 +199,7 @@ bool derive_rank_tables(edm_basis *b, const uint64_t *states)
 {
     const int nbits = b->nbits;
     const int64_t D = b->D;
-    if (nbits < 2 || nbits > 44 || D == 0 || D >= (int64_t)0xFFFFFFFFLL) return false;
+    if (nbits < 2 || nbits > kMaxRankBits || D == 0 || D >= (int64_t)0xFFFFFFFFLL) return false;
     const int nlo = nbits / 2, nhi = nbits - nlo;
     const uint64_t lomask = (1ULL << nlo) - 1ULL;
     constexpr uint32_t kAbsent = 0xFFFFFFFFu;

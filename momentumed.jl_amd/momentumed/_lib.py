@@ -64,6 +64,8 @@ _SIGS = [
     ("edm_connectivity_totals", c_i32, [c_vp, c_vp, c_vp, P(c_i64), P(c_i64)]),
     ("edm_eigsolve", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_vp,
                              c_vp, P(EdmSolveInfo)]),
+    ("edm_eigsolve_general", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp,
+                                     c_vp, c_vp, P(EdmSolveInfo)]),
     ("edm_eigsolve_block", c_i32, [c_vp, c_vp, c_vp, c_i32, c_i32, c_i32, c_dbl, c_i32, c_u64,
                                    c_vp, c_vp, c_vp, P(EdmSolveInfo)]),
     ("edm_batch_create", c_i32, [c_vp, P(c_vp)]),
@@ -91,10 +93,13 @@ _SIGS = [
     ("edm_csr_matvec", c_i32, [c_vp, c_vp, c_vp, c_vp]),
     ("edm_csr_eigsolve", c_i32, [c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp, c_vp, c_vp,
                                  P(EdmSolveInfo)]),
+    ("edm_csr_eigsolve_general", c_i32, [c_vp, c_vp, c_i32, c_i32, c_dbl, c_i32, c_u64, c_vp,
+                                         c_vp, c_vp, P(EdmSolveInfo)]),
     ("edm_csr_dense", c_i32, [c_vp, c_vp, c_vp]),
     ("edm_csr_destroy", c_i32, [c_vp]),
     ("edm_projected_eigh", c_i32, [c_i32, c_vp, c_vp, c_vp]),
     ("edm_projected_heigh", c_i32, [c_i32, c_vp, c_vp, c_vp]),
+    ("edm_projected_schur", c_i32, [c_i32, c_vp, c_vp, c_vp]),
     ("edm_fill_random", c_i32, [c_vp, c_vp, c_i64, c_u64, c_i32]),
 ]
 

@@ -399,9 +399,19 @@ def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True
     (krylovdim 30, tol 1e-12, maxiter 100). Returns (vals, vecs, info); vecs is a list of
     host complex128 arrays (or None). blocksize > 1 runs block Lanczos (edm_eigsolve_block:
     `blocksize` vectors per step through the block matvec; krylovdim is then the basis size,
-    rounded up to a multiple of the block)."""
+    rounded up to a multiple of the block). ishermitian=False runs the general eigensolver
+    (edm_eigsolve_general, Krylov-Schur) as KrylovKit does for a non-Hermitian map: the
+    N_eigen eigenvalues of smallest real part, complex, ascending real part."""
     if not ishermitian:
-        raise ValueError("the device eigensolver implements the Hermitian (:SR) case")
+        if blocksize != 1:
+            raise NotImplementedError("block Lanczos is for Hermitian maps")
+        if H.is_adjoint:
+            raise NotImplementedError("the general eigensolver applies the map itself, not "
+                                      "its adjoint")
+        return _general_solve(H.ctx, H.basis.D, N_eigen, vec0, krylovdim, tol, maxiter, seed,
+                              want_vectors,
+                              lambda *a: H.ctx.L.edm_eigsolve_general(H.ctx.h, H.op.h,
+                                                                      H.basis.h, *a))
     if H.is_adjoint:
         H = H.adjoint()   # H is Hermitian: the adjoint map has the same spectrum
     D = H.basis.D
@@ -424,6 +434,30 @@ def krylov_map_solve(H: ROCmLinearMap, N_eigen: int, *, ishermitian: bool = True
         rc = ctx.L.edm_eigsolve_block(ctx.h, H.op.h, H.basis.h, N_eigen, int(blocksize),
                                       krylovdim, float(tol), maxiter, seed, v0p,
                                       vals.ctypes.data, vp, ctypes.byref(info))
+    if rc not in (_lib.EDM_OK, _lib.EDM_ENOCONV):
+        check(rc, ctx.h)
+    out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
+                "converged": info.converged, "krylovdim": info.krylovdim,
+                "max_residual": info.max_residual, "seconds": info.seconds}
+    _memory_report(ctx)
+    return vals, (list(vecs) if vecs is not None else None), out_info
+
+
+def _general_solve(ctx, D, N_eigen, vec0, krylovdim, tol, maxiter, seed, want_vectors, call):
+    """Shared driver of the general eigensolvers (edm_eigsolve_general and
+    edm_csr_eigsolve_general): complex eigenvalues, ascending real part."""
+    N_eigen = min(N_eigen, D)
+    vals = np.empty(N_eigen, dtype=np.complex128)
+    vecs = np.empty((N_eigen, D), dtype=np.complex128) if want_vectors else None
+    v0 = None
+    if vec0 is not None:
+        v0 = np.ascontiguousarray(np.asarray(vec0, dtype=np.complex128).reshape(-1))
+        if v0.size != D:
+            raise ValueError("Dimension of Hamiltonian linear map mismatches vector length.")
+    info = EdmSolveInfo()
+    rc = call(N_eigen, krylovdim, float(tol), maxiter, seed,
+              v0.ctypes.data if v0 is not None else None, vals.ctypes.data,
+              vecs.ctypes.data if vecs is not None else None, ctypes.byref(info))
     if rc not in (_lib.EDM_OK, _lib.EDM_ENOCONV):
         check(rc, ctx.h)
     out_info = {"iterations": info.iterations, "matvecs": info.matvecs,
@@ -541,7 +575,9 @@ def krylov_matrix_solve(H: DeviceCSR, N_eigen: int, *, ishermitian: bool = True,
         raise NotImplementedError("block Lanczos is provided for the matrix-free map "
                                   "(method=\"map\")")
     if not ishermitian:
-        raise ValueError("the device eigensolver implements the Hermitian (:SR) case")
+        return _general_solve(H.ctx, H.D, N_eigen, vec0, krylovdim, tol, maxiter, seed,
+                              want_vectors,
+                              lambda *a: H.ctx.L.edm_csr_eigsolve_general(H.ctx.h, H.h, *a))
     D = H.D
     N_eigen = min(N_eigen, D)
     vals = np.empty(N_eigen, dtype=np.float64)

@@ -161,7 +161,7 @@ class SolveServer:
                  krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100, seed: int = 0x5EED,
                  mode: str = "batched", prepare_ahead: int = 64, device: int = 0,
                  quantum_rows: int = 1 << 15, big_share: float = 0.5,
-                 head_wait_s: float = 10.0):
+                 head_wait_s: float = 60.0):
         """quantum_rows: the step quantum of the batch's round robin (a problem of D rows
         steps about once every D / quantum_rows steps). big_share: the share of the rows
         stepped that the oldest running problem larger than the quantum receives (its

@@ -71,6 +71,11 @@ def EDsolve(subspace: HilbertSubspace, hamiltonian, *more_scats, N: int, method:
             raise AssertionError(f"Hilbert space dimension {len(subspace)} exceeds limit of "
                                  f"index_type={index_type}.")
     dim = len(subspace)
+    if method in ("sparse", "dense") and not ishermitian:
+        # the MBOperator form forwards to the scatter-list form, which asserts it
+        # (MomentumED.jl:247)
+        raise AssertionError("Current Hamiltonian matrix construction assumes it being "
+                             "Hermitian.")
     if method in ("sparse", "dense"):
         # MomentumED.jl:233-244
         if min_sparse_dim > max_dense_dim:

@@ -199,6 +199,16 @@ def batched_para(ratio: float) -> EDPara:
                   two_momentum_coordinate=False)
 
 
+def literal_batched_para(ratio: float) -> EDPara:
+    """The batched configuration as written: two conserved components on a 4x6 mesh each (48
+    orbitals), intra-component V with c = 1/4 and inter-component ratio * V. With
+    N_up = N_down = 4 its 24 sectors hold about 4.7 M states each."""
+    k = mesh(4, 6)
+    return EDPara(k_list=k, Gk=(4, 6), Nc_conserve=2,
+                  H_two=CoulombGaussian(k, (4, 6), c=0.25, inter_ratio=float(ratio)),
+                  two_momentum_coordinate=False)
+
+
 def batched_ratios(n: int = 64, seed: int = 99) -> np.ndarray:
     return np.random.default_rng(seed).uniform(0.5, 2.0, n)
 

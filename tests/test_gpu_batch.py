@@ -91,6 +91,23 @@ def test_batch_two_component_uneven_sizes(gpu_ctx, oracle_mod):
     assert checked == 2
 
 
+def test_batch_shared_walk_groups_bitwise_equal_alone(gpu_ctx):
+    """Nine parameter points on each of two bases: the batch walks each basis's rows once
+    for up to four problems (groups of 4, 4 and 1 per basis); every problem's values,
+    vectors and matvec count are still those of its lone solve."""
+    ratios = [0.55, 0.7, 0.85, 1.1, 1.25, 1.4, 1.6, 1.8, 1.95]
+    probs = _two_component_problems(gpu_ctx, ratios, [(3, 4)], [0, 5])
+    with B.BatchSolver(gpu_ctx) as bs:
+        ids = [bs.add(m, W.BATCH_NEV, want_vectors=True) for m, _, _ in probs]
+        res = bs.run()
+    for (m, label, _), pid in zip(probs, ids):
+        r = res[pid]
+        vals, vecs, info = _alone(m, W.BATCH_NEV)
+        np.testing.assert_array_equal(r.values, vals)
+        np.testing.assert_array_equal(np.array(r.vectors), vecs)
+        assert r.info["matvecs"] == info["matvecs"], label
+
+
 def test_batch_continuous_admission_and_removal(gpu_ctx):
     """Problems admitted while others are mid-solve, one removed mid-run: every collected
     result equals its solve alone."""

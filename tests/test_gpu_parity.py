@@ -596,3 +596,108 @@ def test_large_sector_enumeration_and_sampled_matvec(gpu_ctx, oracle_mod):
     print("[large] oracle rows", flush=True)
     yo = oracle_mod.matvec(basis, full, x, rows=rows)
     assert np.abs(y[rows] - yo).max() <= Y_TOL * np.abs(yo).max()
+
+
+# ------------------------------------------------------------------ 48 orbitals (rank tables)
+def test_literal_batched_sector_48_bits(gpu_ctx, oracle_mod):
+    """The batched configuration as written: two conserved components on a 4x6 mesh each (48
+    orbitals), N_up = N_down = 4, sector (0,0) (about 4.7 M states). Its basis gets rank
+    tables (24 + 24 bits); H x is checked against the oracle on sampled rows and is bitwise
+    the same with the bucketed binary search (rank tables dropped)."""
+    import torch
+    para = W.literal_batched_para(1.3)
+    op = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    sysd = _system(para, (4, 4))
+    print("[48-bit] enumerate sector (0,0)", flush=True)
+    b = dev.DeviceBasis.enumerate(gpu_ctx, sysd, 0)
+    assert b.rank_mode == 1 and 4_000_000 < b.D < 5_500_000
+    basis = b.download()
+    assert np.all(basis[1:] > basis[:-1])
+    rng = np.random.default_rng(48)
+    sample = np.unique(np.concatenate([np.arange(32), rng.integers(0, b.D, 4096),
+                                       np.arange(b.D - 32, b.D)]))
+    lo = basis[sample] & np.uint64((1 << 24) - 1)
+    hi = basis[sample] >> np.uint64(24)
+    assert all(bin(int(v)).count("1") == 4 for v in lo)
+    assert all(bin(int(v)).count("1") == 4 for v in hi)
+    k = para.k_list
+    for s in basis[sample[:256]]:
+        bits = [i for i in range(48) if (int(s) >> i) & 1]
+        ks = np.sum(k[:, [i % 24 for i in bits]], axis=1) % np.array([4, 6])
+        assert tuple(ks) == (0, 0)
+    dop = dev.DeviceOperator.from_mboperator(gpu_ctx, op, para.bits)
+    x = _rand(b.D, 4848)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    print("[48-bit] matvec with rank tables", flush=True)
+    dev.ROCmLinearMap(dop, b)(yd, xd)
+    y = yd.cpu().numpy()
+    full = _oracle_full(oracle_mod, op)
+    print("[48-bit] oracle rows", flush=True)
+    yo = oracle_mod.matvec(basis, full, x, rows=sample)
+    assert np.abs(y[sample] - yo).max() <= Y_TOL * np.abs(yo).max()
+    b.drop_rank()
+    assert b.rank_mode == 0
+    print("[48-bit] matvec with the bucketed binary search", flush=True)
+    y2 = torch.empty_like(xd)
+    dev.ROCmLinearMap(dop, b)(y2, xd)
+    assert torch.equal(y2, yd)
+
+
+# ------------------------------------------------------------------ general eigensolver
+def _random_general_op(rng, bits, n2=60, n1=20):
+    terms = []
+    for _ in range(n2):
+        a, b_, c, d = rng.choice(bits, 4, replace=False) + 1
+        terms.append(me.Scatter(complex(rng.normal(), rng.normal()), int(a), int(b_), int(c),
+                                int(d), bits=bits))
+    for _ in range(n1):
+        a, c = rng.choice(bits, 2, replace=False) + 1
+        terms.append(me.Scatter(complex(rng.normal(), rng.normal()), int(a), int(c), bits=bits))
+    return me.MBOperator(terms, upper_hermitian=False)
+
+
+def test_eigsolve_general_non_hermitian(gpu_ctx, oracle_mod):
+    """ishermitian=False (Krylov-Schur on the device, as KrylovKit runs for a non-Hermitian
+    map, linear_map.jl:179-191) on a general operator of 10 orbitals and 4 particles: the 6
+    eigenvalues of smallest real part equal LAPACK's (numpy.linalg.eig on the oracle's dense
+    matrix) to 1e-10, right eigenvectors have residual <= 1e-9; the same through
+    EDsolve(method="map", ishermitian=False) and through the stored matrix
+    (krylov_matrix_solve(..., ishermitian=False)); a Hermitian operator solved as a general
+    one gives the Hermitian solver's (real) eigenvalues."""
+    rng = np.random.default_rng(29)
+    bits = 10
+    op = _random_general_op(rng, bits)
+    full = _oracle_full(oracle_mod, op)
+    basis = np.array([s for s in range(1 << bits) if bin(s).count("1") == 4], dtype=np.uint64)
+    Hd = oracle_mod.dense_matrix(basis, full)
+    ev = np.linalg.eigvals(Hd)
+    ev = ev[np.lexsort((ev.imag, ev.real))]
+    scale = np.abs(ev).max()
+    sp = me.HilbertSubspace(basis, bits=bits)
+    lm = dev.create_gpu_linearmap(op, sp)
+    vals, vecs, info = dev.krylov_map_solve(lm, 6, ishermitian=False, krylovdim=30, tol=1e-12)
+    print(f"[general] {info}", flush=True)
+    assert info["converged"] == 6 and vals.dtype == np.complex128
+    np.testing.assert_allclose(vals, ev[:6], rtol=0, atol=1e-10 * scale)
+    for lam, v in zip(vals, vecs):
+        assert abs(np.linalg.norm(v) - 1) < 1e-12
+        assert np.linalg.norm(Hd @ v - lam * v) <= 1e-9 * scale
+    E, V = me.EDsolve(sp, op, N=6, method="map", ishermitian=False, method_info=False)
+    np.testing.assert_array_equal(E, vals)
+    with pytest.raises(AssertionError):
+        me.EDsolve(sp, op, N=6, method="sparse", ishermitian=False, method_info=False)
+    Hm = dev.create_gpu_matrix(op, sp)
+    vm, _, im = dev.krylov_matrix_solve(Hm, 6, ishermitian=False, krylovdim=30)
+    Hm.close()
+    assert im["converged"] == 6
+    np.testing.assert_allclose(vm, ev[:6], rtol=0, atol=1e-10 * scale)
+    # Hermitian operator through the general solver
+    w = W.small()
+    sps = me.ED_momentum_subspaces(w.para, 5)[0][0]
+    Hh = dev.create_gpu_linearmap(w.operator(), sps)
+    vh, _, _ = dev.krylov_map_solve(Hh, 8, want_vectors=False)
+    vg, _, ig = dev.krylov_map_solve(Hh, 8, ishermitian=False, want_vectors=False)
+    assert ig["converged"] == 8
+    np.testing.assert_allclose(vg.real, vh, rtol=1e-10, atol=0)
+    assert np.abs(vg.imag).max() <= 1e-10 * np.abs(vh).max()

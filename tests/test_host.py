@@ -554,3 +554,59 @@ def test_host_enumeration_equals_the_oracle(oracle_mod):
         assert ks == oks
         for sp, ob in zip(subs, lists):
             np.testing.assert_array_equal(sp.list, ob)
+
+
+@pytest.mark.parametrize("kind", ["random", "krylov_schur", "triangular", "hermitian", "jordan",
+                                  "diag", "tiny"])
+def test_projected_schur_matches_lapack(kind):
+    """The complex Schur decomposition of the general eigensolver (edm_projected_schur)
+    against numpy (LAPACK zgeev): A = Z T Z^H to 1e-13 of ||A||, Z unitary, T upper
+    triangular with its diagonal ascending by real part (then imaginary part), the diagonal
+    equal to LAPACK's eigenvalues. 'krylov_schur' is the projected matrix after a restart
+    (upper triangular block, a full coupling row, Hessenberg below); 'jordan' a defective
+    matrix (one eigenvalue, one eigenvector), where the eigenvalues agree to ~eps^(1/n)."""
+    from momentumed import _lib
+    L = _lib.load()
+    rng = np.random.default_rng({"random": 21, "krylov_schur": 22, "triangular": 23,
+                                 "hermitian": 24, "jordan": 25, "diag": 26, "tiny": 27}[kind])
+
+    def crand(*shape):
+        return rng.normal(size=shape) + 1j * rng.normal(size=shape)
+    for n in (1, 2, 3, 5, 12, 30, 64):
+        if kind == "random":
+            A = crand(n, n)
+        elif kind == "krylov_schur":
+            k = (3 * n) // 5
+            A = np.triu(crand(n, n), -1)
+            A[:k, :k] = np.triu(crand(k, k))
+            if k < n:
+                A[k, :k] = crand(k)
+        elif kind == "triangular":
+            A = np.triu(crand(n, n))
+        elif kind == "hermitian":
+            B = crand(n, n)
+            A = B + B.conj().T
+        elif kind == "jordan":
+            A = np.eye(n) * (0.3 + 0.7j) + np.diag(np.ones(n - 1), 1)
+        elif kind == "diag":
+            A = np.diag(crand(n))
+        else:
+            A = crand(n, n) * 1e-300
+        A = np.ascontiguousarray(A, dtype=np.complex128)
+        T = np.zeros_like(A)
+        Z = np.zeros_like(A)
+        assert L.edm_projected_schur(n, A.ctypes.data, T.ctypes.data, Z.ctypes.data) == 0
+        scale = max(np.abs(A).max(), 1e-300)
+        assert np.abs(Z @ T @ Z.conj().T - A).max() <= 1e-13 * scale
+        assert np.abs(Z.conj().T @ Z - np.eye(n)).max() <= 1e-13
+        assert np.all(np.tril(T, -1) == 0)
+        d = np.diag(T)
+        for i in range(n - 1):
+            assert d[i].real < d[i + 1].real or (d[i].real == d[i + 1].real
+                                                 and d[i].imag <= d[i + 1].imag)
+        if kind == "jordan":
+            assert np.abs(d - (0.3 + 0.7j)).max() <= 1e-15 ** (1.0 / n) * 10
+            continue
+        ev = np.linalg.eigvals(A)
+        ev = ev[np.lexsort((ev.imag, ev.real))]
+        assert np.abs(d - ev).max() <= 1e-10 * scale
