@@ -252,6 +252,19 @@ int edm_matvec_between(edm_ctx *ctx, const edm_op *op, const edm_basis *bra,
  * out[0..1] = the complex result. */
 int edm_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
                  const edm_basis *ket, const void *bra_dev, const void *ket_dev, double *out);
+/* edm_bracket2 for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * The same kernels and summation order as edm_bracket2, so the same result bit for bit, but
+ * stream-ordered: out_dev (the caller's device memory, one complex) receives
+ * <bra| op2 op1 |ket> when the stream reaches it, with no host synchronisation. The
+ * partial sums live in a file-scope device array of the library (4096 complex values, 64 KB
+ * per device, held until the process ends); the bracket needs at most 2048 of them, and a
+ * call that would need more is rejected with EDM_EINVAL. It calls no hipMalloc, hipFree or
+ * allocating library routine. Every call on one device shares that array: calls that overlap
+ * on one device (two streams, or two host threads) race. */
+int edm_bracket2_noalloc(edm_ctx *ctx, const edm_op *op2, const edm_op *op1,
+                         const edm_basis *bra, const edm_basis *ket, const void *bra_dev,
+                         const void *ket_dev, void *out_dev);
 /* PES_1rdm (MomentumED/src/analysis/particle_reduced_density_matrix.jl:4-33): for each of
  * npairs (i, j) (0-based orbitals, pairs[2p], pairs[2p+1]) out[2p..2p+1] = <psi|c_i^+ c_j|psi>
  * for the device vector psi in basis b. On a complete sector, pairs that do not conserve it

@@ -123,6 +123,13 @@ int edm_ctx_create(int device, uint64_t hbm_budget_bytes, edm_ctx **out)
             delete c;
             throw EdmError{EDM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
         }
+        try {
+            c->d_noalloc = noalloc_scratch_address();
+        } catch (...) {
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            throw;
+        }
         *out = c;
         return EDM_OK;
     });
@@ -856,6 +863,24 @@ int edm_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_b
         ctx->free(d_out, sizeof(double2));
         out[0] = h.x;
         out[1] = h.y;
+        return EDM_OK;
+    });
+}
+
+int edm_bracket2_noalloc(edm_ctx *ctx, const edm_op *op2, const edm_op *op1,
+                         const edm_basis *bra, const edm_basis *ket, const void *bra_dev,
+                         const void *ket_dev, void *out_dev)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op2 && op1 && bra && ket && out_dev, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op2->ctx == ctx && op1->ctx == ctx && bra->ctx == ctx && ket->ctx == ctx,
+                    EDM_EINVAL, "handles from another context");
+        EDM_REQUIRE(op1->nbits == ket->nbits && op2->nbits == bra->nbits && bra->nbits == ket->nbits,
+                    EDM_EDIM, "operators and bases have different bits");
+        EDM_REQUIRE((ket->D == 0 || ket_dev) && (bra->D == 0 || bra_dev), EDM_EINVAL,
+                    "NULL vector");
+        launch_bracket2_noalloc(ctx, op2, op1, bra, ket, (const double2 *)bra_dev,
+                                (const double2 *)ket_dev, (double2 *)out_dev);
         return EDM_OK;
     });
 }

@@ -65,6 +65,8 @@ struct edm_ctx {
     // reduction scratch
     double *d_partials = nullptr;
     size_t partials_bytes = 0;
+    // this device's address of the file-scope array edm_bracket2_noalloc keeps its partials in
+    double2 *d_noalloc = nullptr;
 
     void *alloc(size_t bytes);
     void free(void *p, size_t bytes);
@@ -294,6 +296,13 @@ void launch_rdm1(edm_ctx *ctx, const edm_basis *b, const double2 *psi, const int
 void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const edm_basis *bra,
                      const edm_basis *ket, const double2 *d_bra, const double2 *d_ket,
                      double2 *d_out);
+// the device address of the file-scope scratch array on the current device
+double2 *noalloc_scratch_address();
+// the same with the chunk partials in the library's file-scope scratch array (no device
+// allocation, no host synchronisation; d_out is the caller's device complex)
+void launch_bracket2_noalloc(edm_ctx *ctx, const edm_op *op2, const edm_op *op1,
+                             const edm_basis *bra, const edm_basis *ket, const double2 *d_bra,
+                             const double2 *d_ket, double2 *d_out);
 // M (nA x B->D, row-major, zeroed by the caller) of the particle reduced density matrix
 // rdm = M M^H; d_counts[0] += (state, part) pairs with the part inside the transformed state,
 // d_counts[1] += those whose complement is in B

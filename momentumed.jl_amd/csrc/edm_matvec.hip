@@ -24,7 +24,6 @@
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
 #include <numeric>
 #include <unordered_map>
 #include <utility>
@@ -2096,6 +2095,48 @@ void launch_bracket2(edm_ctx *ctx, const edm_op *op2, const edm_op *op1, const e
     double2 *part = (double2 *)ctx->partials((size_t)nch * sizeof(double2));
     TermsDev t2{op2->S_full, op2->d_amp, op2->d_out, op2->d_in, op2->d_par};
     TermsDev t1{op1->S_full, op1->d_amp, op1->d_out, op1->d_in, op1->d_par};
+    hipLaunchKernelGGL(k_bracket2, dim3((unsigned)nch), dim3(kRedBlock), 0, ctx->stream, D,
+                       ket->d_states, d_ket, make_search(bra), d_bra, t2, t1, rows, part);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kRedBlock), 0, ctx->stream, part, nch, d_out);
+    EDM_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- allocation-free bracket
+// The chunk partial sums of launch_bracket2_noalloc live in this file-scope array: one copy
+// per device, held until the process ends, shared by every call on that device (calls that
+// overlap on one device race). Host code reaches it only through the address
+// hipGetSymbolAddress gives with the device current, looked up once per context (one context
+// per device) when the context is created.
+constexpr int64_t kBracketScratch = 4096;   // chunk partials; chunk_rows makes at most 2048
+static __device__ __align__(256) double2 scratch[kBracketScratch];
+
+double2 *noalloc_scratch_address()
+{
+    void *p = nullptr;
+    EDM_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    return (double2 *)p;
+}
+
+void launch_bracket2_noalloc(edm_ctx *ctx, const edm_op *op2, const edm_op *op1,
+                             const edm_basis *bra, const edm_basis *ket, const double2 *d_bra,
+                             const double2 *d_ket, double2 *d_out)
+{
+    double2 *part = ctx->d_noalloc;
+    EDM_REQUIRE(part != nullptr, EDM_EINVAL, "no scratch address for this context");
+    const int64_t D = ket->D;
+    if (D == 0 || bra->D == 0) {   // an empty sum: the chunk sum of no partials is 0
+        hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kRedBlock), 0, ctx->stream, part,
+                           (int64_t)0, d_out);
+        EDM_HIP_CHECK(hipGetLastError());
+        return;
+    }
+    const int64_t rows = chunk_rows(D, 1);
+    const int64_t nch = std::max<int64_t>(1, (D + rows - 1) / rows);
+    EDM_REQUIRE(nch <= kBracketScratch, EDM_EINVAL,
+                "bracket needs more partial sums than the allocation-free scratch holds");
+    TermsDev t2{op2->S_full, op2->d_amp, op2->d_out, op2->d_in, op2->d_par};
+    TermsDev t1{op1->S_full, op1->d_amp, op1->d_out, op1->d_in, op1->d_par};
+    // every chunk writes its partial before the sum reads it: nothing to clear
     hipLaunchKernelGGL(k_bracket2, dim3((unsigned)nch), dim3(kRedBlock), 0, ctx->stream, D,
                        ket->d_states, d_ket, make_search(bra), d_bra, t2, t1, rows, part);
     hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(kRedBlock), 0, ctx->stream, part, nch, d_out);

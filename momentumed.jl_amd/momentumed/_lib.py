@@ -84,6 +84,7 @@ _SIGS = [
     ("edm_matvec_tune", c_i32, [c_vp, c_vp, c_vp, P(c_i32), c_vp]),
     ("edm_matvec_between", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("edm_bracket2", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("edm_bracket2_noalloc", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("edm_pes_block_matrix", c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp, c_u64, c_vp, c_vp, c_vp]),
     ("edm_rdm1", c_i32, [c_vp, c_vp, c_vp, c_i32, c_vp, c_vp]),
     ("edm_step_inner_prod", c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
