@@ -142,6 +142,11 @@ struct CompactTable {
     uint16_t *d_code = nullptr;     // noff
     double2 *d_amp = nullptr;       // noff
     double2 *d_diag = nullptr;      // nkeys (zero where a key has no diagonal term)
+    // the same amplitudes position-major: entry p of key k at [p * nkeys + k], for p below the
+    // longest bucket (amp_rows); rows that walk the same entry position of neighbouring keys
+    // read neighbouring words
+    double2 *d_amp_pm = nullptr;
+    int amp_rows = 0;
     size_t lds_bytes = 0;
     size_t off_diag = 0, off_code = 0;   // byte offsets inside the LDS image
     // optional tail of the same image: a 16-bit index per entry into a table of distinct
