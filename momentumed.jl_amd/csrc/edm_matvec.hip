@@ -382,6 +382,19 @@ __device__ __forceinline__ uint64_t pm_one(int p)
     return p >= 63 ? 0ULL : (~0ULL << (p + 1));
 }
 
+// bit x = parity of the set bits of s below x (prefix XOR of s shifted up by one)
+__device__ __forceinline__ uint64_t prefix_parity(uint64_t s)
+{
+    uint64_t p = s << 1;
+    p ^= p << 1;
+    p ^= p << 2;
+    p ^= p << 4;
+    p ^= p << 8;
+    p ^= p << 16;
+    p ^= p << 32;
+    return p;
+}
+
 // Consecutive workgroups land on different XCDs; remap so each XCD works through one
 // contiguous range of rows (its L2 then holds one neighbourhood of x, not eight).
 __device__ __forceinline__ int64_t xcd_block(int swz)
@@ -590,6 +603,7 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
     const double2 *s_amps = (const double2 *)(base + t.off_amps);
     double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
     const auto rrow = rank_row32(rank, s);
+    const uint64_t ps = prefix_parity(s);
 
     // pmi: the entry's index in the position-major amplitude table
     auto take = [&](int e, int pmi, const auto &mview, uint64_t target, uint32_t neg) {
@@ -648,14 +662,17 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
             const uint64_t mid = s & ~(ob | (1ULL << a));
             const uint32_t pout = __popcll(mid & pm_pair(a, b)) & 1u;
             const auto mview = rank_mid32(rrow, mid);
-            for (int e = e0; e < e1; ++e) {   // two-body entries: in = {c, d}, c < d
-                const unsigned code = s_code[e];
+            // bit x: parity of mid's orbitals below x; for free c < d the orbitals of mid
+            // strictly between them have parity bit c ^ bit d
+            const uint64_t pmid = ps ^ pm_one(a) ^ pm_one(b);
+            const int n = e1 - e0;
+            int pmi = key;   // position-major index of entry e0 + q
+            for (int q = 0; q < n; ++q, pmi += t.nkeys) {   // two-body entries: in = {c, d}, c < d
+                const unsigned code = s_code[e0 + q];
                 const int c = code & 0xFF, d = code >> 8;
-                const uint64_t bd = 1ULL << d;
-                const uint64_t inm = bd | (1ULL << c);
-                if (mid & inm) continue;
-                const uint32_t neg = (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u;
-                take(e, (e - e0) * t.nkeys + key, mview, mid | inm, neg);
+                if (((uint32_t)(mid >> c) | (uint32_t)(mid >> d)) & 1u) continue;
+                const uint32_t neg = (pout ^ (uint32_t)(pmid >> c) ^ (uint32_t)(pmid >> d)) & 1u;
+                take(e0 + q, pmi, mview, mid | (1ULL << c) | (1ULL << d), neg);
             }
         }
     }
@@ -1105,7 +1122,7 @@ struct BatchMvGroupDesc {
     RankSearch search;
     int use_lin;
     int64_t blk0;
-    const double2 *amp[kMaxGroup];
+    const double2 *amp_pm[kMaxGroup];   // position-major amplitudes, [p * nkeys + key]
     const double2 *diag[kMaxGroup];
     const double2 *x[kMaxGroup];
     double2 *y[kMaxGroup];
@@ -1121,12 +1138,13 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
 #pragma unroll
     for (int g = 0; g < G; ++g) yr[g] = yi[g] = dr[g] = di[g] = 0.0;
     const auto rrow = rank_row32(rank, s);
-    auto take = [&](int e, const auto &mview, uint64_t target, uint32_t neg) {
+    // pmi: the entry's index in the position-major amplitude tables
+    auto take = [&](int pmi, const auto &mview, uint64_t target, uint32_t neg) {
         const int64_t j = rank_find(rrow, mview, target);
         if (RankMayMiss<Rank>::value && j < 0) return;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            const double2 a = P.amp[g][e];
+            const double2 a = P.amp_pm[g][pmi];
             const double2 xv = P.x[g][j];
             cmac(yr[g], yi[g], flip_sign(a.x, neg), flip_sign(a.y, neg), xv);
         }
@@ -1156,7 +1174,7 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
                 const int c = s_code[e] & 0xFF;
                 const uint64_t inm = 1ULL << c;
                 if (mid & inm) continue;
-                take(e, mview, mid | inm, (pout + (uint32_t)__popcll(mid & pm_one(c))) & 1u);
+                take((e - e0) * P.t.nkeys + b, mview, mid | inm, (pout + (uint32_t)__popcll(mid & pm_one(c))) & 1u);
             }
         }
         uint64_t low = s & (ob - 1ULL);
@@ -1177,7 +1195,8 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
                 const uint64_t bd = 1ULL << d;
                 const uint64_t inm = bd | (1ULL << c);
                 if (mid & inm) continue;
-                take(e, mview, mid | inm, (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u);
+                take((e - e0) * P.t.nkeys + key, mview, mid | inm,
+                     (pout + (uint32_t)__popcll(mid & (bd - (2ULL << c)))) & 1u);
             }
         }
     }
@@ -1941,7 +1960,7 @@ bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const dou
     d.search = make_search(b);
     d.blk0 = blk0;
     for (int g = 0; g < G; ++g) {
-        d.amp[g] = ops[g]->cfwd.d_amp;
+        d.amp_pm[g] = ops[g]->cfwd.d_amp_pm;
         d.diag[g] = ops[g]->cfwd.d_diag;
         d.x[g] = x[g];
         d.y[g] = y[g];
