@@ -300,11 +300,12 @@ def batched_section(args):
     nplain = min(len(bases), len(maps))
     ctx.synchronize()
     t0 = time.perf_counter()
-    plain_vals, plain_mv = [], 0
+    plain_vals, plain_mv, plain_conv = [], 0, 0
     for m in maps[:nplain]:
         v, _, info = dev.krylov_map_solve(m, nev, krylovdim=kd, tol=tol, want_vectors=False)
         plain_vals.append(v)
         plain_mv += info["matvecs"]
+        plain_conv += int(info["converged"] >= nev)
     t_plain = time.perf_counter() - t0
     log(f"[batched] plain loop: {nplain} solves in {t_plain:.2f} s ({nplain / t_plain:.1f}/s)")
     used0, _ = ctx.memory()
@@ -328,6 +329,8 @@ def batched_section(args):
         "problems": len(maps), "D_min": min(dims), "D_max": max(dims), "nev": nev,
         "krylovdim": kd, "tol": tol, "converged": nconv,
         "sector_solves_per_s": round(sps_b, 2), "wall_s": round(t_batch, 3),
+        "converged_sector_solves_per_s": round(nconv / t_batch, 2),
+        "plain_converged": plain_conv,
         "add_s": round(t_add, 3), "matvecs": mv,
         "plain_sector_solves_per_s": round(sps_p, 2), "plain_sample": nplain,
         "plain_sample_s": round(t_plain, 3), "plain_matvecs": plain_mv,
@@ -335,6 +338,63 @@ def batched_section(args):
         "sample_bitwise_equal": bool(same),
         "workspace_bytes": int(peak - used0), "setup_s": round(t_enum + t_ops, 3),
     }
+
+
+def literal_section(args):
+    """One (sector, ratio) point of the batched configuration as written: two conserved
+    components on a 4x6 mesh each (48 orbitals), N_up = N_down = 4, sector (0,0), ratio 1.3
+    (D = 4,705,212): basis enumeration, matvec time, and the lowest 4 by the single-vector
+    iteration (krylovdim 30) and by block Lanczos with block size 4 (krylovdim 60), each
+    timed alone, with the sweep's time extrapolated from them (24 sectors x 64 ratios)."""
+    import torch
+    import momentumed as me
+    from momentumed import device as dev
+    from momentumed import workloads as W
+    para = W.literal_batched_para(1.3)
+    op = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    ctx = dev.get_context(0)
+    sysd = dev.system_struct(para.k_list, para.Gk, para.Nc_mix, para.Nc_conserve, (4, 4))
+    t0 = time.perf_counter()
+    b = dev.DeviceBasis.enumerate(ctx, sysd, 0)
+    t_enum = time.perf_counter() - t0
+    dop = dev.DeviceOperator.from_mboperator(ctx, op, para.bits)
+    lm = dev.ROCmLinearMap(dop, b)
+    x = torch.empty(b.D, dtype=torch.complex128, device="cuda:0")
+    y = torch.empty_like(x)
+    ctx.synchronize()
+    from momentumed._lib import check
+    check(ctx.L.edm_fill_random(ctx.h, x.data_ptr(), b.D, 12345, 1), ctx.h)
+    for _ in range(3):
+        lm.matvec_device(y.data_ptr(), x.data_ptr())
+    ctx.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream, device="cuda:0")
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(10):
+        lm.matvec_device(y.data_ptr(), x.data_ptr())
+    ev1.record(stream)
+    ev1.synchronize()
+    ms_mv = ev0.elapsed_time(ev1) / 10
+    solves = {}
+    for bs, kd in ((1, 30), (4, 60)):
+        t0 = time.perf_counter()
+        v, _, info = dev.krylov_map_solve(lm, 4, krylovdim=kd, tol=1e-12, blocksize=bs,
+                                          want_vectors=False)
+        ts = time.perf_counter() - t0
+        solves[f"block{bs}"] = {"krylovdim": kd, "seconds": round(ts, 3),
+                                "matvecs": info["matvecs"], "converged": info["converged"],
+                                "eigenvalues": [float(e) for e in v]}
+        log(f"[literal] block {bs}: {ts:.2f} s, {info['matvecs']} matvecs, values {v}")
+    log(f"[literal] D={b.D} enumerated in {t_enum:.2f} s, matvec {ms_mv:.3f} ms")
+    t1 = solves["block1"]["seconds"]
+    return {"workload": "two conserved components, 4x6 mesh each (48 orbitals), N_up = N_down = 4, "
+                        "sector (0,0), inter-component ratio 1.3",
+            "D": b.D, "rank_mode": b.rank_mode, "enumerate_s": round(t_enum, 3),
+            "matvec_ms": round(ms_mv, 4), "solves": solves,
+            "sweep_s_extrapolated": round(t1 * 24 * 64, 1),
+            "note": "single solves timed alone; the sweep figure is 1,536 such solves one after "
+                    "another (extrapolated, not run)"}
 
 
 def served_section(args):
@@ -388,6 +448,8 @@ def main():
                     help="skip the block-matvec line (4 vectors per launch)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-literal", action="store_true",
+                    help="skip the sample point of the batched configuration as written")
     ap.add_argument("--batch-ratios", type=int, default=64)
     ap.add_argument("--serve-seconds", type=float, default=120.0)
     ap.add_argument("--serve-plain-seconds", type=float, default=20.0,
@@ -551,6 +613,8 @@ def main():
         out["configs"] = {n: config_section(n, args) for n in ("small", "medium")}
     if not args.no_batched:
         out["batched"] = batched_section(args)
+        if not args.no_literal:
+            out["batched"]["literal_sample"] = literal_section(args)
     if not args.no_served:
         out["served"] = served_section(args)
     print(json.dumps(out), flush=True)
