@@ -116,6 +116,34 @@ def test_edsolve_sparse_against_lapack_and_map(gpu_ctx, oracle_mod):
             assert np.linalg.norm(Hd @ v.vec - E[i] * v.vec) < 1e-9
 
 
+def test_edsolve_float32_results(gpu_ctx, oracle_mod):
+    """element_type=Float32 (MomentumED.jl:220,378): the sparse and dense methods return
+    Float32 energies and ComplexF32 vectors, the Float64 results rounded (the device computes
+    in complex FP64); the map ignores element_type, as the reference's map does."""
+    w = W.small()
+    op = w.operator()
+    full = _oracle_full(oracle_mod, op)
+    sp = me.ED_momentum_subspaces(w.para, 5)[0][0]
+    ev = np.linalg.eigvalsh(oracle_mod.dense_matrix(sp.list, full))[: w.nev]
+    for method in ("sparse", "dense"):
+        E64, V64 = me.EDsolve(sp, op, N=w.nev, method=method, method_info=False)
+        E32, V32 = me.EDsolve(sp, op, N=w.nev, method=method, element_type=np.float32,
+                              method_info=False)
+        assert E64.dtype == np.float64 and E32.dtype == np.float32, method
+        assert all(v.vec.dtype == np.complex64 for v in V32), method
+        np.testing.assert_array_equal(E32, E64.astype(np.float32))
+        for a, b in zip(V32, V64):
+            np.testing.assert_array_equal(a.vec, b.vec.astype(np.complex64))
+        np.testing.assert_allclose(E32, ev, rtol=1e-6, atol=1e-6)
+        E32s, _ = me.EDsolve(sp, op, N=w.nev, method=method, element_type="Float32",
+                             method_info=False)
+        np.testing.assert_array_equal(E32s, E32)
+    Em, Vm = me.EDsolve(sp, op, N=w.nev, method="map", element_type=np.float32,
+                        method_info=False)
+    assert Em.dtype == np.float64 and Vm[0].vec.dtype == np.complex128
+    np.testing.assert_allclose(Em, ev, rtol=1e-10, atol=1e-12)
+
+
 def test_edsolve_sparse_medium_against_arpack(gpu_ctx, oracle_mod):
     import scipy.sparse.linalg as spl
     w = W.medium()

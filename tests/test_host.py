@@ -274,6 +274,11 @@ def test_edsolve_argument_errors():
         me.EDsolve(sp, non_upper, N=1, method_info=False)
     with pytest.raises(ValueError):
         me.EDsolve(sp, [Scatter(1.0, 1, 1, bits=2)], N=1, method="map")
+    # element_type as the reference accepts it (sparse_matrix.jl:72): Float64 or Float32 only,
+    # refused before any device work
+    for bad in (int, np.complex128, "Float16"):
+        with pytest.raises(AssertionError, match="element_type"):
+            me.EDsolve(sp, op, N=1, element_type=bad, method_info=False)
 
 
 # ------------------------------------------------ the C-ABI library
