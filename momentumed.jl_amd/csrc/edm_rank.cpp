@@ -68,15 +68,24 @@ struct LowTables {
     std::unordered_map<QKey, uint32_t, QHash> cnt;
 };
 
+// Two tables of 2^(nbits/2) entries each (at 52 bits 2 x 256 MB, built on the host by a pass
+// over every value of each half) pay for themselves only beside a basis of comparable size:
+// they are built when they take at most 64 MB or at most four times the basis' 8 D bytes;
+// a small basis of a wide system keeps the bucketed binary search.
+static bool rank_tables_worth(int nbits, int64_t D)
+{
+    if (nbits < 2 || nbits > kMaxRankBits || D <= 0 || D >= (int64_t)0xFFFFFFFFLL) return false;
+    const int nlo = nbits / 2, nhi = nbits - nlo;
+    const uint64_t bytes = ((1ULL << nlo) + (1ULL << nhi)) * sizeof(uint32_t);
+    return bytes <= (64ULL << 20) || bytes <= 4ULL * 8ULL * (uint64_t)D;
+}
+
 bool rank_applicable(const edm_basis *b)
 {
     if (!b->has_spec || b->sector < 0) return false;
     const SystemSpec &sp = b->spec;
     if (sp.ncc > 8) return false;
-    if (b->D >= (int64_t)0xFFFFFFFFLL) return false;
-    // two tables of 2^(nbits/2) entries each: at 52 bits 2 x 256 MB; wider systems keep the
-    // bucketed binary search
-    return sp.nbits <= kMaxRankBits && sp.nbits >= 2;
+    return rank_tables_worth(sp.nbits, b->D);
 }
 
 LowTables low_tables(const SystemSpec &sp)
@@ -169,9 +178,16 @@ std::vector<bool> build_rank_tables_many(const std::vector<edm_basis *> &bs)
 {
     std::vector<bool> ok(bs.size(), false);
     if (bs.empty()) return ok;
-    // every basis here belongs to one system: J_lo once, J_hi per sector on host threads
-    for (const edm_basis *b : bs)
-        if (!rank_applicable(b) || b->spec.nbits != bs[0]->spec.nbits) return ok;
+    // every basis here belongs to one system: J_lo once, J_hi per sector on host threads, for
+    // the sectors whose tables are worth their size (rank_applicable)
+    std::vector<char> want(bs.size(), 0);
+    bool any = false;
+    for (size_t k = 0; k < bs.size(); ++k) {
+        if (bs[k]->spec.nbits != bs[0]->spec.nbits) return ok;
+        want[k] = rank_applicable(bs[k]) ? 1 : 0;
+        any = any || want[k];
+    }
+    if (!any) return ok;
     const SystemSpec &sp = bs[0]->spec;
     LowTables lo = low_tables(sp);
     std::vector<std::vector<uint32_t>> jhi(bs.size());
@@ -181,7 +197,7 @@ std::vector<bool> build_rank_tables_many(const std::vector<edm_basis *> &bs)
     std::atomic<size_t> next{0};
     auto work = [&]() {
         for (size_t k; (k = next.fetch_add(1)) < bs.size();)
-            good[k] = high_table(sp, lo, bs[k]->sector, bs[k]->D, jhi[k]);
+            if (want[k]) good[k] = high_table(sp, lo, bs[k]->sector, bs[k]->D, jhi[k]);
     };
     std::vector<std::thread> th;
     for (size_t t = 1; t < nth; ++t) th.emplace_back(work);
@@ -199,7 +215,7 @@ bool derive_rank_tables(edm_basis *b, const uint64_t *states)
 {
     const int nbits = b->nbits;
     const int64_t D = b->D;
-    if (nbits < 2 || nbits > kMaxRankBits || D == 0 || D >= (int64_t)0xFFFFFFFFLL) return false;
+    if (!rank_tables_worth(nbits, D)) return false;
     const int nlo = nbits / 2, nhi = nbits - nlo;
     const uint64_t lomask = (1ULL << nlo) - 1ULL;
     constexpr uint32_t kAbsent = 0xFFFFFFFFu;

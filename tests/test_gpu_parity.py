@@ -642,6 +642,17 @@ def test_literal_batched_sector_48_bits(gpu_ctx, oracle_mod):
     y2 = torch.empty_like(xd)
     dev.ROCmLinearMap(dop, b)(y2, xd)
     assert torch.equal(y2, yd)
+    # a small sector of the same 48-bit system (N_up = N_down = 1): tables of 2 x 2^24 words
+    # would dwarf its few states, so it keeps the bucketed search, with the same result as
+    # the oracle on every row
+    sys1 = _system(para, (1, 1))
+    b1 = dev.DeviceBasis.enumerate(gpu_ctx, sys1, 0)
+    assert b1.rank_mode == 0 and 0 < b1.D < 1000
+    basis1 = b1.download()
+    x1 = _rand(b1.D, 11)
+    y1 = dev.ROCmLinearMap(dop, b1)(x1)
+    yo1 = oracle_mod.matvec(basis1, full, x1)
+    assert np.abs(y1 - yo1).max() <= Y_TOL * np.abs(yo1).max()
 
 
 # ------------------------------------------------------------------ general eigensolver
