@@ -117,7 +117,9 @@ int edm_basis_device_ptr(const edm_basis *b, const uint64_t **states_dev);
  * system, or with one whose sector the list is not), which an operator uses only after a
  * device check that every target of every state under it lies in the basis, and binary
  * search otherwise. The reference's device extension uploads bare state lists
- * (MomentumEDCUDAExt.jl:94-113), so mode 2 is what its call sequence gets. */
+ * (MomentumEDCUDAExt.jl:94-113), so mode 2 is what its call sequence gets. Tables (2 x
+ * 2^(nbits/2) 32-bit words) are built for up to 52 bits when they take at most 64 MB or four
+ * times the basis' 8*D bytes; other bases search (mode 0). */
 int edm_basis_rank_mode(const edm_basis *b, int32_t *mode);
 /* Release the rank tables (mode 0 from then on). No reference counterpart: it lets a caller
  * compare the two lookups or give back the tables' memory. Not while a batch holds a
