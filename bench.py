@@ -36,21 +36,24 @@ def log(*a):
 
 
 def _kernel_name(rank_mode: int, block: int = 512) -> str:
-    """Name of the matvec kernel libedm.so launches for the bucketed path (EDM_MATVEC, or the
-    tuner's code: rows per workgroup | 0x10000 for the LDS-amplitude kernel | lanes per row
-    << 20 for the split-row kernel)."""
+    """Name of the matvec kernel libedm.so launches for the bucketed path, from the code
+    edm_matvec_kernel reports (rows per workgroup | 0x10000 for the LDS-amplitude kernel |
+    0x20000 for the LDS rank-table kernel | lanes per row << 20 for the split-row kernel) and
+    EDM_MATVEC."""
     rank = "RankLin" if rank_mode == 1 else "RankSearch"
     v = os.environ.get("EDM_MATVEC", "u1")
+    if block & 0x20000:
+        return f"k_matvec_narrow<{block & 0xFFFF}>"
     if v == "u1" and (block >> 20) > 1:
         return f"k_matvec_split<{rank}, {(block >> 20) & 0xFF}>"
     if v == "u1a" or (v == "u1" and block & 0x10000):
         return f"k_matvec_ldsa<{rank}, {block & 0xFFFF if block & 0x10000 else 1024}>"
-    if v == "basic":
+    if v == "basic" or block == 0:
         return f"k_matvec_fast<{rank}>"
     if v in ("hi", "lo"):
         return f"k_matvec_lds<{rank}, {str(v == 'hi').lower()}>"
     u = {"u2": 2, "u4": 4}.get(v, 1)
-    return f"k_matvec_ldsu<{rank}, true, {u}, {block if u == 1 else 512}>"
+    return f"k_matvec_ldsu<{rank}, true, {u}, {block & 0xFFFF if u == 1 else 512}>"
 
 
 def cpu_share():
@@ -512,11 +515,12 @@ def main():
     torch.cuda.synchronize()
     lm = dev.ROCmLinearMap(dop, basis)
     if args.no_tune:
-        block, tune_ms = 512, None
+        tune_ms = None
     else:
-        block, tune_ms = lm.tune()
-        log(f"[bench] tuner: kernel {block:#x} ({tune_ms} ms per matvec)")
-        block = block or 512
+        _, tune_ms = lm.tune()
+    block = lm.kernel()
+    log(f"[bench] matvec kernel {block:#x} ({_kernel_name(basis.rank_mode, block)}); tuner: "
+        f"{tune_ms} ms per matvec")
 
     stream = torch.cuda.ExternalStream(ctx.stream, device="cuda:0")
     for _ in range(args.warmup):
@@ -566,6 +570,7 @@ def main():
                    "sector": list(w.sector), "D": D, "S_full": S, "nnz_offdiag": nnz_off.value,
                    "parallelism": "single-gpu", "matvec_workgroup": block & 0xFFFF,
                    "matvec_amplitudes_in_lds": bool(block & 0x10000),
+                   "matvec_rank_low_table_in_lds": bool(block & 0x20000),
                    "tuner_ms": tune_ms},
         "roofline": roofline,
         "hbm_gbps_algorithmic": round(achieved, 1),

@@ -185,21 +185,30 @@ int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, 
 /* Autotuner (no reference counterpart; the reference has one GPU kernel configuration): times
  * on this operator and basis (three launches each, on scratch vectors) the default matvec
  * kernel with 256-, 512- and 1024-row workgroups, the variant that also stages the (distinct)
- * amplitudes in LDS at the same three sizes when they fit, and the split-row variant with 2,
- * 4, 8 and 16 lanes per row (512-lane workgroups), and uses the fastest for every later
- * edm_matvec / edm_eigsolve on the pair. The per-row arithmetic and its order are the same in
- * all ten, so y does not change. *kernel = chosen code: rows per workgroup,
- * | EDM_TUNE_AMP_LDS for the LDS-amplitude variant, | lanes << EDM_TUNE_SPLIT_SHIFT for the
- * split-row variant (0 when the pair runs another kernel); ms (may be NULL) =
- * ms[EDM_TUNE_CANDIDATES], the mean milliseconds per launch of 256, 512, 1024, LDS-amplitude
- * 256, 512, 1024, split 2, 4, 8, 16 (-1: not run). A pair that was not tuned uses one lane
- * per row when D >= 2^18 and otherwise the smallest power of two of lanes (at most 4) that
- * brings D * lanes to 2^18. */
+ * amplitudes in LDS at the same three sizes when they fit, the split-row variant with 2, 4, 8
+ * and 16 lanes per row (512-lane workgroups), and, when the basis has a rank split whose low
+ * table (at most 2^14 16-bit entries) fits in LDS beside the term image, the variant that
+ * reads the low-half rank table from LDS with 512- and 1024-row workgroups; it uses the
+ * fastest for every later edm_matvec / edm_eigsolve on the pair. The per-row arithmetic and
+ * its order are the same in all twelve, so y does not change. *kernel = chosen code: rows per
+ * workgroup, | EDM_TUNE_AMP_LDS for the LDS-amplitude variant, | EDM_TUNE_NARROW for the
+ * LDS rank-table variant, | lanes << EDM_TUNE_SPLIT_SHIFT for the split-row variant (0 when
+ * the pair runs another kernel); ms (may be NULL) = ms[EDM_TUNE_CANDIDATES], the mean
+ * milliseconds per launch of 256, 512, 1024, LDS-amplitude 256, 512, 1024, split 2, 4, 8, 16,
+ * LDS-rank 512, 1024 (-1: not run). A pair that was not tuned uses the LDS-rank variant with
+ * 1024-row workgroups when D >= 2^18 and the basis has such a split, else one lane per row
+ * when D >= 2^18, and otherwise the smallest power of two of lanes (at most 4) that brings
+ * D * lanes to 2^18. */
 #define EDM_TUNE_AMP_LDS 0x10000
+#define EDM_TUNE_NARROW 0x20000
 #define EDM_TUNE_SPLIT_SHIFT 20
-#define EDM_TUNE_CANDIDATES 10
+#define EDM_TUNE_CANDIDATES 12
 int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel,
                     double *ms);
+/* The kernel code (as edm_matvec_tune reports it) that edm_matvec(op, b) launches now: the
+ * tuner's pick or the size-based default, 0 when the pair runs the global-table kernels. No
+ * reference counterpart: it names the measured kernel. */
+int edm_matvec_kernel(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel);
 
 /* ----------------------------------------------------------------- eigensolve (a9-a10) */
 /* krylov_map_solve(H::AbstractGPULinearMap, N; ishermitian=true, vec0, krylovdim, tol,

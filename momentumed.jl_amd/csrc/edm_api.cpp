@@ -38,6 +38,7 @@ void free_basis(edm_basis *b)
     if (b->d_jlo) ctx->free(b->d_jlo, (1ULL << b->nlo) * sizeof(uint32_t));
     if (b->d_jhi) ctx->free(b->d_jhi, (1ULL << b->nhi) * sizeof(uint32_t));
     if (b->d_jlo16) ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
+    free_narrow_tables(b);
     if (b->d_start) ctx->free(b->d_start, (b->nbuckets + 1) * sizeof(uint32_t));
     delete b;
 }
@@ -49,6 +50,7 @@ void drop_tables(edm_basis *b)
     if (b->d_jlo16) b->ctx->free(b->d_jlo16, (1ULL << b->nlo) * sizeof(uint16_t));
     b->d_jlo = b->d_jhi = nullptr;
     b->d_jlo16 = nullptr;
+    free_narrow_tables(b);
     b->rank_mode = 0;
     b->rank_derived = false;
 }
@@ -56,10 +58,19 @@ void drop_tables(edm_basis *b)
 // keep freshly built rank tables only when they reproduce every state's position
 void verify_or_drop(edm_basis *b)
 {
-    if (verify_rank_tables(b))
+    if (verify_rank_tables(b)) {
         b->rank_mode = 1;
-    else
+        // the LDS low-table split is an optimisation: a budget that cannot hold it keeps the
+        // tables above
+        try {
+            build_narrow_tables(b);
+        } catch (const EdmError &e) {
+            if (e.code != EDM_ENOMEM) throw;
+            free_narrow_tables(b);
+        }
+    } else {
         drop_tables(b);
+    }
 }
 
 // the binary search's bucket index for a basis whose lookups may search (no system rank
@@ -759,6 +770,16 @@ int edm_matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t 
         *kernel = matvec_tune(ctx, op, b, t);
         if (ms)
             for (int k = 0; k < EDM_TUNE_CANDIDATES; ++k) ms[k] = t[k];
+        return EDM_OK;
+    });
+}
+
+int edm_matvec_kernel(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int32_t *kernel)
+{
+    return guarded(ctx, [&]() -> int {
+        EDM_REQUIRE(ctx && op && b && kernel, EDM_EINVAL, "NULL argument");
+        EDM_REQUIRE(op->ctx == ctx && b->ctx == ctx, EDM_EINVAL, "handles from another context");
+        *kernel = matvec_kernel_code(op, b);
         return EDM_OK;
     });
 }

@@ -101,6 +101,12 @@ struct edm_basis {
     int nlo = 0, nhi = 0;
     uint32_t *d_jlo = nullptr, *d_jhi = nullptr;
     uint16_t *d_jlo16 = nullptr;   // 16-bit copy of d_jlo when every entry fits
+    // the same rank split at bit kNarrowLo instead of nbits/2 (built beside rank tables whose
+    // low half is wider than that): J_lo then has 2^kNarrowLo 16-bit entries, small enough to
+    // be staged in LDS by the matvec, and J_hi 2^(nbits - kNarrowLo) entries
+    uint32_t *d_jhi_n = nullptr;
+    uint16_t *d_jlo_n16 = nullptr;
+    int nhi_n = 0;
     bool has_spec = false;
     SystemSpec spec;
     int64_t sector = -1;
@@ -165,8 +171,8 @@ struct edm_op {
     // cached op_conserves() results keyed by basis id
     mutable std::vector<std::pair<uint64_t, bool>> conserve_cache;
     // matvec kernel per basis id chosen by matvec_tune: rows per workgroup (default 512),
-    // | EDM_TUNE_AMP_LDS when the amplitudes are staged in LDS, | lanes per row <<
-    // EDM_TUNE_SPLIT_SHIFT for the split-row kernel
+    // | EDM_TUNE_AMP_LDS when the amplitudes are staged in LDS, | EDM_TUNE_NARROW when the
+    // low-half rank table is, | lanes per row << EDM_TUNE_SPLIT_SHIFT for the split-row kernel
     mutable std::vector<std::pair<uint64_t, int>> tuned_block;
     int nbits = 0;
     bool upper = false;
@@ -236,6 +242,8 @@ bool derive_rank_tables(edm_basis *b, const uint64_t *states_host);
 enum class ApplyMode { Forward, Adjoint };
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
                    double2 *y, ApplyMode mode);
+// the kernel code edm_matvec(op, b) launches (edm.h edm_matvec_kernel)
+int matvec_kernel_code(const edm_op *op, const edm_basis *b);
 // y_v = H x_v for v < nv. When the operator has an LDS term image, x_0..x_{nv-1} are first
 // interleaved into xi (nv * D complex, caller-provided) and each group of up to 8 vectors is
 // one launch of the block kernel; otherwise nv single matvecs (xi unused). Column v equals
@@ -255,6 +263,14 @@ edm_op *build_op(edm_ctx *ctx, int nbits, const double *amp, const uint64_t *out
                  const uint64_t *in, int64_t S, bool upper);
 void destroy_op(edm_op *op);
 bool verify_rank_tables(edm_basis *b);
+// rank split of the LDS low-table matvec: J_lo over the lowest kNarrowLo bits
+constexpr int kNarrowLo = 14;
+// For a basis with verified rank tables whose low half is wider than kNarrowLo bits: derive
+// the kNarrowLo split from the state list on the device (J_hi_n[h] = first state with high
+// part h, J_lo_n[l] = position of l in its block), check it on every state and keep it, or
+// free it when a check fails. High parts wider than 24 bits (J_hi_n over 64 MB) get none.
+void build_narrow_tables(edm_basis *b);
+void free_narrow_tables(edm_basis *b);
 // the bucket index of the binary search (a no-op for small bases or when it exists); every
 // basis gets one when it is created (edm_matvec.hip)
 void build_search_index(edm_basis *b);

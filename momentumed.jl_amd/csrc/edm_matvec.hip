@@ -584,6 +584,56 @@ struct NoMid {};
 __device__ __forceinline__ NoMid rank_mid32(const RankSearch &, uint64_t) { return NoMid{}; }
 __device__ __forceinline__ int64_t rank_find(const RankSearch &r, NoMid, uint64_t t) { return r(t); }
 
+// Rank tables split at bit kNarrowLo with J_lo (16-bit, 2^kNarrowLo entries) staged in LDS:
+// the low-half lookup, which lanes walking different rows make at scattered addresses (about
+// half of the L2 requests of the global-table walk on the 6x6 N=12 sector), becomes an LDS
+// read; only J_hi stays in global memory, and lanes that walk the same key of rows sharing
+// their high bits mostly read the same J_hi word. sjlo is set inside the kernel, after
+// staging. A partner that shares its high part with mid or with the row takes their word.
+struct RankNarrow {
+    const uint32_t *jhi;
+    const uint16_t *sjlo;
+    int nlo;
+    uint32_t lomask;
+};
+template <> struct RankMayMiss<RankNarrow> { static constexpr bool value = false; };
+struct NarrowRow {
+    const uint32_t *jhi;
+    const uint16_t *sjlo;
+    int nlo;
+    uint32_t lomask, hs, jhs;
+};
+struct NarrowMid {
+    uint32_t hm, jhm;
+};
+__device__ __forceinline__ NarrowRow rank_row32(const RankNarrow &r, uint64_t s)
+{
+    const uint32_t hs = (uint32_t)(s >> r.nlo);
+    return NarrowRow{r.jhi, r.sjlo, r.nlo, r.lomask, hs, r.jhi[hs]};
+}
+__device__ __forceinline__ NarrowMid rank_mid32(const NarrowRow &r, uint64_t mid)
+{
+    const uint32_t hm = (uint32_t)(mid >> r.nlo);
+    return NarrowMid{hm, hm == r.hs ? r.jhs : r.jhi[hm]};
+}
+__device__ __forceinline__ int64_t rank_find(const NarrowRow &r, const NarrowMid &m, uint64_t t)
+{
+    const uint32_t h = (uint32_t)(t >> r.nlo);
+    const uint32_t hv = h == m.hm ? m.jhm : (h == r.hs ? r.jhs : r.jhi[h]);
+    return (int64_t)(hv + (uint32_t)r.sjlo[(uint32_t)t & r.lomask]);
+}
+// the same tables read from global memory (checks of the split, on every basis state)
+struct RankNarrowGlobal {
+    const uint32_t *jhi;
+    const uint16_t *jlo;
+    int nlo;
+    uint64_t lomask;
+    __device__ __forceinline__ int64_t operator()(uint64_t t) const
+    {
+        return (int64_t)jhi[t >> nlo] + (int64_t)jlo[t & lomask];
+    }
+};
+
 // One row, one candidate at a time: pair buckets (key >= 64) hold only two-body entries and
 // single-orbital buckets only one-body entries, so each bucket kind has its own loop; the
 // per-bucket part of the fermion sign is hoisted; the sign is folded into the amplitude.
@@ -955,6 +1005,71 @@ k_matvec_ldsa(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
     if (i >= D) return;
     y[i] = row_one<Rank, true, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
                                      INT64_MAX, 1, false);
+}
+
+// k_matvec_ldsu's row walk (row_one, same order and arithmetic, so the same y bit for bit)
+// with the partner lookup of RankNarrow: the workgroup stages the term image and, 16-byte
+// aligned after it, the 2^kNarrowLo-entry J_lo. BLOCK = 1024 keeps the occupancy of the
+// 512-row kernel (two workgroups of the ~67 KiB image per CU on the 6x6 N=12 sector).
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, RankNarrow rank,
+                const uint16_t *__restrict__ jlo, const double2 *__restrict__ x,
+                double2 *__restrict__ y)
+{
+    extern __shared__ uint4 lds_u4[];
+    stage_image(t, lds_u4);
+    const int off = (t.lds_bytes + 15) & ~15;
+    {
+        const uint4 *g = (const uint4 *)jlo;
+        uint4 *d = (uint4 *)((unsigned char *)lds_u4 + off);
+        const int n16 = (int)(((size_t)rank.lomask + 1) * sizeof(uint16_t) / 16);
+        for (int k = threadIdx.x; k < n16; k += BLOCK) d[k] = g[k];
+    }
+    __syncthreads();
+    rank.sjlo = (const uint16_t *)((const unsigned char *)lds_u4 + off);
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= D) return;
+    y[i] = row_one<RankNarrow, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
+                                     INT64_MAX, 1, false);
+}
+
+// J_hi_n[h] = index of the first state whose high part (bits >= nlo) is h (entries of absent
+// high parts stay as filled)
+__global__ void __launch_bounds__(256)
+k_narrow_hi(int64_t D, const uint64_t *__restrict__ states, int nlo, uint32_t *__restrict__ jhi)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint64_t h = states[i] >> nlo;
+    if (i == 0 || (states[i - 1] >> nlo) != h) jhi[h] = (uint32_t)i;
+}
+
+// J_lo_n[l] = position of state i in its block, for l its low part. In a basis with the
+// two-table structure every block that holds l puts it at the same position, so concurrent
+// writers store the same value; otherwise the check on every state that follows fails.
+// Positions above 65535 are counted in *bad.
+__global__ void __launch_bounds__(256)
+k_narrow_lo(int64_t D, const uint64_t *__restrict__ states, int nlo,
+            const uint32_t *__restrict__ jhi, uint16_t *__restrict__ jlo,
+            unsigned int *__restrict__ bad)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    const uint64_t s = states[i];
+    const int64_t pos = i - (int64_t)jhi[s >> nlo];
+    if (pos < 0 || pos > 0xFFFF) {
+        atomicAdd(bad, 1u);
+        return;
+    }
+    jlo[s & ((1ULL << nlo) - 1ULL)] = (uint16_t)pos;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *__restrict__ p, int64_t n, uint32_t v)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = v;
 }
 
 // ---------------------------------------------------------------- block matvec
@@ -1617,6 +1732,58 @@ bool verify_rank_tables(edm_basis *b)
     return bad == 0;
 }
 
+void free_narrow_tables(edm_basis *b)
+{
+    if (b->d_jhi_n) b->ctx->free(b->d_jhi_n, (1ULL << b->nhi_n) * sizeof(uint32_t));
+    if (b->d_jlo_n16) b->ctx->free(b->d_jlo_n16, (1ULL << kNarrowLo) * sizeof(uint16_t));
+    b->d_jhi_n = nullptr;
+    b->d_jlo_n16 = nullptr;
+    b->nhi_n = 0;
+}
+
+void build_narrow_tables(edm_basis *b)
+{
+    free_narrow_tables(b);
+    const int nhi = b->nbits - kNarrowLo;
+    // only where the tables split wider than kNarrowLo, J_hi_n stays within 64 MB and within
+    // the basis' own 8 D bytes (a small sector of a wide system keeps its tables)
+    if (b->rank_mode != 1 || b->nlo <= kNarrowLo || nhi < 1 || nhi > 24 || b->D <= 0 ||
+        (4ULL << nhi) > 8ULL * (uint64_t)b->D)
+        return;
+    edm_ctx *ctx = b->ctx;
+    b->nhi_n = nhi;
+    b->d_jhi_n = (uint32_t *)ctx->alloc((1ULL << nhi) * sizeof(uint32_t));
+    b->d_jlo_n16 = (uint16_t *)ctx->alloc((1ULL << kNarrowLo) * sizeof(uint16_t));
+    const int64_t nh = (int64_t)1 << nhi, nl32 = ((int64_t)1 << kNarrowLo) / 2;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<int64_t>((nh + 255) / 256, 65536)),
+                       dim3(256), 0, ctx->stream, b->d_jhi_n, nh, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nl32 + 255) / 256)), dim3(256), 0,
+                       ctx->stream, (uint32_t *)b->d_jlo_n16, nl32, 0u);
+    hipLaunchKernelGGL(k_narrow_hi, dim3(nblk(b->D)), dim3(256), 0, ctx->stream, b->D,
+                       b->d_states, kNarrowLo, b->d_jhi_n);
+    unsigned int *d_bad = (unsigned int *)ctx->alloc(sizeof(unsigned int));
+    unsigned int bad = 0;
+    try {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, d_bad, 1, 0u);
+        hipLaunchKernelGGL(k_narrow_lo, dim3(nblk(b->D)), dim3(256), 0, ctx->stream, b->D,
+                           b->d_states, kNarrowLo, b->d_jhi_n, b->d_jlo_n16, d_bad);
+        const RankNarrowGlobal chk{b->d_jhi_n, b->d_jlo_n16, kNarrowLo,
+                                   (1ULL << kNarrowLo) - 1ULL};
+        hipLaunchKernelGGL(k_verify_rank<RankNarrowGlobal>, dim3(nblk(b->D)), dim3(kRowBlock), 0,
+                           ctx->stream, b->D, b->d_states, chk, d_bad);
+        EDM_HIP_CHECK(hipGetLastError());
+        EDM_HIP_CHECK(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->free(d_bad, sizeof(unsigned int));
+        throw;
+    }
+    ctx->free(d_bad, sizeof(unsigned int));
+    if (bad != 0) free_narrow_tables(b);
+}
+
 // Kernel selection for the bucketed path (all variants compute the same y; the choice only
 // changes speed). Default: the LDS-staged kernel with per-row rank view, one candidate at a
 // time (u1) — the fastest measured on the 6x6 N=12 sector. EDM_MATVEC=u2|u4 resolves 2|4
@@ -1637,6 +1804,8 @@ static int matvec_variant()
     if (!strcmp(v, "u1a")) return 13;
     if (!strcmp(v, "u2")) return 12;
     if (!strcmp(v, "u4")) return 14;
+    if (!strcmp(v, "u1n")) return 15;
+    if (!strcmp(v, "u1n512")) return 16;
     return 11;
 }
 
@@ -1758,11 +1927,76 @@ static void launch_split(edm_ctx *ctx, const edm_basis *b, const CompactTable &c
 // of two, at most 4) that bring the threads to 2^18. Measured: 174 k rows run fastest with
 // 2 lanes (4 and more are slower), rows of 1 k - 20 k with 4 (8 and 16 within 5 %). The
 // choice only changes speed: y is bitwise the same.
-static int default_kernel_code(int64_t D)
+static int default_kernel_code(int64_t D, bool narrow)
 {
+    if (narrow && D >= ((int64_t)1 << 18)) return 1024 | EDM_TUNE_NARROW;
     int lanes = 1;
     while (lanes < 4 && D * lanes < ((int64_t)1 << 18)) lanes *= 2;
     return lanes == 1 ? kLdsBlock : (kLdsBlock | (lanes << EDM_TUNE_SPLIT_SHIFT));
+}
+
+// The RankNarrow view of b's rank tables: the kNarrowLo split when it was built, or the
+// tables themselves when their low half is at most kNarrowLo bits wide and has a 16-bit copy.
+static bool narrow_view(const edm_basis *b, RankNarrow &r, const uint16_t *&jlo)
+{
+    if (b->rank_mode != 1) return false;
+    if (b->d_jlo_n16 && b->d_jhi_n) {
+        r = RankNarrow{b->d_jhi_n, nullptr, kNarrowLo, (1u << kNarrowLo) - 1u};
+        jlo = b->d_jlo_n16;
+        return true;
+    }
+    if (b->d_jlo16 && b->d_jhi && b->nlo >= 3 && b->nlo <= kNarrowLo && b->nhi <= 32) {
+        r = RankNarrow{b->d_jhi, nullptr, b->nlo, (1u << b->nlo) - 1u};
+        jlo = b->d_jlo16;
+        return true;
+    }
+    return false;
+}
+
+static size_t narrow_lds_bytes(const CompactTable &ct, const RankNarrow &r)
+{
+    return (((size_t)ct.lds_bytes + 15) & ~(size_t)15) + ((size_t)r.lomask + 1) * sizeof(uint16_t);
+}
+
+// whether the LDS low-table kernel can run the pair (rank tables usable for the operator, a
+// narrow view, and image + J_lo within the LDS of a CU)
+static bool narrow_ok(const edm_op *op, const edm_basis *b, const CompactTable &ct)
+{
+    RankNarrow r;
+    const uint16_t *jlo;
+    return ct.ok && use_lin(op, b) && narrow_view(b, r, jlo) &&
+           narrow_lds_bytes(ct, r) <= kLdsMaxBytes;
+}
+
+template <int BLOCK>
+static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
+                                const double2 *x, double2 *y)
+{
+    RankNarrow r;
+    const uint16_t *jlo = nullptr;
+    EDM_REQUIRE(narrow_view(b, r, jlo), EDM_EINVAL, "basis has no LDS rank split");
+    const size_t lds = narrow_lds_bytes(ct, r);
+    EDM_REQUIRE(lds <= kLdsMaxBytes, EDM_EINVAL, "term image and rank table exceed the LDS");
+    if (lds > 64 * 1024) {   // per-kernel opt-in beyond 64 KiB, once per device
+        static std::atomic<uint64_t> attr_set{0};
+        const uint64_t bit = 1ULL << (ctx->device & 63);
+        if (!(attr_set.load() & bit)) {
+            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_narrow<BLOCK>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kLdsMaxBytes));
+            attr_set.fetch_or(bit);
+        }
+    }
+    const unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL((k_matvec_narrow<BLOCK>), dim3(nb), dim3(BLOCK), lds, ctx->stream, b->D,
+                       b->d_states, compact_dev(ct), r, jlo, x, y);
+}
+
+static void launch_narrow(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
+                          const double2 *x, double2 *y, int block)
+{
+    if (block == 512) launch_narrow_block<512>(ctx, b, ct, x, y);
+    else launch_narrow_block<1024>(ctx, b, ct, x, y);
 }
 
 template <class Rank>
@@ -1801,6 +2035,34 @@ static void launch_compact(edm_ctx *ctx, const edm_basis *b, const CompactTable 
                            ctx->stream, b->D, b->d_states, cd, rank, x, y, swz);
 }
 
+// The bucketed-path kernel code a matvec of (op, b) runs with the given table and variant:
+// rows per workgroup | EDM_TUNE_NARROW (LDS rank-table kernel) | EDM_TUNE_AMP_LDS |
+// lanes << EDM_TUNE_SPLIT_SHIFT; the tuner's pick (forward direction) over the size-based
+// default, EDM_MATVEC_LANES over both, EDM_MATVEC=u1n / u1n512 forcing the LDS rank-table
+// kernel where it can run.
+static int bucketed_code(const edm_op *op, const edm_basis *b, ApplyMode mode,
+                         const CompactTable &ct, int variant)
+{
+    const bool nok = narrow_ok(op, b, ct);
+    if (variant == 15 || variant == 16)
+        return nok ? ((variant == 16 ? 512 : 1024) | EDM_TUNE_NARROW) : kLdsBlock;
+    int block = default_kernel_code(b->D, nok);
+    for (auto &kv : op->tuned_block)
+        if (mode == ApplyMode::Forward && kv.first == b->id) block = kv.second;
+    if (const int lanes = forced_lanes())
+        block = kLdsBlock | (lanes > 1 ? lanes << EDM_TUNE_SPLIT_SHIFT : 0);
+    if ((block & EDM_TUNE_NARROW) && (!nok || variant != 11)) block = kLdsBlock;
+    return block;
+}
+
+int matvec_kernel_code(const edm_op *op, const edm_basis *b)
+{
+    if (!op->fast || !op->cfwd.ok || b->D == 0) return 0;
+    const int variant = matvec_variant();
+    if (variant == 0) return 0;
+    return bucketed_code(op, b, ApplyMode::Forward, op->cfwd, variant);
+}
+
 void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const double2 *x,
                    double2 *y, ApplyMode mode)
 {
@@ -1810,15 +2072,15 @@ void launch_matvec(edm_ctx *ctx, const edm_op *op, const edm_basis *b, const dou
         const CompactTable &ct = (mode == ApplyMode::Adjoint && !op->upper) ? op->cadj : op->cfwd;
         int variant = matvec_variant();
         if (ct.ok && variant != 0) {
-            int block = default_kernel_code(b->D);
-            for (auto &kv : op->tuned_block)
-                if (mode == ApplyMode::Forward && kv.first == b->id) block = kv.second;
-            if (const int lanes = forced_lanes())
-                block = kLdsBlock | (lanes > 1 ? lanes << EDM_TUNE_SPLIT_SHIFT : 0);
-            if (use_lin(op, b))
-                launch_compact(ctx, b, ct, make_lin(b), x, y, variant, block);
+            const int block = bucketed_code(op, b, mode, ct, variant);
+            if (block & EDM_TUNE_NARROW)
+                launch_narrow(ctx, b, ct, x, y, block & 0xFFFF);
+            else if (use_lin(op, b))
+                launch_compact(ctx, b, ct, make_lin(b), x, y, variant >= 15 ? 11 : variant,
+                               block);
             else
-                launch_compact(ctx, b, ct, make_search(b), x, y, variant, block);
+                launch_compact(ctx, b, ct, make_search(b), x, y, variant >= 15 ? 11 : variant,
+                               block);
             EDM_HIP_CHECK(hipGetLastError());
             return;
         }
@@ -2054,14 +2316,19 @@ int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms)
         EDM_HIP_CHECK(hipEventCreate(&e0));
         EDM_HIP_CHECK(hipEventCreate(&e1));
         const int blocks[3] = {256, 512, 1024};
+        const bool nok = narrow_ok(op, b, op->cfwd);
         double bestms = 1e300;
         for (int k = 0; k < EDM_TUNE_CANDIDATES; ++k) {
             const bool amp_lds = k >= 3 && k < 6;
             if (amp_lds && op->cfwd.lds_full == 0) continue;
-            const int code = k < 6 ? (blocks[k % 3] | (amp_lds ? EDM_TUNE_AMP_LDS : 0))
-                                   : (kLdsBlock | ((2 << (k - 6)) << EDM_TUNE_SPLIT_SHIFT));
+            if (k >= 10 && !nok) continue;
+            const int code = k < 6    ? (blocks[k % 3] | (amp_lds ? EDM_TUNE_AMP_LDS : 0))
+                             : k < 10 ? (kLdsBlock | ((2 << (k - 6)) << EDM_TUNE_SPLIT_SHIFT))
+                                      : ((k == 10 ? 512 : 1024) | EDM_TUNE_NARROW);
             auto run = [&]() {
-                if (use_lin(op, b))
+                if (code & EDM_TUNE_NARROW)
+                    launch_narrow(ctx, b, op->cfwd, x, y, code & 0xFFFF);
+                else if (use_lin(op, b))
                     launch_compact(ctx, b, op->cfwd, make_lin(b), x, y, 11, code);
                 else
                     launch_compact(ctx, b, op->cfwd, make_search(b), x, y, 11, code);

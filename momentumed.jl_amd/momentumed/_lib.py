@@ -82,6 +82,7 @@ _SIGS = [
     ("edm_batch_set_weight", c_i32, [c_vp, c_i64, ctypes.c_double]),
     ("edm_batch_problem_bytes", c_i32, [c_i64, c_i32, c_i32, P(c_u64)]),
     ("edm_matvec_tune", c_i32, [c_vp, c_vp, c_vp, P(c_i32), c_vp]),
+    ("edm_matvec_kernel", c_i32, [c_vp, c_vp, c_vp, P(c_i32)]),
     ("edm_matvec_between", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("edm_bracket2", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("edm_bracket2_noalloc", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -131,8 +131,9 @@ def system_struct(k_list, Gk, Nc_mix, Nc_conserve, N_each):
 
 # ------------------------------------------------------------------------- basis
 TUNE_AMP_LDS = 0x10000   # edm.h EDM_TUNE_AMP_LDS
+TUNE_NARROW = 0x20000    # edm.h EDM_TUNE_NARROW
 TUNE_SPLIT_SHIFT = 20    # edm.h EDM_TUNE_SPLIT_SHIFT
-TUNE_CANDIDATES = 10     # edm.h EDM_TUNE_CANDIDATES
+TUNE_CANDIDATES = 12     # edm.h EDM_TUNE_CANDIDATES
 
 
 class DeviceBasis:
@@ -280,15 +281,24 @@ class ROCmLinearMap:
         """Pick the matvec kernel for this operator and basis by timing (edm.h
         edm_matvec_tune); returns (chosen kernel code or 0, {candidate: ms per matvec}).
         The code is the rows per workgroup, | TUNE_AMP_LDS when the amplitudes are staged in
-        LDS, | lanes << TUNE_SPLIT_SHIFT for the split-row kernel; candidates are named 256,
-        512, 1024, "lds256", "lds512", "lds1024", "split2", "split4", "split8", "split16"."""
+        LDS, | TUNE_NARROW when the low-half rank table is, | lanes << TUNE_SPLIT_SHIFT for the
+        split-row kernel; candidates are named 256, 512, 1024, "lds256", "lds512", "lds1024",
+        "split2", "split4", "split8", "split16", "rank512", "rank1024"."""
         code = ctypes.c_int32()
         ms = (ctypes.c_double * TUNE_CANDIDATES)()
         check(self.ctx.L.edm_matvec_tune(self.ctx.h, self.op.h, self.basis.h,
                                          ctypes.byref(code), ms), self.ctx.h)
         names = [256, 512, 1024, "lds256", "lds512", "lds1024", "split2", "split4", "split8",
-                 "split16"]
+                 "split16", "rank512", "rank1024"]
         return code.value, {n: ms[k] for k, n in enumerate(names) if ms[k] >= 0}
+
+    def kernel(self) -> int:
+        """The kernel code edm_matvec launches for this operator and basis now (edm.h
+        edm_matvec_kernel; same encoding as tune(), 0 for the global-table kernels)."""
+        code = ctypes.c_int32()
+        check(self.ctx.L.edm_matvec_kernel(self.ctx.h, self.op.h, self.basis.h,
+                                           ctypes.byref(code)), self.ctx.h)
+        return code.value
 
     def matvec_device(self, y_ptr: int, x_ptr: int):
         """Stream-ordered y = H x on raw device pointers (no synchronisation)."""

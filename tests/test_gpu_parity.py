@@ -264,7 +264,9 @@ def test_matvec_tune_keeps_y(gpu_ctx, oracle_mod):
     blk, ms = H.tune()
     print(f"[tune] medium: chose {blk:#x}, ms {ms}", flush=True)
     assert blk & 0xFFFF in (256, 512, 1024) and all(v > 0 for v in ms.values())
-    assert len(ms) == 10    # the medium operator's amplitudes fit in LDS
+    # the medium operator's amplitudes fit in LDS, and so does its 13-bit low rank table
+    assert len(ms) == 12
+    assert H.kernel() == blk
     np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, dop, b, x), y0)
     import os
     os.environ["EDM_MATVEC"] = "u1a"
@@ -325,6 +327,56 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
             np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, d, basis, x), y1,
                                           err_msg=f"{name} lanes={lanes}")
         print(f"[split] {name}: D={basis.D} bitwise equal for 2..16 lanes", flush=True)
+
+
+def test_matvec_lds_rank_table_bitwise(gpu_ctx, oracle_mod, monkeypatch):
+    """The kernel that stages the low-half rank table in LDS (k_matvec_narrow) walks each row
+    as the global-table kernel does and looks partners up through the same kind of split:
+    its y is bitwise the one-lane kernel's (1024- and 512-row workgroups), and the one-lane y
+    is the oracle's to 1e-13. Cases: rank tables whose low half is already at most 14 bits
+    (3x5 N=5, 3x9 N=9, two conserved components on 3x4 meshes), used as they are, and a
+    30-orbital sector (5x6 mesh, N=7; 15 + 15 bit tables) whose 14 + 16 bit split the library
+    derives from the state list on the device and checks on every state."""
+    cases = []
+    ws = W.small()
+    dops = dev.DeviceOperator.from_mboperator(gpu_ctx, ws.operator(), ws.para.bits)
+    cases.append(("small s3", ws.operator(), dops,
+                  dev.DeviceBasis.enumerate(gpu_ctx, _system(ws.para, ws.N_each), 3)))
+    w = W.medium()
+    dopm = dev.DeviceOperator.from_mboperator(gpu_ctx, w.operator(), w.para.bits)
+    cases.append(("medium", w.operator(), dopm,
+                  dev.DeviceBasis.enumerate(gpu_ctx, _system(w.para, w.N_each), 0)))
+    para = W.batched_para(0.7)
+    op2 = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
+    dop2 = dev.DeviceOperator.from_mboperator(gpu_ctx, op2, para.bits)
+    cases.append(("two-component", op2, dop2,
+                  dev.DeviceBasis.enumerate(gpu_ctx, _system(para, (3, 4)), 5)))
+    k56 = W.mesh(5, 6)
+    para56 = me.EDPara(k_list=k56, Gk=(5, 6), H_two=W.CoulombGaussian(k56, (5, 6), c=0.25),
+                       two_momentum_coordinate=False)
+    op56 = me.MBOperator(me.ED_scatterlist_twobody(para56), upper_hermitian=True)
+    dop56 = dev.DeviceOperator.from_mboperator(gpu_ctx, op56, para56.bits)
+    b56 = dev.DeviceBasis.enumerate(gpu_ctx, _system(para56, (7,)), 0)
+    assert b56.rank_mode == 1 and b56.D > 50000
+    cases.append(("5x6 N=7", op56, dop56, b56))
+    for name, op, d, basis in cases:
+        x = _rand(basis.D, 93)
+        H = dev.ROCmLinearMap(d, basis)
+        monkeypatch.setenv("EDM_MATVEC", "u1")
+        monkeypatch.setenv("EDM_MATVEC_LANES", "1")
+        assert H.kernel() & 0xFFFF == 512 and not H.kernel() & dev.TUNE_NARROW
+        y1 = _gpu_matvec(gpu_ctx, d, basis, x)
+        monkeypatch.delenv("EDM_MATVEC_LANES")
+        rows = np.arange(basis.D) if basis.D <= 20000 else \
+            np.random.default_rng(3).integers(0, basis.D, 3000)
+        yo = oracle_mod.matvec(basis.download(), _oracle_full(oracle_mod, op), x, rows=rows)
+        assert np.abs(y1[rows] - yo).max() <= Y_TOL * np.abs(yo).max(), name
+        for v, blk in (("u1n", 1024), ("u1n512", 512)):
+            monkeypatch.setenv("EDM_MATVEC", v)
+            assert H.kernel() == blk | dev.TUNE_NARROW, (name, v, hex(H.kernel()))
+            np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, d, basis, x), y1,
+                                          err_msg=f"{name} {v}")
+        print(f"[lds-rank] {name}: D={basis.D} bitwise equal (1024, 512)", flush=True)
 
 
 @pytest.mark.parametrize("N", [0, 1, 2, 13, 14, 15])
@@ -589,8 +641,22 @@ def test_large_sector_enumeration_and_sampled_matvec(gpu_ctx, oracle_mod):
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     print("[large] matvec", flush=True)
-    dev.ROCmLinearMap(dop, b)(yd, xd)
+    H = dev.ROCmLinearMap(dop, b)
+    # the default for this sector: the LDS rank-table kernel (14 + 22 bit split) at 1024 rows
+    assert H.kernel() == 1024 | dev.TUNE_NARROW
+    H(yd, xd)
     y = yd.cpu().numpy()
+    # bitwise the global-table kernel's y
+    os.environ["EDM_MATVEC"] = "u1"
+    os.environ["EDM_MATVEC_LANES"] = "1"
+    try:
+        assert H.kernel() == 512
+        y2 = torch.empty_like(xd)
+        H(y2, xd)
+        assert torch.equal(y2, yd)
+    finally:
+        del os.environ["EDM_MATVEC"]
+        del os.environ["EDM_MATVEC_LANES"]
     rows = np.concatenate([np.arange(64), sample, np.arange(b.D - 64, b.D)])
     full = _oracle_full(oracle_mod, op)
     print("[large] oracle rows", flush=True)
