@@ -1011,6 +1011,20 @@ k_matvec_ldsa(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
 // with the partner lookup of RankNarrow: the workgroup stages the term image and, 16-byte
 // aligned after it, the 2^kNarrowLo-entry J_lo. BLOCK = 1024 keeps the occupancy of the
 // 512-row kernel (two workgroups of the ~67 KiB image per CU on the 6x6 N=12 sector).
+// the 2^nlo-entry J_lo, 16 bytes per thread per step, 16-byte aligned after the term image;
+// returns its LDS address (valid after the barrier that follows)
+__device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, const RankNarrow &r,
+                                                        const uint16_t *__restrict__ jlo,
+                                                        uint4 *lds)
+{
+    const int off = (t.lds_bytes + 15) & ~15;
+    const uint4 *g = (const uint4 *)jlo;
+    uint4 *d = (uint4 *)((unsigned char *)lds + off);
+    const int n16 = (int)(((size_t)r.lomask + 1) * sizeof(uint16_t) / 16);
+    for (int k = threadIdx.x; k < n16; k += blockDim.x) d[k] = g[k];
+    return (const uint16_t *)((const unsigned char *)lds + off);
+}
+
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, RankNarrow rank,
@@ -1019,15 +1033,9 @@ k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Ra
 {
     extern __shared__ uint4 lds_u4[];
     stage_image(t, lds_u4);
-    const int off = (t.lds_bytes + 15) & ~15;
-    {
-        const uint4 *g = (const uint4 *)jlo;
-        uint4 *d = (uint4 *)((unsigned char *)lds_u4 + off);
-        const int n16 = (int)(((size_t)rank.lomask + 1) * sizeof(uint16_t) / 16);
-        for (int k = threadIdx.x; k < n16; k += BLOCK) d[k] = g[k];
-    }
+    const uint16_t *sjlo = stage_narrow(t, rank, jlo, lds_u4);
     __syncthreads();
-    rank.sjlo = (const uint16_t *)((const unsigned char *)lds_u4 + off);
+    rank.sjlo = sjlo;
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= D) return;
     y[i] = row_one<RankNarrow, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
@@ -1172,6 +1180,33 @@ k_matvec_block(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Ran
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = make_double2(0.0, 0.0);
     row_block<Rank, NV>(states[i], (const unsigned char *)lds_u4, t, rank, xi, acc, &dg);
+    const double2 *xr = xi + i * NV;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        cmac(acc[v].x, acc[v].y, dg.x, dg.y, xr[v]);
+        y.p[v][i] = acc[v];
+    }
+}
+
+// k_matvec_block with the partner lookup of RankNarrow (J_lo staged in LDS after the image):
+// the same row walk and arithmetic, so the same Y bit for bit.
+template <int NV, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+k_matvec_block_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t,
+                      RankNarrow rank, const uint16_t *__restrict__ jlo,
+                      const double2 *__restrict__ xi, VecPtrs<NV> y)
+{
+    extern __shared__ uint4 lds_u4[];
+    stage_image(t, lds_u4);
+    const uint16_t *sjlo = stage_narrow(t, rank, jlo, lds_u4);
+    __syncthreads();
+    rank.sjlo = sjlo;
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= D) return;
+    double2 acc[NV], dg;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = make_double2(0.0, 0.0);
+    row_block<RankNarrow, NV>(states[i], (const unsigned char *)lds_u4, t, rank, xi, acc, &dg);
     const double2 *xr = xi + i * NV;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -2140,6 +2175,77 @@ static void launch_block_rank(edm_ctx *ctx, const edm_basis *b, const CompactTab
     }
 }
 
+template <int NV, int BLOCK>
+static void launch_block_narrow_nv(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
+                                   const double2 *const *x, double2 *xi, double2 *const *y)
+{
+    RankNarrow r;
+    const uint16_t *jlo = nullptr;
+    EDM_REQUIRE(narrow_view(b, r, jlo), EDM_EINVAL, "basis has no LDS rank split");
+    const size_t lds = narrow_lds_bytes(ct, r);
+    EDM_REQUIRE(lds <= kLdsMaxBytes, EDM_EINVAL, "term image and rank table exceed the LDS");
+    if (lds > 64 * 1024) {   // per-kernel opt-in beyond 64 KiB, once per device
+        static std::atomic<uint64_t> attr_set{0};
+        const uint64_t bit = 1ULL << (ctx->device & 63);
+        if (!(attr_set.load() & bit)) {
+            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_block_narrow<NV, BLOCK>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)kLdsMaxBytes));
+            attr_set.fetch_or(bit);
+        }
+    }
+    CVecPtrs<NV> xp;
+    VecPtrs<NV> yp;
+    for (int v = 0; v < NV; ++v) {
+        xp.p[v] = x[v];
+        yp.p[v] = y[v];
+    }
+    hipLaunchKernelGGL(k_interleave<NV>, dim3(nblk(b->D)), dim3(kRowBlock), 0, ctx->stream, b->D,
+                       xp, xi);
+    const unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL((k_matvec_block_narrow<NV, BLOCK>), dim3(nb), dim3(BLOCK), lds,
+                       ctx->stream, b->D, b->d_states, compact_dev(ct), r, jlo, xi, yp);
+}
+
+// EDM_BLOCK_NARROW=0 keeps the block matvec on the global-table lookup; =512 or =1024 runs
+// the LDS rank-table block kernel with that workgroup wherever it can run. Default: that kernel
+// from 2^18 rows on, with 1024 rows up to 4 vectors per
+// launch, 512 for 8 (6x6 N=12 sector, ms per vector: 2 vectors 59.4 global table / 57.4 /
+// 50.8; 4 vectors 47.9 / 42.7 / 41.1; 8 vectors 44.3 / 41.5 / 47.5).
+static int block_narrow_setting()
+{
+    const char *v = getenv("EDM_BLOCK_NARROW");
+    if (!v) return -1;
+    const int n = atoi(v);
+    return n == 0 ? 0 : (n == 1024 ? 1024 : 512);
+}
+
+static void launch_block_narrow(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, int nv,
+                                const double2 *const *x, double2 *xi, double2 *const *y,
+                                int setting)
+{
+    int v = 0;
+    while (v < nv) {
+        const int n = nv - v >= 8 ? 8 : nv - v >= 4 ? 4 : nv - v >= 2 ? 2 : 1;
+        const bool wide = setting == 1024 || (setting < 0 && n < 8);
+        if (n == 8) {
+            if (wide) launch_block_narrow_nv<8, 1024>(ctx, b, ct, x + v, xi, y + v);
+            else launch_block_narrow_nv<8, 512>(ctx, b, ct, x + v, xi, y + v);
+        } else if (n == 4) {
+            if (wide) launch_block_narrow_nv<4, 1024>(ctx, b, ct, x + v, xi, y + v);
+            else launch_block_narrow_nv<4, 512>(ctx, b, ct, x + v, xi, y + v);
+        } else if (n == 2) {
+            if (wide) launch_block_narrow_nv<2, 1024>(ctx, b, ct, x + v, xi, y + v);
+            else launch_block_narrow_nv<2, 512>(ctx, b, ct, x + v, xi, y + v);
+        } else {
+            if (wide) launch_block_narrow_nv<1, 1024>(ctx, b, ct, x + v, xi, y + v);
+            else launch_block_narrow_nv<1, 512>(ctx, b, ct, x + v, xi, y + v);
+        }
+        EDM_HIP_CHECK(hipGetLastError());
+        v += n;
+    }
+}
+
 void launch_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int nv,
                          const double2 *const *x, double2 *xi, double2 *const *y)
 {
@@ -2147,6 +2253,11 @@ void launch_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int
     if (b->D == 0 || nv <= 0) return;
     if (!(op->fast && op->cfwd.ok && matvec_variant() == 11)) {
         for (int v = 0; v < nv; ++v) launch_matvec(ctx, op, b, x[v], y[v], ApplyMode::Forward);
+        return;
+    }
+    const int nb = block_narrow_setting();
+    if (nb != 0 && (nb > 0 || b->D >= ((int64_t)1 << 18)) && narrow_ok(op, b, op->cfwd)) {
+        launch_block_narrow(ctx, b, op->cfwd, nv, x, xi, y, nb);
         return;
     }
     if (use_lin(op, b))

@@ -44,6 +44,20 @@ def _gpu_matvec(ctx, dop, basis, x):
     return y
 
 
+def _gpu_matvec_block(ctx, dop, basis, xs):
+    """edm_matvec_block on device copies of the host vectors xs; returns the host results."""
+    import torch
+    X = [torch.from_numpy(xv).cuda() for xv in xs]
+    Y = [torch.empty_like(t) for t in X]
+    work = torch.empty(len(xs) * basis.D, dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    xp = (ctypes.c_void_p * len(xs))(*[t.data_ptr() for t in X])
+    yp = (ctypes.c_void_p * len(xs))(*[t.data_ptr() for t in Y])
+    check(ctx.L.edm_matvec_block(ctx.h, dop.h, basis.h, len(xs), xp, yp, work.data_ptr()), ctx.h)
+    ctx.synchronize()
+    return [t.cpu().numpy() for t in Y]
+
+
 def _gpu_connectivity(ctx, dop, basis, row0, nrows):
     counts = np.zeros(max(nrows, 1), dtype=np.int64)
     check(ctx.L.edm_connectivity_counts(ctx.h, dop.h, basis.h, row0, nrows, counts.ctypes.data), ctx.h)
@@ -376,7 +390,18 @@ def test_matvec_lds_rank_table_bitwise(gpu_ctx, oracle_mod, monkeypatch):
             assert H.kernel() == blk | dev.TUNE_NARROW, (name, v, hex(H.kernel()))
             np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, d, basis, x), y1,
                                           err_msg=f"{name} {v}")
-        print(f"[lds-rank] {name}: D={basis.D} bitwise equal (1024, 512)", flush=True)
+        # the block kernel with the same lookup: every column bitwise the single matvec
+        monkeypatch.setenv("EDM_MATVEC", "u1")
+        nv = 4
+        xs = [np.ascontiguousarray(x * (v + 1) + _rand(basis.D, 94 + v)) for v in range(nv)]
+        ys = [_gpu_matvec(gpu_ctx, d, basis, xv) for xv in xs]
+        for wg in ("512", "1024"):
+            monkeypatch.setenv("EDM_BLOCK_NARROW", wg)
+            yb = _gpu_matvec_block(gpu_ctx, d, basis, xs)
+            for v in range(nv):
+                np.testing.assert_array_equal(yb[v], ys[v], err_msg=f"{name} block {wg} col {v}")
+        monkeypatch.delenv("EDM_BLOCK_NARROW")
+        print(f"[lds-rank] {name}: D={basis.D} bitwise equal (1024, 512, block)", flush=True)
 
 
 @pytest.mark.parametrize("N", [0, 1, 2, 13, 14, 15])
