@@ -1230,6 +1230,8 @@ k_interleave(int64_t D, CVecPtrs<NV> x, double2 *__restrict__ xi)
 // Many independent problems (own basis, own term image, own vectors) in one launch: the
 // workgroup -> problem table routes each 512-row block to its problem, which stages its own
 // term image. Each problem's rows are computed exactly as by k_matvec_ldsu<., true, 1>.
+// use_lin: 2 = rank tables with the low half staged in LDS (RankNarrow, jlo_n), 1 = rank
+// tables in global memory, 0 = binary search
 struct BatchMvDesc {
     int64_t D;
     const uint64_t *states;
@@ -1238,6 +1240,8 @@ struct BatchMvDesc {
     CompactDev t;
     RankLin lin;
     RankSearch search;
+    RankNarrow nar;
+    const uint16_t *jlo_n;
     int use_lin;
     int64_t blk0;
 };
@@ -1248,11 +1252,15 @@ k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ bl
     extern __shared__ uint4 lds_u4[];
     const BatchMvDesc &P = descs[blk2prob[blockIdx.x]];
     stage_image(P.t, lds_u4);
+    RankNarrow nar = P.nar;
+    if (P.use_lin == 2) nar.sjlo = stage_narrow(P.t, nar, P.jlo_n, lds_u4);
     __syncthreads();
     int64_t i = ((int64_t)blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
-    double2 v = P.use_lin
+    double2 v = P.use_lin == 2
+        ? row_one<RankNarrow, true>(i, P.states[i], base, P.t, nar, P.x, 0, INT64_MAX, 1, false)
+        : P.use_lin
         ? ldsu_row<RankLin, true, 1>(i, P.states[i], base, P.t, P.lin, P.x, 0, INT64_MAX, 1)
         : ldsu_row<RankSearch, true, 1>(i, P.states[i], base, P.t, P.search, P.x, 0, INT64_MAX, 1);
     P.y[i] = v;
@@ -1270,7 +1278,9 @@ struct BatchMvGroupDesc {
     CompactDev t;   // the shared structure (the first problem's image)
     RankLin lin;
     RankSearch search;
-    int use_lin;
+    RankNarrow nar;
+    const uint16_t *jlo_n;
+    int use_lin;   // as in BatchMvDesc
     int64_t blk0;
     const double2 *amp_pm[kMaxGroup];   // position-major amplitudes, [p * nkeys + key]
     const double2 *diag[kMaxGroup];
@@ -1364,11 +1374,14 @@ k_batch_mv_group(const BatchMvGroupDesc *__restrict__ descs, const int *__restri
     extern __shared__ uint4 lds_u4[];
     const BatchMvGroupDesc &P = descs[blk2grp[blockIdx.x]];
     stage_image(P.t, lds_u4);
+    RankNarrow nar = P.nar;
+    if (P.use_lin == 2) nar.sjlo = stage_narrow(P.t, nar, P.jlo_n, lds_u4);
     __syncthreads();
     const int64_t i = ((int64_t)blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
-    if (P.use_lin) row_group<RankLin, G>(i, P.states[i], base, P, P.lin);
+    if (P.use_lin == 2) row_group<RankNarrow, G>(i, P.states[i], base, P, nar);
+    else if (P.use_lin) row_group<RankLin, G>(i, P.states[i], base, P, P.lin);
     else row_group<RankSearch, G>(i, P.states[i], base, P, P.search);
 }
 
@@ -2266,6 +2279,28 @@ void launch_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int
         launch_block_rank(ctx, b, op->cfwd, make_search(b), nv, x, xi, y);
 }
 
+// EDM_BATCH_NARROW=0 keeps the batched matvec's rank lookups in global memory
+static bool batch_narrow_enabled()
+{
+    const char *v = getenv("EDM_BATCH_NARROW");
+    return !(v && v[0] == '0');
+}
+// the batched kernels stage at most this much per workgroup (image and low rank table), so a
+// CU keeps at least two of their 512-row workgroups
+constexpr size_t kBatchLdsMax = 80 * 1024;
+
+// dynamic LDS beyond 64 KiB needs the per-kernel opt-in, once per device
+static void batch_lds_optin(edm_ctx *ctx, const void *kernel, std::atomic<uint64_t> &flag,
+                            size_t lds_bytes)
+{
+    if (lds_bytes <= 64 * 1024) return;
+    const uint64_t bit = 1ULL << (ctx->device & 63);
+    if (flag.load() & bit) return;
+    EDM_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)kLdsMaxBytes));
+    flag.fetch_or(bit);
+}
+
 size_t batch_mv_desc_size() { return sizeof(BatchMvDesc); }
 
 int64_t batch_mv_blocks(int64_t D) { return (D + kLdsBlock - 1) / kLdsBlock; }
@@ -2286,8 +2321,13 @@ bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, doubl
     if (d.use_lin) d.lin = make_lin(b);
     d.search = make_search(b);
     d.blk0 = blk0;
-    memcpy(out, &d, sizeof(d));
     *lds_bytes = ct.lds_bytes;
+    if (batch_narrow_enabled() && d.use_lin && narrow_ok(op, b, ct) &&
+        narrow_lds_bytes(ct, d.nar) <= kBatchLdsMax && narrow_view(b, d.nar, d.jlo_n)) {
+        d.use_lin = 2;
+        *lds_bytes = narrow_lds_bytes(ct, d.nar);
+    }
+    memcpy(out, &d, sizeof(d));
     return true;
 }
 
@@ -2295,6 +2335,8 @@ void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, i
                      size_t lds_bytes)
 {
     if (nblocks == 0) return;
+    static std::atomic<uint64_t> optin{0};
+    batch_lds_optin(ctx, (const void *)k_batch_matvec, optin, lds_bytes);
     hipLaunchKernelGGL(k_batch_matvec, dim3((unsigned)nblocks), dim3(kLdsBlock), lds_bytes,
                        ctx->stream, (const BatchMvDesc *)d_descs, d_blk2prob);
     EDM_HIP_CHECK(hipGetLastError());
@@ -2338,8 +2380,14 @@ bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const dou
         d.x[g] = x[g];
         d.y[g] = y[g];
     }
+    const CompactTable &ct = ops[0]->cfwd;
+    *lds_bytes = ct.lds_bytes;
+    if (batch_narrow_enabled() && lin && narrow_ok(ops[0], b, ct) &&
+        narrow_view(b, d.nar, d.jlo_n) && narrow_lds_bytes(ct, d.nar) <= kBatchLdsMax) {
+        d.use_lin = 2;
+        *lds_bytes = narrow_lds_bytes(ct, d.nar);
+    }
     memcpy(out, &d, sizeof(d));
-    *lds_bytes = ops[0]->cfwd.lds_bytes;
     return true;
 }
 
@@ -2348,6 +2396,12 @@ void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *
 {
     if (nblocks == 0) return;
     const auto *dd = (const BatchMvGroupDesc *)d_descs;
+    static std::atomic<uint64_t> optin[kMaxGroup + 1];
+    batch_lds_optin(ctx,
+                    G == 2 ? (const void *)k_batch_mv_group<2>
+                           : G == 3 ? (const void *)k_batch_mv_group<3>
+                                    : (const void *)k_batch_mv_group<4>,
+                    optin[G == 2 ? 2 : G == 3 ? 3 : 4], lds_bytes);
     switch (G) {
     case 2:
         hipLaunchKernelGGL(k_batch_mv_group<2>, dim3((unsigned)nblocks), dim3(kLdsBlock),
