@@ -107,7 +107,7 @@ def cpu_baseline(w, basis, full_terms, x_host, y_dev_host, seconds, nthreads):
     }, err
 
 
-def block_section(ctx, dop, basis, x, y, stream, args, nnz_off, S, ms_single):
+def block_section(ctx, dop, basis, x, y, stream, args, nnz_off, S, ms_single, narrow=False):
     """y_v = H x_v for 4 vectors per edm_matvec_block call (the block-Lanczos matvec: one
     candidate walk and one partner lookup serve all 4 vectors; x interleaved so a partner's 4
     entries share a cache line). Timed with HIP events on the library stream like the single
@@ -147,8 +147,9 @@ def block_section(ctx, dop, basis, x, y, stream, args, nnz_off, S, ms_single):
     return {"vectors_per_call": nv, "calls": steps, "ms_per_call": round(ms, 4),
             "matvecs_per_s": round(nv * 1e3 / ms, 3),
             "speedup_per_vector_vs_single": round(ms_single * nv / ms, 3),
-            "kernel": f"k_matvec_block<{'RankLin' if basis.rank_mode else 'RankSearch'},4,512> "
-                      "+ k_interleave<4>",
+            "kernel": ("k_matvec_block_narrow<4, 1024>" if narrow else
+                       f"k_matvec_block<{'RankLin' if basis.rank_mode else 'RankSearch'}, 4, 512>")
+                      + " + k_interleave<4>",
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS,
                          "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
                          "bytes_model": "D*(8+4*32) + 4*16*nnz_off + 40*S_full",
@@ -580,8 +581,10 @@ def main():
     }
 
     if not args.no_block:
+        # the block kernel reads the low rank table from LDS where the single one does, for
+        # bases of 2^18 rows and more (edm_matvec_block)
         out["block4"] = block_section(ctx, dop, basis, x, y, stream, args, nnz_off.value, S,
-                                      ms_per)
+                                      ms_per, narrow=bool(block & 0x20000) and D >= (1 << 18))
 
     if not args.no_solve:
         log(f"[bench] eigensolve nev={w.nev} krylovdim={w.krylovdim} tol={w.tol}")

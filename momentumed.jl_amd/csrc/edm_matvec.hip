@@ -2285,9 +2285,10 @@ static bool batch_narrow_enabled()
     const char *v = getenv("EDM_BATCH_NARROW");
     return !(v && v[0] == '0');
 }
-// the batched kernels stage at most this much per workgroup (image and low rank table), so a
-// CU keeps at least two of their 512-row workgroups
-constexpr size_t kBatchLdsMax = 80 * 1024;
+// the batched kernels stage the low rank table only while image and table stay within a
+// third of a CU's LDS: one launch stages as much as its largest problem needs, and three
+// 512-row workgroups per CU (24 waves) is as far as the occupancy of the whole launch may fall
+constexpr size_t kBatchLdsMax = kLdsMaxBytes / 3;
 
 // dynamic LDS beyond 64 KiB needs the per-kernel opt-in, once per device
 static void batch_lds_optin(edm_ctx *ctx, const void *kernel, std::atomic<uint64_t> &flag,
@@ -2311,6 +2312,12 @@ bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, doubl
     EDM_REQUIRE(op->nbits == b->nbits, EDM_EINVAL, "operator and basis have different bits");
     const CompactTable &ct = op->cfwd;
     if (!op->fast || !ct.ok || matvec_variant() != 11 || matvec_passes() != 1) return false;
+    // a problem whose own matvec is the 1,024-row LDS rank-table kernel gets its own launch of
+    // it (the batch engine's lone path; same row walk, so the same y): in the shared 512-row
+    // launch its image and table would halve the occupancy of every problem
+    if (matvec_kernel_code(op, b) & EDM_TUNE_NARROW &&
+        narrow_ok(op, b, ct) && b->D >= ((int64_t)1 << 18))
+        return false;
     BatchMvDesc d{};
     d.D = b->D;
     d.states = b->d_states;
