@@ -606,19 +606,25 @@ struct NarrowRow {
 struct NarrowMid {
     uint32_t hm, jhm;
 };
+// t >> nlo for nlo < 32 and a result below 2^32 (every basis with a narrow view): one funnel
+// shift of the two 32-bit halves instead of a 64-bit shift
+__device__ __forceinline__ uint32_t high_part(uint64_t t, int nlo)
+{
+    return __builtin_amdgcn_alignbit((uint32_t)(t >> 32), (uint32_t)t, (uint32_t)nlo);
+}
 __device__ __forceinline__ NarrowRow rank_row32(const RankNarrow &r, uint64_t s)
 {
-    const uint32_t hs = (uint32_t)(s >> r.nlo);
+    const uint32_t hs = high_part(s, r.nlo);
     return NarrowRow{r.jhi, r.sjlo, r.nlo, r.lomask, hs, r.jhi[hs]};
 }
 __device__ __forceinline__ NarrowMid rank_mid32(const NarrowRow &r, uint64_t mid)
 {
-    const uint32_t hm = (uint32_t)(mid >> r.nlo);
+    const uint32_t hm = high_part(mid, r.nlo);
     return NarrowMid{hm, hm == r.hs ? r.jhs : r.jhi[hm]};
 }
 __device__ __forceinline__ int64_t rank_find(const NarrowRow &r, const NarrowMid &m, uint64_t t)
 {
-    const uint32_t h = (uint32_t)(t >> r.nlo);
+    const uint32_t h = high_part(t, r.nlo);
     const uint32_t hv = h == m.hm ? m.jhm : (h == r.hs ? r.jhs : r.jhi[h]);
     return (int64_t)(hv + (uint32_t)r.sjlo[(uint32_t)t & r.lomask]);
 }
@@ -1025,18 +1031,31 @@ __device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, con
     return (const uint16_t *)((const unsigned char *)lds + off);
 }
 
+// Row block of this workgroup when consecutive dispatches go round-robin to the 8 XCDs: with
+// chunk G > 1, every 8 G dispatches cover 8 G consecutive blocks, XCD x taking G consecutive
+// ones of them (a bijection on each group of 8 G; a tail shorter than that keeps its order).
+__device__ __forceinline__ int64_t xcd_chunk_block(int G)
+{
+    const int64_t bid = blockIdx.x;
+    if (G <= 1) return bid;
+    const int64_t grp = (int64_t)kNumXcd * G;
+    if (bid >= ((int64_t)gridDim.x / grp) * grp) return bid;
+    const int64_t xcd = bid % kNumXcd, idx = bid / kNumXcd;
+    return (idx / G) * grp + xcd * G + (idx % G);
+}
+
 template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, RankNarrow rank,
                 const uint16_t *__restrict__ jlo, const double2 *__restrict__ x,
-                double2 *__restrict__ y)
+                double2 *__restrict__ y, int chunk)
 {
     extern __shared__ uint4 lds_u4[];
     stage_image(t, lds_u4);
     const uint16_t *sjlo = stage_narrow(t, rank, jlo, lds_u4);
     __syncthreads();
     rank.sjlo = sjlo;
-    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t i = xcd_chunk_block(chunk) * BLOCK + threadIdx.x;
     if (i >= D) return;
     y[i] = row_one<RankNarrow, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
                                      INT64_MAX, 1, false);
@@ -1194,14 +1213,14 @@ template <int NV, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
 k_matvec_block_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t,
                       RankNarrow rank, const uint16_t *__restrict__ jlo,
-                      const double2 *__restrict__ xi, VecPtrs<NV> y)
+                      const double2 *__restrict__ xi, VecPtrs<NV> y, int chunk)
 {
     extern __shared__ uint4 lds_u4[];
     stage_image(t, lds_u4);
     const uint16_t *sjlo = stage_narrow(t, rank, jlo, lds_u4);
     __syncthreads();
     rank.sjlo = sjlo;
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    int64_t i = xcd_chunk_block(chunk) * BLOCK + threadIdx.x;
     if (i >= D) return;
     double2 acc[NV], dg;
 #pragma unroll
@@ -2016,6 +2035,21 @@ static bool narrow_ok(const edm_op *op, const edm_basis *b, const CompactTable &
            narrow_lds_bytes(ct, r) <= kLdsMaxBytes;
 }
 
+// Workgroup order of the LDS rank-table kernel: each XCD takes kNarrowChunk consecutive row
+// blocks of every 8 kNarrowChunk, so the blocks resident on one XCD (32 CUs x 2 workgroups)
+// are neighbours and their partners share that XCD's L2, while all XCDs stay in one stretch of
+// rows. 6x6 N=12 sector: 60.3-61.2 ms in dispatch order, 56.5-57.3 ms with chunks of 32 to
+// 128 (y the same: only the order of workgroups changes). EDM_XCD_CHUNK=n overrides (1: the
+// dispatch order).
+constexpr int kNarrowChunk = 64;
+static int narrow_chunk()
+{
+    const char *v = getenv("EDM_XCD_CHUNK");
+    if (!v) return kNarrowChunk;
+    const int n = atoi(v);
+    return n < 1 ? 1 : (n > 4096 ? 4096 : n);
+}
+
 template <int BLOCK>
 static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
                                 const double2 *x, double2 *y)
@@ -2036,8 +2070,9 @@ static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactT
         }
     }
     const unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
+    const int chunk = narrow_chunk();
     hipLaunchKernelGGL((k_matvec_narrow<BLOCK>), dim3(nb), dim3(BLOCK), lds, ctx->stream, b->D,
-                       b->d_states, compact_dev(ct), r, jlo, x, y);
+                       b->d_states, compact_dev(ct), r, jlo, x, y, chunk);
 }
 
 static void launch_narrow(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
@@ -2217,7 +2252,8 @@ static void launch_block_narrow_nv(edm_ctx *ctx, const edm_basis *b, const Compa
                        xp, xi);
     const unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
     hipLaunchKernelGGL((k_matvec_block_narrow<NV, BLOCK>), dim3(nb), dim3(BLOCK), lds,
-                       ctx->stream, b->D, b->d_states, compact_dev(ct), r, jlo, xi, yp);
+                       ctx->stream, b->D, b->d_states, compact_dev(ct), r, jlo, xi, yp,
+                       narrow_chunk());
 }
 
 // EDM_BLOCK_NARROW=0 keeps the block matvec on the global-table lookup; =512 or =1024 runs
