@@ -407,6 +407,19 @@ __device__ __forceinline__ int64_t xcd_block(int swz)
     return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Row block of this workgroup when consecutive dispatches go round-robin to the 8 XCDs: with
+// chunk G > 1, every 8 G dispatches cover 8 G consecutive blocks, XCD x taking G consecutive
+// ones of them (a bijection on each group of 8 G; a tail shorter than that keeps its order).
+__device__ __forceinline__ int64_t xcd_chunk_block(int G)
+{
+    const int64_t bid = blockIdx.x;
+    if (G <= 1) return bid;
+    const int64_t grp = (int64_t)kNumXcd * G;
+    if (bid >= ((int64_t)gridDim.x / grp) * grp) return bid;
+    const int64_t xcd = bid % kNumXcd, idx = bid / kNumXcd;
+    return (idx / G) * grp + xcd * G + (idx % G);
+}
+
 template <class Rank, bool HighFirst>
 __global__ void __launch_bounds__(kLdsBlock)
 k_matvec_lds(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
@@ -955,12 +968,12 @@ template <class Rank, bool HighFirst, int U, int BLOCK = kLdsBlock>
 __global__ void __launch_bounds__(BLOCK)
 k_matvec_ldsu(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank rank,
               const double2 *__restrict__ x, double2 *__restrict__ y, int64_t jmin, int64_t jmax,
-              int accumulate, int with_diag)
+              int accumulate, int with_diag, int chunk)
 {
     extern __shared__ uint4 lds_u4[];
     stage_image(t, lds_u4);
     __syncthreads();
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = xcd_chunk_block(chunk) * blockDim.x + threadIdx.x;
     if (i >= D) return;
     double2 v = ldsu_row<Rank, HighFirst, U>(i, states[i], (const unsigned char *)lds_u4, t, rank,
                                              x, jmin, jmax, with_diag);
@@ -1029,19 +1042,6 @@ __device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, con
     const int n16 = (int)(((size_t)r.lomask + 1) * sizeof(uint16_t) / 16);
     for (int k = threadIdx.x; k < n16; k += blockDim.x) d[k] = g[k];
     return (const uint16_t *)((const unsigned char *)lds + off);
-}
-
-// Row block of this workgroup when consecutive dispatches go round-robin to the 8 XCDs: with
-// chunk G > 1, every 8 G dispatches cover 8 G consecutive blocks, XCD x taking G consecutive
-// ones of them (a bijection on each group of 8 G; a tail shorter than that keeps its order).
-__device__ __forceinline__ int64_t xcd_chunk_block(int G)
-{
-    const int64_t bid = blockIdx.x;
-    if (G <= 1) return bid;
-    const int64_t grp = (int64_t)kNumXcd * G;
-    if (bid >= ((int64_t)gridDim.x / grp) * grp) return bid;
-    const int64_t xcd = bid % kNumXcd, idx = bid / kNumXcd;
-    return (idx / G) * grp + xcd * G + (idx % G);
 }
 
 template <int BLOCK>
@@ -1899,6 +1899,17 @@ static int matvec_passes()
     return n < 1 ? 1 : (n > 16 ? 16 : n);
 }
 
+// workgroup order of k_matvec_ldsu: row blocks in XCD chunks of 64 as in the LDS rank-table
+// kernel (6x6 N=12 sector: 84.7 -> 82.7 ms; 48-orbital two-component sector: 8.23 -> 8.06 ms;
+// chunks of 32 and 128 alike); EDM_XCD_CHUNK_LDSU=n overrides (1: the dispatch order)
+static int ldsu_chunk()
+{
+    const char *v = getenv("EDM_XCD_CHUNK_LDSU");
+    if (!v) return 64;
+    const int n = atoi(v);
+    return n < 1 ? 1 : (n > 4096 ? 4096 : n);
+}
+
 template <class Rank, int U>
 static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct, Rank rank,
                         const double2 *x, double2 *y, int block = kLdsBlock)
@@ -1911,11 +1922,11 @@ static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct
         if (block == 256)
             hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, 1, 256>), dim3(nb), dim3(256),
                                ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, rank, x, y, 0,
-                               INT64_MAX, 0, 1);
+                               INT64_MAX, 0, 1, ldsu_chunk());
         else
             hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, 1, 1024>), dim3(nb), dim3(1024),
                                ct.lds_bytes, ctx->stream, b->D, b->d_states, cd, rank, x, y, 0,
-                               INT64_MAX, 0, 1);
+                               INT64_MAX, 0, 1, ldsu_chunk());
         return;
     }
     unsigned nb = (unsigned)((b->D + kLdsBlock - 1) / kLdsBlock);
@@ -1924,7 +1935,7 @@ static void launch_ldsu(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct
         if (passes == 1) jmax = INT64_MAX;   // no window
         hipLaunchKernelGGL((k_matvec_ldsu<Rank, true, U>), dim3(nb), dim3(kLdsBlock), ct.lds_bytes,
                            ctx->stream, b->D, b->d_states, cd, rank, x, y, jmin, jmax, p > 0 ? 1 : 0,
-                           p == 0 ? 1 : 0);
+                           p == 0 ? 1 : 0, ldsu_chunk());
     }
 }
 
