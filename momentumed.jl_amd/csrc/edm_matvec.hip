@@ -602,7 +602,9 @@ __device__ __forceinline__ int64_t rank_find(const RankSearch &r, NoMid, uint64_
 // half of the L2 requests of the global-table walk on the 6x6 N=12 sector), becomes an LDS
 // read; only J_hi stays in global memory, and lanes that walk the same key of rows sharing
 // their high bits mostly read the same J_hi word. sjlo is set inside the kernel, after
-// staging. A partner that shares its high part with mid or with the row takes their word.
+// staging. A partner that shares its high part with mid takes mid's word (a partner with the
+// row's own high part but not mid's needs a conserved component of several orbitals per
+// momentum; the reuse check for it cost more than it saved).
 struct RankNarrow {
     const uint32_t *jhi;
     const uint16_t *sjlo;
@@ -638,7 +640,7 @@ __device__ __forceinline__ NarrowMid rank_mid32(const NarrowRow &r, uint64_t mid
 __device__ __forceinline__ int64_t rank_find(const NarrowRow &r, const NarrowMid &m, uint64_t t)
 {
     const uint32_t h = high_part(t, r.nlo);
-    const uint32_t hv = h == m.hm ? m.jhm : (h == r.hs ? r.jhs : r.jhi[h]);
+    const uint32_t hv = h == m.hm ? m.jhm : r.jhi[h];
     return (int64_t)(hv + (uint32_t)r.sjlo[(uint32_t)t & r.lomask]);
 }
 // the same tables read from global memory (checks of the split, on every basis state)
