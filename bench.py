@@ -414,12 +414,13 @@ def served_section(args):
            "plain_stream": f"the first {args.serve_plain_seconds:g} s of the same stream "
                            f"({len(prefix)} requests), same budget and queue bound"}
     out["policy"] = {"quantum_rows": 1 << 15, "big_share": args.serve_big_share,
-                     "head_wait_s": args.serve_head_wait}
+                     "head_wait_s": args.serve_head_wait, "big_queue_limit": args.serve_big_queue}
     for mode, rq in (("batched", reqs), ("plain", prefix)):
         if mode == "plain" and args.serve_plain_seconds <= 0:
             continue
         srv = S.SolveServer(hbm_budget_bytes=200 * 10 ** 9, max_queue=256, mode=mode,
-                            big_share=args.serve_big_share, head_wait_s=args.serve_head_wait)
+                            big_share=args.serve_big_share, head_wait_s=args.serve_head_wait,
+                            big_queue_limit=args.serve_big_queue)
         try:
             rep = srv.run(rq, drain_s=args.serve_drain)
         finally:
@@ -463,6 +464,9 @@ def main():
                     help="share of the stepped rows the oldest large running problem gets")
     ap.add_argument("--serve-head-wait", type=float, default=60.0,
                     help="seconds after which the queue's head is no longer passed")
+    ap.add_argument("--serve-big-queue", type=int, default=128,
+                    help="large requests (more rows than the step quantum) allowed to wait in "
+                         "the queue (0: no limit beside the queue bound)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the small and medium configuration lines")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -98,6 +98,7 @@ class _Job:
     op: DeviceOperator | None = None
     pid: int = -1
     t_admit: float = 0.0
+    big: bool = False                    # counted against big_queue_limit
 
 
 @dataclass
@@ -161,14 +162,17 @@ class SolveServer:
                  krylovdim: int = 30, tol: float = 1e-12, maxiter: int = 100, seed: int = 0x5EED,
                  mode: str = "batched", prepare_ahead: int = 64, device: int = 0,
                  quantum_rows: int = 1 << 15, big_share: float = 0.5,
-                 head_wait_s: float = 60.0):
+                 head_wait_s: float = 60.0, big_queue_limit: int = 128):
         """quantum_rows: the step quantum of the batch's round robin (a problem of D rows
         steps about once every D / quantum_rows steps). big_share: the share of the rows
         stepped that the oldest running problem larger than the quantum receives (its
         round-robin weight is set so), so a large request keeps moving however many small
         ones run beside it (0 turns this off). head_wait_s: once the head of the queue has
         waited this long, requests behind it are no longer admitted ahead of it, so the
-        budget frees up for it."""
+        budget frees up for it. big_queue_limit: at most
+        this many requests larger than the quantum wait in the queue (0: no limit beside
+        max_queue); a large request arriving beyond it is rejected, so that under overload the
+        queue keeps room for small ones (the large ones are bound by the device either way)."""
         if mode not in ("batched", "plain"):
             raise ValueError("mode must be 'batched' or 'plain'")
         if not 0.0 <= big_share < 1.0:
@@ -177,6 +181,7 @@ class SolveServer:
         self.quantum = int(quantum_rows)
         self.big_share = float(big_share)
         self.head_wait_s = float(head_wait_s)
+        self.big_queue_limit = int(big_queue_limit)
         self._weights: dict = {}
         self.ctx = Context(device, hbm_budget_bytes)
         self.budget = int(hbm_budget_bytes)
@@ -257,6 +262,7 @@ class SolveServer:
         worker = threading.Thread(target=prep_worker, daemon=True)
         worker.start()
         queued_for_prep: set = set()
+        nbig_waiting = 0                  # queued large requests (big_queue_limit)
         t0 = time.perf_counter()
         i = 0
         last_arrival = reqs[-1].arrival if reqs else 0.0
@@ -264,19 +270,29 @@ class SolveServer:
             while True:
                 now = time.perf_counter() - t0
                 while i < len(reqs) and reqs[i].arrival <= now:
-                    if len(waiting) >= self.max_queue:
+                    big = self.big_queue_limit and self.sector_size(reqs[i]) > self.quantum
+                    if len(waiting) >= self.max_queue or \
+                            (big and nbig_waiting >= self.big_queue_limit):
                         rep.rejected += 1
                         rep.rejected_labels.append(reqs[i].label)
                     else:
-                        waiting.append(_Job(reqs[i]))
+                        waiting.append(_Job(reqs[i], big=bool(big)))
+                        nbig_waiting += 1 if big else 0
                     i += 1
-                # keep the head of the queue prepared on the worker thread
-                for job in waiting[: self.prepare_ahead]:
+                # keep prepare_ahead requests of the queue in preparation on the worker thread,
+                # front first; requests already prepared (say large ones that wait for budget
+                # at the front) do not count, so those behind them get prepared too
+                pending = prep_in.qsize()
+                for job in waiting:
+                    if pending >= self.prepare_ahead:
+                        break
                     if job.req.rid not in queued_for_prep:
                         queued_for_prep.add(job.req.rid)
                         prep_in.put(job)
+                        pending += 1
                 if self.mode == "batched":
                     self._admit_batched(waiting, running, rep, prep_lock, now)
+                    nbig_waiting = sum(1 for j in waiting if j.big)
                     if running:
                         self._rebalance(running)
                         _, nfin = self.batch.step(1)
@@ -290,6 +306,7 @@ class SolveServer:
                 else:
                     if waiting and self._ready(waiting[0], prep_lock):
                         job = waiting.pop(0)
+                        nbig_waiting -= 1 if job.big else 0
                         self._solve_plain(job, rep, t0, on_result)
                     elif i < len(reqs) and not waiting:
                         time.sleep(min(0.005, max(0.0, reqs[i].arrival - now)))
@@ -328,7 +345,8 @@ class SolveServer:
         while k < len(waiting):
             job = waiting[k]
             if not self._ready(job, lock):
-                break
+                k += 1                         # still in preparation: the ones behind may go
+                continue
             need = self.estimate_bytes(job.req)
             if need > self.budget:
                 waiting.pop(k)                 # can never fit
