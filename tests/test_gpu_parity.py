@@ -671,17 +671,28 @@ def test_large_sector_enumeration_and_sampled_matvec(gpu_ctx, oracle_mod):
     assert H.kernel() == 1024 | dev.TUNE_NARROW
     H(yd, xd)
     y = yd.cpu().numpy()
-    # bitwise the global-table kernel's y
+    # bitwise the global-table kernel's y, and both kernels' y the same in dispatch order as in
+    # XCD chunks (34k row blocks: 66 full groups of 512 blocks and a tail)
+    y2 = torch.empty_like(xd)
+    os.environ["EDM_XCD_CHUNK"] = "1"
+    try:
+        H(y2, xd)
+        assert torch.equal(y2, yd)
+    finally:
+        del os.environ["EDM_XCD_CHUNK"]
     os.environ["EDM_MATVEC"] = "u1"
     os.environ["EDM_MATVEC_LANES"] = "1"
     try:
         assert H.kernel() == 512
-        y2 = torch.empty_like(xd)
+        H(y2, xd)
+        assert torch.equal(y2, yd)
+        os.environ["EDM_XCD_CHUNK_LDSU"] = "1"
         H(y2, xd)
         assert torch.equal(y2, yd)
     finally:
         del os.environ["EDM_MATVEC"]
         del os.environ["EDM_MATVEC_LANES"]
+        os.environ.pop("EDM_XCD_CHUNK_LDSU", None)
     rows = np.concatenate([np.arange(64), sample, np.arange(b.D - 64, b.D)])
     full = _oracle_full(oracle_mod, op)
     print("[large] oracle rows", flush=True)
