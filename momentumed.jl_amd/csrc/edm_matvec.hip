@@ -1268,15 +1268,16 @@ struct BatchMvDesc {
 };
 
 __global__ void __launch_bounds__(kLdsBlock)
-k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ blk2prob)
+k_batch_matvec(const BatchMvDesc *__restrict__ descs, const int *__restrict__ blk2prob, int chunk)
 {
     extern __shared__ uint4 lds_u4[];
-    const BatchMvDesc &P = descs[blk2prob[blockIdx.x]];
+    const int64_t blk = xcd_chunk_block(chunk);
+    const BatchMvDesc &P = descs[blk2prob[blk]];
     stage_image(P.t, lds_u4);
     RankNarrow nar = P.nar;
     if (P.use_lin == 2) nar.sjlo = stage_narrow(P.t, nar, P.jlo_n, lds_u4);
     __syncthreads();
-    int64_t i = ((int64_t)blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
+    int64_t i = (blk - P.blk0) * blockDim.x + threadIdx.x;
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
     double2 v = P.use_lin == 2
@@ -1390,15 +1391,17 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
 
 template <int G>
 __global__ void __launch_bounds__(kLdsBlock)
-k_batch_mv_group(const BatchMvGroupDesc *__restrict__ descs, const int *__restrict__ blk2grp)
+k_batch_mv_group(const BatchMvGroupDesc *__restrict__ descs, const int *__restrict__ blk2grp,
+                 int chunk)
 {
     extern __shared__ uint4 lds_u4[];
-    const BatchMvGroupDesc &P = descs[blk2grp[blockIdx.x]];
+    const int64_t blk = xcd_chunk_block(chunk);
+    const BatchMvGroupDesc &P = descs[blk2grp[blk]];
     stage_image(P.t, lds_u4);
     RankNarrow nar = P.nar;
     if (P.use_lin == 2) nar.sjlo = stage_narrow(P.t, nar, P.jlo_n, lds_u4);
     __syncthreads();
-    const int64_t i = ((int64_t)blockIdx.x - P.blk0) * blockDim.x + threadIdx.x;
+    const int64_t i = (blk - P.blk0) * blockDim.x + threadIdx.x;
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
     if (P.use_lin == 2) row_group<RankNarrow, G>(i, P.states[i], base, P, nar);
@@ -2328,6 +2331,18 @@ void launch_matvec_block(edm_ctx *ctx, const edm_op *op, const edm_basis *b, int
         launch_block_rank(ctx, b, op->cfwd, make_search(b), nv, x, xi, y);
 }
 
+// XCD chunk of the batched matvec kernels' row blocks: a problem's consecutive blocks stay on
+// one XCD, so its x is cached in one L2 instead of eight (3,840-problem sweep: 491 / 508 / 516
+// sector-solves/s with chunks of 1 / 16 / 64); EDM_XCD_CHUNK_BATCH=n overrides (1: dispatch
+// order)
+static int batch_chunk()
+{
+    const char *v = getenv("EDM_XCD_CHUNK_BATCH");
+    if (!v) return 64;
+    const int n = atoi(v);
+    return n < 1 ? 1 : (n > 4096 ? 4096 : n);
+}
+
 // EDM_BATCH_NARROW=0 keeps the batched matvec's rank lookups in global memory
 static bool batch_narrow_enabled()
 {
@@ -2394,7 +2409,7 @@ void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, i
     static std::atomic<uint64_t> optin{0};
     batch_lds_optin(ctx, (const void *)k_batch_matvec, optin, lds_bytes);
     hipLaunchKernelGGL(k_batch_matvec, dim3((unsigned)nblocks), dim3(kLdsBlock), lds_bytes,
-                       ctx->stream, (const BatchMvDesc *)d_descs, d_blk2prob);
+                       ctx->stream, (const BatchMvDesc *)d_descs, d_blk2prob, batch_chunk());
     EDM_HIP_CHECK(hipGetLastError());
 }
 
@@ -2461,15 +2476,15 @@ void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *
     switch (G) {
     case 2:
         hipLaunchKernelGGL(k_batch_mv_group<2>, dim3((unsigned)nblocks), dim3(kLdsBlock),
-                           lds_bytes, ctx->stream, dd, d_blk2grp);
+                           lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
         break;
     case 3:
         hipLaunchKernelGGL(k_batch_mv_group<3>, dim3((unsigned)nblocks), dim3(kLdsBlock),
-                           lds_bytes, ctx->stream, dd, d_blk2grp);
+                           lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
         break;
     default:
         hipLaunchKernelGGL(k_batch_mv_group<4>, dim3((unsigned)nblocks), dim3(kLdsBlock),
-                           lds_bytes, ctx->stream, dd, d_blk2grp);
+                           lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
         break;
     }
     EDM_HIP_CHECK(hipGetLastError());
