@@ -621,6 +621,9 @@ struct BlockWorkspace {
     double *beta(int c) { return (double *)(d_c + (size_t)c * ld + 2 * (M + bs)); }
     void release()
     {
+        // an error may unwind while a block step's kernels are still queued: let the stream
+        // drain before their buffers go back to the allocator
+        (void)hipStreamSynchronize(ctx->stream);
         for (auto p : V) ctx->free(p, vbytes);
         for (auto p : tmp) ctx->free(p, vbytes);
         V.clear();
