@@ -215,6 +215,8 @@ class SolveServer:
         nbits = req.mesh[0] * req.mesh[1]
         nhi = max(0, nbits - (nbits + 1) // 2)
         rank = (1 << ((nbits + 1) // 2)) * 6 + (1 << nhi) * 4
+        if 14 < nbits <= 38:   # the split at bit 14 the LDS rank-table matvec reads
+            rank += (4 << (nbits - 14)) + (2 << 14)
         ws = problem_bytes(D, req.nev, self.kw["krylovdim"]) if self.mode == "batched" else \
             (2 * min(max(self.kw["krylovdim"], req.nev + 2), D) + 1) * D * 16
         return ws + D * 8 + rank + (4 << 20)
@@ -344,10 +346,12 @@ class SolveServer:
         k = 0
         while k < len(waiting):
             job = waiting[k]
+            need = self.estimate_bytes(job.req)
             if not self._ready(job, lock):
+                if k == 0 and need > free and now - job.req.arrival > self.head_wait_s:
+                    break                      # owed the budget, even while in preparation
                 k += 1                         # still in preparation: the ones behind may go
                 continue
-            need = self.estimate_bytes(job.req)
             if need > self.budget:
                 waiting.pop(k)                 # can never fit
                 rep.rejected += 1
