@@ -304,15 +304,18 @@ k_breduce(const BatchVecDesc *__restrict__ descs, int which)
     if (threadIdx.x == 0) (which ? P.h2 : P.h)[i] = s;
 }
 
+// rev: row blocks taken in reverse dispatch order (block b does the work of block nblk-1-b)
 __global__ void __launch_bounds__(kBlk)
-k_baxpy(const BatchVecDesc *__restrict__ descs, const int *__restrict__ blk2prob, int which)
+k_baxpy(const BatchVecDesc *__restrict__ descs, const int *__restrict__ blk2prob, int which,
+        int rev)
 {
     __shared__ double2 hs[kMaxVec];
-    const BatchVecDesc &P = descs[blk2prob[blockIdx.x]];
+    const int64_t bx = rev ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
+    const BatchVecDesc &P = descs[blk2prob[bx]];
     const double2 *c = which ? P.h2 : P.h;
     for (int i = threadIdx.x; i < P.nvec; i += kBlk) hs[i] = c[i];
     __syncthreads();
-    int64_t r = ((int64_t)blockIdx.x - P.blk0) * kBlk + threadIdx.x;
+    int64_t r = (bx - P.blk0) * kBlk + threadIdx.x;
     if (r >= P.D) return;
     double2 acc = P.w[r];
     for (int i = 0; i < P.nvec; ++i) {
@@ -459,13 +462,17 @@ void bblas_cgs2_norm(edm_ctx *ctx, const BatchVecDesc *d_descs, int nprob, int m
     hipStream_t st = ctx->stream;
     const int span = dot_span(ntiles, std::max(maxvec, 1));
     const unsigned ny = (unsigned)((std::max(maxvec, 1) + span - 1) / span);
+    // The multi-axpy walks its blocks in reverse: it starts on the vectors the multi-dot before
+    // it read last, and the multi-dot after it starts where it ended, so each launch begins on
+    // data still in the MALL (the running problems' vectors are GBs per launch, the MALL
+    // 256 MB). Each block's work is unchanged, so the results are too.
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_bmultidot_partial, dim3((unsigned)ntiles, ny), dim3(kBlk), 0, st,
                            d_descs, d_tile2prob, span);
         hipLaunchKernelGGL(k_breduce, dim3((unsigned)nprob, (unsigned)maxvec), dim3(kBlk), 0, st,
                            d_descs, pass);
         hipLaunchKernelGGL(k_baxpy, dim3((unsigned)nblk), dim3(kBlk), 0, st, d_descs, d_blk2prob,
-                           pass);
+                           pass, 1);
     }
     hipLaunchKernelGGL(k_bnorm_partial, dim3((unsigned)ntiles), dim3(kBlk), 0, st, d_descs,
                        d_tile2prob);
