@@ -38,12 +38,14 @@ def log(*a):
 def _kernel_name(rank_mode: int, block: int = 512) -> str:
     """Name of the matvec kernel libedm.so launches for the bucketed path, from the code
     edm_matvec_kernel reports (rows per workgroup | 0x10000 for the LDS-amplitude kernel |
-    0x20000 for the LDS rank-table kernel | lanes per row << 20 for the split-row kernel) and
-    EDM_MATVEC."""
+    0x20000 for the LDS rank-table kernel | lanes per row << 20 for the split-row kernel),
+    EDM_MATVEC and EDM_NARROW_PIPE (the LDS rank-table kernel's pipelined walk, on unless it
+    would cost occupancy; it is on for the workloads measured here)."""
     rank = "RankLin" if rank_mode == 1 else "RankSearch"
     v = os.environ.get("EDM_MATVEC", "u1")
     if block & 0x20000:
-        return f"k_matvec_narrow<{block & 0xFFFF}>"
+        pipe = os.environ.get("EDM_NARROW_PIPE", "1") != "0"
+        return f"k_matvec_narrow<{block & 0xFFFF}, {str(pipe).lower()}>"
     if v == "u1" and (block >> 20) > 1:
         return f"k_matvec_split<{rank}, {(block >> 20) & 0xFF}>"
     if v == "u1a" or (v == "u1" and block & 0x10000):

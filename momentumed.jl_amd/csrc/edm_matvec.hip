@@ -347,6 +347,7 @@ constexpr int kLdsBlock = 512;
 // LDS a workgroup may use on gfx950 (160 KiB per CU)
 constexpr size_t kLdsMaxBytes = 160 * 1024;
 constexpr int kNumXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin across them
+constexpr int kCuThreads = 2048;   // resident threads per CU (32 wavefronts of 64)
 
 struct CompactDev {
     const uint16_t *keyptr;
@@ -661,11 +662,26 @@ struct RankNarrowGlobal {
 // AmpLds: the amplitudes come from the LDS tail of the image (distinct values, with a
 // conjugation flag per entry) instead of the global per-entry table; the values are the same
 // bits, so the result is too.
-template <class Rank, bool HighFirst, bool AmpLds = false>
+// s_waitcnt vmcnt(0): the wavefront's loads into LDS (global_load_lds) have landed. The
+// compiler does not order every read of the slot after them by itself.
+__device__ __forceinline__ void lds_dma_wait()
+{
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt 0, expcnt 7, lgkmcnt 15: only vmcnt waits
+    asm volatile("" ::: "memory");
+}
+// Pipe: each accepted partner's x entry is fetched straight into this lane's 16-byte LDS slot
+// (global_load_lds, no registers held while it is in flight) and multiplied in when the lane
+// takes its next partner (or at the end of the row), so the gather overlaps the walk to the
+// next partner. The products are added in the same order, so y is the same bit for bit.
+// wave_slot: the wavefront's 64 slots (wave-uniform), lane_slot: this lane's.
+template <class Rank, bool HighFirst, bool AmpLds = false, bool Pipe = false,
+          class = std::enable_if_t<!(Pipe && AmpLds)>>
 __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned char *base,
                                            const CompactDev &t, const Rank &rank,
                                            const double2 *__restrict__ x, int64_t jmin,
-                                           int64_t jmax, int with_diag, bool windowed)
+                                           int64_t jmax, int with_diag, bool windowed,
+                                           double2 *wave_slot = nullptr,
+                                           const double2 *lane_slot = nullptr)
 {
     const uint16_t *s_keyptr = (const uint16_t *)base;
     const double2 *s_diag = (const double2 *)(base + t.off_diag);
@@ -675,26 +691,48 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
     double yr = 0.0, yi = 0.0, dr = 0.0, di = 0.0;
     const auto rrow = rank_row32(rank, s);
     const uint64_t ps = prefix_parity(s);
+    bool pend = false;            // Pipe: a partner's x is in flight to the lane's slot
+    double par = 0.0, pai = 0.0;  // Pipe: its amplitude
+    uint32_t pneg = 0;            // Pipe: and sign
 
     // pmi: the entry's index in the position-major amplitude table
     auto take = [&](int e, int pmi, const auto &mview, uint64_t target, uint32_t neg) {
         int64_t j = rank_find(rrow, mview, target);
         if (RankMayMiss<Rank>::value && j < 0) return;
         if (windowed && (j < jmin || j >= jmax)) return;
-        double2 a;
-        uint32_t cj = 0;
-        if constexpr (AmpLds) {
-            const uint32_t m = s_ampidx[e];
-            a = s_amps[m & 0x7FFFu];
-            cj = m >> 15;
+        if constexpr (Pipe) {
+            // the previous partner first (its x has been in flight since its take); then this
+            // partner's amplitude, loaded into the pending registers only now, so that no copy
+            // of a load still in flight is needed; its sign is applied when it is consumed
+            if (pend) {
+                lds_dma_wait();
+                cmac(yr, yi, flip_sign(par, pneg), flip_sign(pai, pneg), *lane_slot);
+            }
+            asm volatile("" ::: "memory");
+            const double2 a = t.amp_pm[pmi];
+            par = a.x;
+            pai = a.y;
+            pneg = neg;
+            __builtin_amdgcn_global_load_lds((const void *)(x + j),
+                                             (__attribute__((address_space(3))) void *)wave_slot,
+                                             16, 0, 0);
+            pend = true;
         } else {
-            a = t.amp_pm[pmi];
+            double2 a;
+            uint32_t cj = 0;
+            if constexpr (AmpLds) {
+                const uint32_t m = s_ampidx[e];
+                a = s_amps[m & 0x7FFFu];
+                cj = m >> 15;
+            } else {
+                a = t.amp_pm[pmi];
+            }
+            double2 xv = x[j];
+            cmac(yr, yi, flip_sign(a.x, neg), flip_sign(a.y, neg ^ cj), xv);
         }
-        double2 xv = x[j];
-        cmac(yr, yi, flip_sign(a.x, neg), flip_sign(a.y, neg ^ cj), xv);
     };
     auto enter = [&](int key, int &e0, int &e1) {
-        if (with_diag) {
+        if (with_diag && !Pipe) {   // Pipe: summed after the walk, in the same key order
             double2 dg = s_diag[key];
             dr += dg.x;
             di += dg.y;
@@ -744,6 +782,36 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
                 if (((uint32_t)(mid >> c) | (uint32_t)(mid >> d)) & 1u) continue;
                 const uint32_t neg = (pout ^ (uint32_t)(pmid >> c) ^ (uint32_t)(pmid >> d)) & 1u;
                 take(e0 + q, pmi, mview, mid | (1ULL << c) | (1ULL << d), neg);
+            }
+        }
+    }
+    if constexpr (Pipe) {
+        if (pend) {
+            lds_dma_wait();
+            cmac(yr, yi, flip_sign(par, pneg), flip_sign(pai, pneg), *lane_slot);
+        }
+        // the diagonal terms of the keys the walk entered, in its order, from the global copy
+        // of the table: a load inside the walk would have to wait for the x in flight
+        if (with_diag) {
+            uint64_t r2 = s;
+            while (r2) {
+                const int b = HighFirst ? 63 - __builtin_clzll(r2) : __builtin_ctzll(r2);
+                r2 ^= 1ULL << b;
+                if (b < t.nkeys) {
+                    const double2 dg = t.diag[b];
+                    dr += dg.x;
+                    di += dg.y;
+                }
+                uint64_t low = s & ((1ULL << b) - 1ULL);
+                const int kb = 64 + ((b * (b - 1)) >> 1);
+                while (low) {
+                    const int a = HighFirst ? 63 - __builtin_clzll(low) : __builtin_ctzll(low);
+                    low ^= 1ULL << a;
+                    if (kb + a >= t.nkeys) continue;
+                    const double2 dg = t.diag[kb + a];
+                    dr += dg.x;
+                    di += dg.y;
+                }
             }
         }
     }
@@ -953,6 +1021,22 @@ __device__ __forceinline__ void stage_image(const CompactDev &t, uint4 *lds)
     for (int k = threadIdx.x; k < n16; k += blockDim.x) lds[k] = g[k];
 }
 
+// The image without its diagonal table (keyptr | code), for kernels that read the diagonal
+// from global memory: image_nodiag gives the offsets of that layout.
+__host__ __device__ __forceinline__ CompactDev image_nodiag(CompactDev t)
+{
+    t.lds_bytes = t.off_diag + (t.lds_bytes - t.off_code);
+    t.off_code = t.off_diag;
+    return t;
+}
+__device__ __forceinline__ void stage_image_nodiag(const CompactDev &t, uint4 *lds)
+{
+    const uint4 *g = (const uint4 *)t.keyptr;
+    const int nk16 = t.off_diag / 16, skip = (t.off_code - t.off_diag) / 16;
+    const int n16 = (t.lds_bytes - (t.off_code - t.off_diag)) / 16;
+    for (int k = threadIdx.x; k < n16; k += blockDim.x) lds[k] = g[k < nk16 ? k : k + skip];
+}
+
 // the image with its amplitude tail
 __device__ __forceinline__ void stage_image_full(const CompactDev &t, uint4 *lds)
 {
@@ -1046,21 +1130,32 @@ __device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, con
     return (const uint16_t *)((const unsigned char *)lds + off);
 }
 
-template <int BLOCK>
+template <int BLOCK, bool PIPE = false>
 __global__ void __launch_bounds__(BLOCK)
 k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, RankNarrow rank,
                 const uint16_t *__restrict__ jlo, const double2 *__restrict__ x,
                 double2 *__restrict__ y, int chunk)
 {
     extern __shared__ uint4 lds_u4[];
-    stage_image(t, lds_u4);
+    __shared__ double2 pipe_slots[PIPE ? BLOCK : 1];
+    if constexpr (PIPE) {
+        stage_image_nodiag(t, lds_u4);
+        t = image_nodiag(t);
+    } else {
+        stage_image(t, lds_u4);
+    }
     const uint16_t *sjlo = stage_narrow(t, rank, jlo, lds_u4);
     __syncthreads();
     rank.sjlo = sjlo;
     const int64_t i = xcd_chunk_block(chunk) * BLOCK + threadIdx.x;
     if (i >= D) return;
-    y[i] = row_one<RankNarrow, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
-                                     INT64_MAX, 1, false);
+    if constexpr (PIPE)
+        y[i] = row_one<RankNarrow, true, false, true>(
+            i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0, INT64_MAX, 1, false,
+            pipe_slots + (threadIdx.x & ~63u), pipe_slots + threadIdx.x);
+    else
+        y[i] = row_one<RankNarrow, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x,
+                                         0, INT64_MAX, 1, false);
 }
 
 // J_hi_n[h] = index of the first state whose high part (bits >= nlo) is h (entries of absent
@@ -2066,36 +2161,71 @@ static int narrow_chunk()
     return n < 1 ? 1 : (n > 4096 ? 4096 : n);
 }
 
-template <int BLOCK>
+// Whether the LDS rank-table kernel runs its pipelined walk (each x gather lands in an LDS
+// slot while the walk goes on to the next partner; the diagonal comes from global memory, so
+// the image in LDS drops its diagonal table and the 16-byte slots take its place): when it
+// keeps at least the workgroups per CU of the plain walk (LDS and the CU's 32 wavefronts).
+// 6x6 N=12 sector: 55.3 ms against 56.3 (y the same bit for bit). EDM_NARROW_PIPE=0 turns it
+// off, =2 turns it on wherever it fits the LDS.
+static bool narrow_pipe(const CompactTable &ct, const RankNarrow &r, int block)
+{
+    const char *v = getenv("EDM_NARROW_PIPE");
+    const int mode = v ? atoi(v) : 1;
+    if (mode == 0) return false;
+    const size_t jlo = ((size_t)r.lomask + 1) * sizeof(uint16_t);
+    const size_t plain = (((size_t)ct.lds_bytes + 15) & ~(size_t)15) + jlo;
+    const CompactDev cd = image_nodiag(compact_dev(ct));
+    const size_t pipe = (((size_t)cd.lds_bytes + 15) & ~(size_t)15) + jlo +
+                        (size_t)block * sizeof(double2);
+    if (pipe > kLdsMaxBytes) return false;
+    if (mode == 2) return true;
+    const size_t waves = (size_t)kCuThreads / (size_t)block;   // workgroups per CU by threads
+    return std::min(kLdsMaxBytes / pipe, waves) >= std::min(kLdsMaxBytes / plain, waves);
+}
+
+template <int BLOCK, bool PIPE>
 static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
                                 const double2 *x, double2 *y)
 {
     RankNarrow r;
     const uint16_t *jlo = nullptr;
     EDM_REQUIRE(narrow_view(b, r, jlo), EDM_EINVAL, "basis has no LDS rank split");
-    const size_t lds = narrow_lds_bytes(ct, r);
-    EDM_REQUIRE(lds <= kLdsMaxBytes, EDM_EINVAL, "term image and rank table exceed the LDS");
-    if (lds > 64 * 1024) {   // per-kernel opt-in beyond 64 KiB, once per device
+    const CompactDev cd = PIPE ? image_nodiag(compact_dev(ct)) : compact_dev(ct);
+    const size_t lds = (((size_t)cd.lds_bytes + 15) & ~(size_t)15) +
+                       ((size_t)r.lomask + 1) * sizeof(uint16_t);
+    const size_t slots = PIPE ? (size_t)BLOCK * sizeof(double2) : 0;   // static LDS
+    EDM_REQUIRE(lds + slots <= kLdsMaxBytes, EDM_EINVAL,
+                "term image and rank table exceed the LDS");
+    if (lds + slots > 64 * 1024) {   // per-kernel opt-in beyond 64 KiB, once per device
         static std::atomic<uint64_t> attr_set{0};
         const uint64_t bit = 1ULL << (ctx->device & 63);
         if (!(attr_set.load() & bit)) {
-            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_narrow<BLOCK>,
+            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_narrow<BLOCK, PIPE>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              (int)kLdsMaxBytes));
+                                              (int)(kLdsMaxBytes - slots)));
             attr_set.fetch_or(bit);
         }
     }
     const unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
     const int chunk = narrow_chunk();
-    hipLaunchKernelGGL((k_matvec_narrow<BLOCK>), dim3(nb), dim3(BLOCK), lds, ctx->stream, b->D,
-                       b->d_states, compact_dev(ct), r, jlo, x, y, chunk);
+    hipLaunchKernelGGL((k_matvec_narrow<BLOCK, PIPE>), dim3(nb), dim3(BLOCK), lds, ctx->stream,
+                       b->D, b->d_states, compact_dev(ct), r, jlo, x, y, chunk);
 }
 
 static void launch_narrow(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
                           const double2 *x, double2 *y, int block)
 {
-    if (block == 512) launch_narrow_block<512>(ctx, b, ct, x, y);
-    else launch_narrow_block<1024>(ctx, b, ct, x, y);
+    RankNarrow r;
+    const uint16_t *jlo = nullptr;
+    EDM_REQUIRE(narrow_view(b, r, jlo), EDM_EINVAL, "basis has no LDS rank split");
+    const bool pipe = narrow_pipe(ct, r, block == 512 ? 512 : 1024);
+    if (block == 512) {
+        if (pipe) launch_narrow_block<512, true>(ctx, b, ct, x, y);
+        else launch_narrow_block<512, false>(ctx, b, ct, x, y);
+    } else {
+        if (pipe) launch_narrow_block<1024, true>(ctx, b, ct, x, y);
+        else launch_narrow_block<1024, false>(ctx, b, ct, x, y);
+    }
 }
 
 template <class Rank>

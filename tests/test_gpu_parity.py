@@ -346,8 +346,9 @@ def test_matvec_split_rows_bitwise(gpu_ctx, oracle_mod, monkeypatch):
 def test_matvec_lds_rank_table_bitwise(gpu_ctx, oracle_mod, monkeypatch):
     """The kernel that stages the low-half rank table in LDS (k_matvec_narrow) walks each row
     as the global-table kernel does and looks partners up through the same kind of split:
-    its y is bitwise the one-lane kernel's (1024- and 512-row workgroups), and the one-lane y
-    is the oracle's to 1e-13. Cases: rank tables whose low half is already at most 14 bits
+    its y is bitwise the one-lane kernel's (1024- and 512-row workgroups, with the walk that
+    gathers x into LDS slots while it goes on and with the plain walk), and the one-lane y is
+    the oracle's to 1e-13. Cases: rank tables whose low half is already at most 14 bits
     (3x5 N=5, 3x9 N=9, two conserved components on 3x4 meshes), used as they are, and a
     30-orbital sector (5x6 mesh, N=7; 15 + 15 bit tables) whose 14 + 16 bit split the library
     derives from the state list on the device and checks on every state."""
@@ -388,8 +389,16 @@ def test_matvec_lds_rank_table_bitwise(gpu_ctx, oracle_mod, monkeypatch):
         for v, blk in (("u1n", 1024), ("u1n512", 512)):
             monkeypatch.setenv("EDM_MATVEC", v)
             assert H.kernel() == blk | dev.TUNE_NARROW, (name, v, hex(H.kernel()))
-            np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, d, basis, x), y1,
-                                          err_msg=f"{name} {v}")
+            # pipelined walk where it keeps the occupancy (default), plain walk, pipelined
+            # walk wherever it fits
+            for pipe in (None, "0", "2"):
+                if pipe is None:
+                    monkeypatch.delenv("EDM_NARROW_PIPE", raising=False)
+                else:
+                    monkeypatch.setenv("EDM_NARROW_PIPE", pipe)
+                np.testing.assert_array_equal(_gpu_matvec(gpu_ctx, d, basis, x), y1,
+                                              err_msg=f"{name} {v} pipe {pipe}")
+            monkeypatch.delenv("EDM_NARROW_PIPE")
         # the block kernel with the same lookup: every column bitwise the single matvec
         monkeypatch.setenv("EDM_MATVEC", "u1")
         nv = 4
@@ -680,6 +689,13 @@ def test_large_sector_enumeration_and_sampled_matvec(gpu_ctx, oracle_mod):
         assert torch.equal(y2, yd)
     finally:
         del os.environ["EDM_XCD_CHUNK"]
+    # the plain walk (the default gathers x into LDS slots while it walks on)
+    os.environ["EDM_NARROW_PIPE"] = "0"
+    try:
+        H(y2, xd)
+        assert torch.equal(y2, yd)
+    finally:
+        del os.environ["EDM_NARROW_PIPE"]
     os.environ["EDM_MATVEC"] = "u1"
     os.environ["EDM_MATVEC_LANES"] = "1"
     try:
