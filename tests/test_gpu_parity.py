@@ -351,7 +351,8 @@ def test_matvec_lds_rank_table_bitwise(gpu_ctx, oracle_mod, monkeypatch):
     the oracle's to 1e-13. Cases: rank tables whose low half is already at most 14 bits
     (3x5 N=5, 3x9 N=9, two conserved components on 3x4 meshes), used as they are, and a
     30-orbital sector (5x6 mesh, N=7; 15 + 15 bit tables) whose 14 + 16 bit split the library
-    derives from the state list on the device and checks on every state."""
+    derives from the state list on the device and checks on every state; and a general
+    (non-Hermitian) operator with complex diagonal terms on a bare list of states."""
     cases = []
     ws = W.small()
     dops = dev.DeviceOperator.from_mboperator(gpu_ctx, ws.operator(), ws.para.bits)
@@ -374,6 +375,21 @@ def test_matvec_lds_rank_table_bitwise(gpu_ctx, oracle_mod, monkeypatch):
     b56 = dev.DeviceBasis.enumerate(gpu_ctx, _system(para56, (7,)), 0)
     assert b56.rank_mode == 1 and b56.D > 50000
     cases.append(("5x6 N=7", op56, dop56, b56))
+    # a general operator (not Hermitian) with complex diagonal terms on all 6-particle states
+    # of 16 orbitals: the pipelined walk sums the diagonal apart from the walk
+    rng = np.random.default_rng(41)
+    terms = list(_random_general_op(rng, 16).scats)
+    for _ in range(12):
+        a, b_ = rng.choice(16, 2, replace=False) + 1
+        terms.append(me.Scatter(complex(rng.normal(), rng.normal()), int(a), int(b_), int(b_),
+                                int(a), bits=16))
+    for a in range(1, 17, 3):
+        terms.append(me.Scatter(complex(rng.normal(), rng.normal()), a, a, bits=16))
+    opg = me.MBOperator(terms, upper_hermitian=False)
+    states = np.array([v for v in range(1 << 16) if bin(v).count("1") == 6], dtype=np.uint64)
+    bg = dev.device_basis_of(me.HilbertSubspace(states, bits=16), gpu_ctx)
+    cases.append(("general, complex diagonal", opg,
+                  dev.DeviceOperator.from_mboperator(gpu_ctx, opg, 16), bg))
     for name, op, d, basis in cases:
         x = _rand(basis.D, 93)
         H = dev.ROCmLinearMap(d, basis)
