@@ -669,6 +669,12 @@ __device__ __forceinline__ void lds_dma_wait()
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt 0, expcnt 7, lgkmcnt 15: only vmcnt waits
     asm volatile("" ::: "memory");
 }
+// s_waitcnt lgkmcnt(0): the wavefront's LDS reads have returned their data.
+__device__ __forceinline__ void lds_read_wait()
+{
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // vmcnt 63, expcnt 7, lgkmcnt 0: only lgkmcnt waits
+    asm volatile("" ::: "memory");
+}
 // Pipe: each accepted partner's x entry is fetched straight into this lane's 16-byte LDS slot
 // (global_load_lds, no registers held while it is in flight) and multiplied in when the lane
 // takes its next partner (or at the end of the row), so the gather overlaps the walk to the
@@ -707,6 +713,7 @@ __device__ __forceinline__ double2 row_one(int64_t i, uint64_t s, const unsigned
             if (pend) {
                 lds_dma_wait();
                 cmac(yr, yi, flip_sign(par, pneg), flip_sign(pai, pneg), *lane_slot);
+                lds_read_wait();   // the slot has been read before the next load refills it
             }
             asm volatile("" ::: "memory");
             const double2 a = t.amp_pm[pmi];
