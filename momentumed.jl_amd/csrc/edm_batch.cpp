@@ -305,11 +305,31 @@ struct edm_problem {
     double &t(int a, int c) { return T[(size_t)a * m + c]; }
 };
 
+// One interleave job of a step: the G x vectors (D rows) of a shared-walk group copied into xi.
+struct XJob {
+    int G;
+    int64_t D;
+    const double2 *x[8];
+    double2 *xi;
+};
+
+// Rows from which a shared-walk group gathers x from an interleaved copy of its vectors: below,
+// the vectors are small enough for the L2 and the copy would not pay. EDM_BATCH_INTERLEAVE_MIN
+// overrides (0: every group, -1: none).
+static int64_t batch_interleave_min()
+{
+    const char *v = getenv("EDM_BATCH_INTERLEAVE_MIN");
+    if (!v) return (int64_t)1 << 18;
+    return (int64_t)atoll(v);
+}
+
 struct edm_batch {
     edm_ctx *ctx = nullptr;
     std::map<int64_t, edm_problem *> probs;   // ordered by id: launch order is admission order
     PinnedBuf h_plan, h_coef, h_rot;
     DevArena d_plan, d_coef, d_rot;
+    DevArena d_xi;               // interleaved x copies of large shared-walk groups (per step)
+    std::vector<XJob> xjobs;     // this step's interleave jobs
     SlabArena slabs;
     HostPool pool;
     // host scratch of the per-step plan, kept between steps so that building a plan does
@@ -682,36 +702,68 @@ static int batch_step_once(edm_batch *bt)
             else
                 solo.push_back(p);
         }
-        const edm_op *ops[8];
-        const double2 *xs[8];
-        double2 *ys[8];
+        // groups: runs of up to maxg problems of one (basis, structure); large ones gather x
+        // from an interleaved copy of the group's vectors (interleave jobs, run before the
+        // shared walk), carved from one arena per step
+        std::vector<std::pair<edm_problem *const *, int>> groups;
         for (auto &kv : byshape) {
             std::vector<edm_problem *> &v = kv.second;
             for (size_t k = 0; k < v.size();) {
                 const int G = (int)std::min<size_t>((size_t)maxg, v.size() - k);
-                if (G == 1) {
-                    solo.push_back(v[k++]);
-                    continue;
-                }
-                for (int g = 0; g < G; ++g) {
-                    ops[g] = v[k + g]->op;
-                    xs[g] = v[k + g]->V[v[k + g]->j];
-                    ys[g] = v[k + g]->V[v[k + g]->j + 1];
-                }
-                std::vector<unsigned char> &dv = bt->gdesc[G];
-                std::vector<int> &bv = bt->gblk[G];
-                const size_t at = dv.size();
-                dv.resize(at + gsz);
-                size_t l = 0;
-                if (batch_mv_group_desc(ops, v[k]->b, xs, ys, G, (int64_t)bv.size(),
-                                        dv.data() + at, &l)) {
-                    bv.insert(bv.end(), (size_t)batch_mv_blocks(v[k]->D), (int)(at / gsz));
-                    lds = std::max(lds, l);
-                } else {
-                    dv.resize(at);
-                    for (int g = 0; g < G; ++g) solo.push_back(v[k + g]);
-                }
+                if (G == 1) solo.push_back(v[k]);
+                else groups.emplace_back(v.data() + k, G);
                 k += (size_t)G;
+            }
+        }
+        const int64_t imin = batch_interleave_min();
+        size_t xi_bytes = 0;
+        for (auto &gr : groups)
+            if (imin >= 0 && gr.first[0]->D >= imin)
+                xi_bytes += align_up((size_t)gr.second * gr.first[0]->D * sizeof(double2));
+        bool inter = xi_bytes > 0;
+        if (inter) {
+            try {
+                bt->d_xi.ensure(ctx, xi_bytes);
+            } catch (const EdmError &e) {
+                if (e.code != EDM_ENOMEM) throw;
+                inter = false;   // no room for the copies: gather from the vectors themselves
+            }
+        }
+        bt->xjobs.clear();
+        size_t xi_at = 0;
+        const edm_op *ops[8];
+        const double2 *xs[8];
+        double2 *ys[8];
+        for (auto &gr : groups) {
+            edm_problem *const *v = gr.first;
+            const int G = gr.second;
+            for (int g = 0; g < G; ++g) {
+                ops[g] = v[g]->op;
+                xs[g] = v[g]->V[v[g]->j];
+                ys[g] = v[g]->V[v[g]->j + 1];
+            }
+            double2 *xi = nullptr;
+            if (inter && v[0]->D >= imin) {
+                xi = (double2 *)((unsigned char *)bt->d_xi.p + xi_at);
+                xi_at += align_up((size_t)G * v[0]->D * sizeof(double2));
+            }
+            std::vector<unsigned char> &dv = bt->gdesc[G];
+            std::vector<int> &bv = bt->gblk[G];
+            const size_t at = dv.size();
+            dv.resize(at + gsz);
+            size_t l = 0;
+            if (batch_mv_group_desc(ops, v[0]->b, xs, ys, G, (int64_t)bv.size(), dv.data() + at,
+                                    &l, xi)) {
+                bv.insert(bv.end(), (size_t)batch_mv_blocks(v[0]->D), (int)(at / gsz));
+                lds = std::max(lds, l);
+                if (xi) {
+                    XJob job{G, v[0]->D, {}, xi};
+                    for (int g = 0; g < G; ++g) job.x[g] = xs[g];
+                    bt->xjobs.push_back(job);
+                }
+            } else {
+                dv.resize(at);
+                for (int g = 0; g < G; ++g) solo.push_back(v[g]);
             }
         }
     }
@@ -787,6 +839,7 @@ static int batch_step_once(edm_batch *bt)
             launch_matvec(ctx, p->op, p->b, p->V[p->j], p->V[p->j + 1], ApplyMode::Forward);
         launch_batch_mv(ctx, dplan_base + mv_off, (const int *)(dplan_base + mvblk_off),
                         (int64_t)mvblk.size(), lds);
+        for (const XJob &job : bt->xjobs) launch_batch_interleave(ctx, job.G, job.x, job.D, job.xi);
         for (int g = 2; g <= maxg; ++g)
             launch_batch_mv_group(ctx, g, dplan_base + gdesc_off[g],
                                   (const int *)(dplan_base + gblk_off[g]),
@@ -867,6 +920,7 @@ void batch_destroy(edm_batch *bt)
     bt->slabs.release_all(ctx);
     bt->d_plan.release(ctx);
     bt->d_coef.release(ctx);
+    bt->d_xi.release(ctx);
     bt->d_rot.release(ctx);
     delete bt;
 }

@@ -365,8 +365,13 @@ void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, i
 int batch_mv_max_group();
 size_t batch_mv_group_desc_size();
 bool batch_mv_same_shape(const edm_op *a, const edm_op *b);
+// xi: the group's x vectors interleaved by launch_batch_interleave (nullptr: read x[g])
 bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const double2 *const *x,
-                         double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes);
+                         double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes,
+                         const double2 *xi);
+// xi[j * G + g] = x[g][j] for the D rows of G vectors (G = 2..batch_mv_max_group())
+void launch_batch_interleave(edm_ctx *ctx, int G, const double2 *const *x, int64_t D,
+                             double2 *xi);
 void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,
                            int64_t nblocks, size_t lds_bytes);
 // multi-problem Lanczos engine (edm_batch.cpp)

@@ -1410,9 +1410,12 @@ struct BatchMvGroupDesc {
     const double2 *diag[kMaxGroup];
     const double2 *x[kMaxGroup];
     double2 *y[kMaxGroup];
+    // the group's x vectors interleaved, xi[j * G + g] = x[g][j] (nullptr: read x[g]); a
+    // partner's G entries then share one 16 G-byte run instead of G cache lines
+    const double2 *xi;
 };
 
-template <class Rank, int G>
+template <class Rank, int G, bool Inter>
 __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned char *base,
                                           const BatchMvGroupDesc &P, const Rank &rank)
 {
@@ -1429,7 +1432,7 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const double2 a = P.amp_pm[g][pmi];
-            const double2 xv = P.x[g][j];
+            const double2 xv = Inter ? P.xi[j * G + g] : P.x[g][j];
             cmac(yr[g], yi[g], flip_sign(a.x, neg), flip_sign(a.y, neg), xv);
         }
     };
@@ -1486,7 +1489,7 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
     }
 #pragma unroll
     for (int g = 0; g < G; ++g) {
-        cmac(yr[g], yi[g], dr[g], di[g], P.x[g][i]);
+        cmac(yr[g], yi[g], dr[g], di[g], Inter ? P.xi[i * G + g] : P.x[g][i]);
         P.y[g][i] = make_double2(yr[g], yi[g]);
     }
 }
@@ -1506,9 +1509,15 @@ k_batch_mv_group(const BatchMvGroupDesc *__restrict__ descs, const int *__restri
     const int64_t i = (blk - P.blk0) * blockDim.x + threadIdx.x;
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
-    if (P.use_lin == 2) row_group<RankNarrow, G>(i, P.states[i], base, P, nar);
-    else if (P.use_lin) row_group<RankLin, G>(i, P.states[i], base, P, P.lin);
-    else row_group<RankSearch, G>(i, P.states[i], base, P, P.search);
+    if (P.xi) {
+        if (P.use_lin == 2) row_group<RankNarrow, G, true>(i, P.states[i], base, P, nar);
+        else if (P.use_lin) row_group<RankLin, G, true>(i, P.states[i], base, P, P.lin);
+        else row_group<RankSearch, G, true>(i, P.states[i], base, P, P.search);
+    } else {
+        if (P.use_lin == 2) row_group<RankNarrow, G, false>(i, P.states[i], base, P, nar);
+        else if (P.use_lin) row_group<RankLin, G, false>(i, P.states[i], base, P, P.lin);
+        else row_group<RankSearch, G, false>(i, P.states[i], base, P, P.search);
+    }
 }
 
 // ---------------------------------------------------------------- eigenvector consumers
@@ -2563,7 +2572,8 @@ bool batch_mv_same_shape(const edm_op *a, const edm_op *b)
 }
 
 bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const double2 *const *x,
-                         double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes)
+                         double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes,
+                         const double2 *xi)
 {
     EDM_REQUIRE(G >= 2 && G <= kMaxGroup, EDM_EINVAL, "group size out of range");
     if (matvec_variant() != 11 || matvec_passes() != 1) return false;
@@ -2582,6 +2592,7 @@ bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const dou
     if (lin) d.lin = make_lin(b);
     d.search = make_search(b);
     d.blk0 = blk0;
+    d.xi = xi;
     for (int g = 0; g < G; ++g) {
         d.amp_pm[g] = ops[g]->cfwd.d_amp_pm;
         d.diag[g] = ops[g]->cfwd.d_diag;
@@ -2597,6 +2608,34 @@ bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const dou
     }
     memcpy(out, &d, sizeof(d));
     return true;
+}
+
+void launch_batch_interleave(edm_ctx *ctx, int G, const double2 *const *x, int64_t D,
+                             double2 *xi)
+{
+    EDM_REQUIRE(G >= 2 && G <= kMaxGroup, EDM_EINVAL, "group size out of range");
+    if (D == 0) return;
+    switch (G) {
+    case 2: {
+        CVecPtrs<2> p{{x[0], x[1]}};
+        hipLaunchKernelGGL(k_interleave<2>, dim3(nblk(D)), dim3(kRowBlock), 0, ctx->stream, D, p,
+                           xi);
+        break;
+    }
+    case 3: {
+        CVecPtrs<3> p{{x[0], x[1], x[2]}};
+        hipLaunchKernelGGL(k_interleave<3>, dim3(nblk(D)), dim3(kRowBlock), 0, ctx->stream, D, p,
+                           xi);
+        break;
+    }
+    default: {
+        CVecPtrs<4> p{{x[0], x[1], x[2], x[3]}};
+        hipLaunchKernelGGL(k_interleave<4>, dim3(nblk(D)), dim3(kRowBlock), 0, ctx->stream, D, p,
+                           xi);
+        break;
+    }
+    }
+    EDM_HIP_CHECK(hipGetLastError());
 }
 
 void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,

@@ -91,10 +91,15 @@ def test_batch_two_component_uneven_sizes(gpu_ctx, oracle_mod):
     assert checked == 2
 
 
-def test_batch_shared_walk_groups_bitwise_equal_alone(gpu_ctx):
+@pytest.mark.parametrize("interleave", [None, "0"])
+def test_batch_shared_walk_groups_bitwise_equal_alone(gpu_ctx, monkeypatch, interleave):
     """Nine parameter points on each of two bases: the batch walks each basis's rows once
     for up to four problems (groups of 4, 4 and 1 per basis); every problem's values,
-    vectors and matvec count are still those of its lone solve."""
+    vectors and matvec count are still those of its lone solve. Also with every group
+    gathering x from an interleaved copy of its vectors (by default only groups of 2^18 rows
+    and more do)."""
+    if interleave is not None:
+        monkeypatch.setenv("EDM_BATCH_INTERLEAVE_MIN", interleave)
     ratios = [0.55, 0.7, 0.85, 1.1, 1.25, 1.4, 1.6, 1.8, 1.95]
     probs = _two_component_problems(gpu_ctx, ratios, [(3, 4)], [0, 5])
     with B.BatchSolver(gpu_ctx) as bs:
