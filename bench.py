@@ -351,7 +351,8 @@ def literal_section(args):
     components on a 4x6 mesh each (48 orbitals), N_up = N_down = 4, sector (0,0), ratio 1.3
     (D = 4,705,212): basis enumeration, matvec time, and the lowest 4 by the single-vector
     iteration (krylovdim 30) and by block Lanczos with block size 4 (krylovdim 60), each
-    timed alone, with the sweep's time extrapolated from them (24 sectors x 64 ratios)."""
+    timed alone, with the sweep's time extrapolated from them (24 sectors x 64 ratios); and
+    the batched form at this size: 8 ratios of the sector in one batch against single solves."""
     import torch
     import momentumed as me
     from momentumed import device as dev
@@ -393,12 +394,43 @@ def literal_section(args):
                                 "eigenvalues": [float(e) for e in v]}
         log(f"[literal] block {bs}: {ts:.2f} s, {info['matvecs']} matvecs, values {v}")
     log(f"[literal] D={b.D} enumerated in {t_enum:.2f} s, matvec {ms_mv:.3f} ms")
+    # the sweep's batched form at this size: the first 8 of the sweep's 64 ratios on the same
+    # sector in one batch (two shared-walk groups of four) against single solves of the first 2
+    import numpy as np
+    from momentumed import batch as B
+    ratios = W.batched_ratios(64)[:8]
+    maps = []
+    for r in ratios:
+        pr = W.literal_batched_para(float(r))
+        o = me.MBOperator(me.ED_scatterlist_twobody(pr), upper_hermitian=True)
+        maps.append(dev.ROCmLinearMap(dev.DeviceOperator.from_mboperator(ctx, o, pr.bits), b))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plain = [dev.krylov_map_solve(m, 4, krylovdim=30, tol=1e-12, want_vectors=False)[0]
+             for m in maps[:2]]
+    t_plain = (time.perf_counter() - t0) / 2
+    t0 = time.perf_counter()
+    with B.BatchSolver(ctx) as bs:
+        ids = [bs.add(m, 4, krylovdim=30, tol=1e-12) for m in maps]
+        res = bs.run()
+    t_batch = time.perf_counter() - t0
+    same = all(np.array_equal(res[ids[i]].values, plain[i]) for i in range(2))
+    batched = {"problems": len(maps), "seconds": round(t_batch, 3),
+               "seconds_per_solve": round(t_batch / len(maps), 3),
+               "plain_seconds_per_solve": round(t_plain, 3),
+               "speedup_vs_plain": round(t_plain * len(maps) / t_batch, 2),
+               "sample_bitwise_equal": bool(same),
+               "note": "the first 8 of the sweep's 64 ratios, sector (0,0), one batch; plain: the "
+                       "first 2 solved one after another"}
+    log(f"[literal] batched: {len(maps)} solves in {t_batch:.2f} s ({t_batch / len(maps):.2f} s "
+        f"each) against {t_plain:.2f} s alone: {batched['speedup_vs_plain']}x, bitwise {same}")
     t1 = solves["block1"]["seconds"]
     return {"workload": "two conserved components, 4x6 mesh each (48 orbitals), N_up = N_down = 4, "
                         "sector (0,0), inter-component ratio 1.3",
             "D": b.D, "rank_mode": b.rank_mode, "enumerate_s": round(t_enum, 3),
             "matvec_ms": round(ms_mv, 4), "solves": solves,
             "sweep_s_extrapolated": round(t1 * 24 * 64, 1),
+            "batched": batched,
             "note": "single solves timed alone; the sweep figure is 1,536 such solves one after "
                     "another (extrapolated, not run)"}
 
