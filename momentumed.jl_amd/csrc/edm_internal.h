@@ -107,6 +107,15 @@ struct edm_basis {
     uint32_t *d_jhi_n = nullptr;
     uint16_t *d_jlo_n16 = nullptr;
     int nhi_n = 0;
+    // where J_hi over nbits - kNarrowLo bits would be too large (bases above 38 bits), the same
+    // split with J_hi itself in two levels: the high part h = t >> kNarrowLo is cut into its top
+    // 24 bits and n2_m low bits; d_n2_top[top] is the row of that top value (tops present in
+    // the basis, in order) and d_n2_rows[row << n2_m | (h & mask)] the first index of the states
+    // with high part h. J_lo is d_jlo_n16.
+    uint32_t *d_n2_top = nullptr;
+    uint32_t *d_n2_rows = nullptr;
+    int n2_m = 0;
+    int64_t n2_nrows = 0;
     bool has_spec = false;
     SystemSpec spec;
     int64_t sector = -1;

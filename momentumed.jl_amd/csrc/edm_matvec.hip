@@ -644,6 +644,80 @@ __device__ __forceinline__ int64_t rank_find(const NarrowRow &r, const NarrowMid
     const uint32_t hv = h == m.hm ? m.jhm : r.jhi[h];
     return (int64_t)(hv + (uint32_t)r.sjlo[(uint32_t)t & r.lomask]);
 }
+// The split at kNarrowLo with J_hi in two levels (bases above kNarrowLo + 24 bits; the
+// tables of edm_basis::d_n2_top / d_n2_rows): the high part h = t >> kNarrowLo is cut into
+// its top kNarrow2Top bits, which select a row, and m low bits, which index the row. A partner
+// that shares h with mid takes mid's word, one that shares the top takes mid's row.
+constexpr int kNarrow2Top = 24;
+constexpr size_t kNarrow2RowsMax = (size_t)128 << 20;   // bytes of rows at most
+struct RankNarrow2 {
+    const uint32_t *top;
+    const uint32_t *rows;
+    const uint16_t *sjlo;
+    int m;
+    uint32_t lomask;   // (1 << kNarrowLo) - 1, as RankNarrow's
+};
+template <> struct RankMayMiss<RankNarrow2> { static constexpr bool value = false; };
+struct Narrow2Row {
+    const uint32_t *top, *rows;
+    const uint16_t *sjlo;
+    int m;
+    uint64_t hs;        // the row's high part
+    uint32_t tops, rs, jhs;
+};
+struct Narrow2Mid {
+    uint64_t hm;
+    uint32_t topm, rm, jhm;
+};
+__device__ __forceinline__ Narrow2Row rank_row32(const RankNarrow2 &r, uint64_t s)
+{
+    const uint64_t hs = s >> kNarrowLo;
+    const uint32_t tp = (uint32_t)(hs >> r.m);
+    const uint32_t rs = r.top[tp];
+    const uint32_t mm = (1u << r.m) - 1u;
+    return Narrow2Row{r.top, r.rows, r.sjlo, r.m, hs, tp, rs,
+                      r.rows[((size_t)rs << r.m) | ((uint32_t)hs & mm)]};
+}
+__device__ __forceinline__ Narrow2Mid rank_mid32(const Narrow2Row &r, uint64_t mid)
+{
+    const uint64_t hm = mid >> kNarrowLo;
+    if (hm == r.hs) return Narrow2Mid{hm, r.tops, r.rs, r.jhs};
+    const uint32_t tp = (uint32_t)(hm >> r.m);
+    const uint32_t rm = tp == r.tops ? r.rs : r.top[tp];
+    // mid has one or two particles fewer than a basis state, so its top part may be the top of
+    // no state (no row): then no partner shares mid's top or high part, and mid's words are
+    // never used; the rows must not be read with the missing row's index
+    if (rm == 0xFFFFFFFFu) return Narrow2Mid{hm, tp, rm, 0u};
+    const uint32_t mm = (1u << r.m) - 1u;
+    return Narrow2Mid{hm, tp, rm, r.rows[((size_t)rm << r.m) | ((uint32_t)hm & mm)]};
+}
+__device__ __forceinline__ int64_t rank_find(const Narrow2Row &r, const Narrow2Mid &m,
+                                             uint64_t t)
+{
+    const uint64_t h = t >> kNarrowLo;
+    uint32_t hv = m.jhm;
+    if (h != m.hm) {
+        const uint32_t tp = (uint32_t)(h >> r.m);
+        const uint32_t rr = tp == m.topm ? m.rm : r.top[tp];
+        hv = r.rows[((size_t)rr << r.m) | ((uint32_t)h & ((1u << r.m) - 1u))];
+    }
+    return (int64_t)(hv + (uint32_t)r.sjlo[(uint32_t)t & ((1u << kNarrowLo) - 1u)]);
+}
+// the two-level tables read from global memory (checks on every basis state)
+struct RankNarrow2Global {
+    const uint32_t *top, *rows;
+    const uint16_t *jlo;
+    int m;
+    __device__ __forceinline__ int64_t operator()(uint64_t t) const
+    {
+        const uint64_t h = t >> kNarrowLo;
+        const uint32_t r = top[h >> m];
+        if (r == 0xFFFFFFFFu) return -1;
+        const uint32_t hv = rows[((size_t)r << m) | (size_t)(h & ((1ULL << m) - 1ULL))];
+        return (int64_t)hv + (int64_t)jlo[t & ((1ULL << kNarrowLo) - 1ULL)];
+    }
+};
+
 // the same tables read from global memory (checks of the split, on every basis state)
 struct RankNarrowGlobal {
     const uint32_t *jhi;
@@ -1125,7 +1199,8 @@ k_matvec_ldsa(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Rank
 // 512-row kernel (two workgroups of the ~67 KiB image per CU on the 6x6 N=12 sector).
 // the 2^nlo-entry J_lo, 16 bytes per thread per step, 16-byte aligned after the term image;
 // returns its LDS address (valid after the barrier that follows)
-__device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, const RankNarrow &r,
+template <class R>
+__device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, const R &r,
                                                         const uint16_t *__restrict__ jlo,
                                                         uint4 *lds)
 {
@@ -1137,9 +1212,9 @@ __device__ __forceinline__ const uint16_t *stage_narrow(const CompactDev &t, con
     return (const uint16_t *)((const unsigned char *)lds + off);
 }
 
-template <int BLOCK, bool PIPE = false>
+template <int BLOCK, bool PIPE = false, class R = RankNarrow>
 __global__ void __launch_bounds__(BLOCK)
-k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, RankNarrow rank,
+k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, R rank,
                 const uint16_t *__restrict__ jlo, const double2 *__restrict__ x,
                 double2 *__restrict__ y, int chunk)
 {
@@ -1157,12 +1232,12 @@ k_matvec_narrow(int64_t D, const uint64_t *__restrict__ states, CompactDev t, Ra
     const int64_t i = xcd_chunk_block(chunk) * BLOCK + threadIdx.x;
     if (i >= D) return;
     if constexpr (PIPE)
-        y[i] = row_one<RankNarrow, true, false, true>(
+        y[i] = row_one<R, true, false, true>(
             i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0, INT64_MAX, 1, false,
             pipe_slots + (threadIdx.x & ~63u), pipe_slots + threadIdx.x);
     else
-        y[i] = row_one<RankNarrow, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x,
-                                         0, INT64_MAX, 1, false);
+        y[i] = row_one<R, true>(i, states[i], (const unsigned char *)lds_u4, t, rank, x, 0,
+                                INT64_MAX, 1, false);
 }
 
 // J_hi_n[h] = index of the first state whose high part (bits >= nlo) is h (entries of absent
@@ -1919,15 +1994,97 @@ void free_narrow_tables(edm_basis *b)
 {
     if (b->d_jhi_n) b->ctx->free(b->d_jhi_n, (1ULL << b->nhi_n) * sizeof(uint32_t));
     if (b->d_jlo_n16) b->ctx->free(b->d_jlo_n16, (1ULL << kNarrowLo) * sizeof(uint16_t));
+    if (b->d_n2_top) b->ctx->free(b->d_n2_top, (1ULL << kNarrow2Top) * sizeof(uint32_t));
+    if (b->d_n2_rows)
+        b->ctx->free(b->d_n2_rows, ((size_t)b->n2_nrows << b->n2_m) * sizeof(uint32_t));
     b->d_jhi_n = nullptr;
     b->d_jlo_n16 = nullptr;
     b->nhi_n = 0;
+    b->d_n2_top = nullptr;
+    b->d_n2_rows = nullptr;
+    b->n2_m = 0;
+    b->n2_nrows = 0;
+}
+
+// The two-level form of the split (bases of kNarrowLo + kNarrow2Top + 1 .. + 12 bits), derived
+// on the host from the sorted states, uploaded, and checked on every state on the device; no
+// tables (and the global-table kernel) when a state's position in its block exceeds 16 bits,
+// a low part would need two positions, or the rows exceed kNarrow2RowsMax bytes.
+static void build_narrow2_tables(edm_basis *b)
+{
+    const int m = b->nbits - kNarrowLo - kNarrow2Top;
+    if (b->rank_mode != 1 || m < 1 || m > 12 || b->D <= 0 || b->D > 0xFFFFFFFFLL) return;
+    edm_ctx *ctx = b->ctx;
+    const int64_t D = b->D;
+    std::vector<uint64_t> st((size_t)D);
+    EDM_HIP_CHECK(hipMemcpyAsync(st.data(), b->d_states, (size_t)D * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    const uint64_t mmask = (1ULL << m) - 1ULL, lmask = (1ULL << kNarrowLo) - 1ULL;
+    std::vector<uint32_t> top((size_t)1 << kNarrow2Top, 0xFFFFFFFFu);
+    std::vector<uint32_t> rows;
+    std::vector<uint16_t> jlo((size_t)1 << kNarrowLo, 0);
+    std::vector<uint8_t> seen((size_t)1 << kNarrowLo, 0);
+    uint64_t prev_top = ~0ULL, prev_h = ~0ULL;
+    int64_t first = 0, nrows = 0;
+    for (int64_t i = 0; i < D; ++i) {
+        const uint64_t t = st[(size_t)i], h = t >> kNarrowLo, tp = h >> m;
+        if (tp != prev_top) {
+            if ((size_t)(nrows + 1) << m > kNarrow2RowsMax / sizeof(uint32_t)) return;
+            top[(size_t)tp] = (uint32_t)nrows++;
+            rows.resize((size_t)nrows << m, 0xFFFFFFFFu);
+            prev_top = tp;
+        }
+        if (h != prev_h) {
+            rows[((size_t)(nrows - 1) << m) | (size_t)(h & mmask)] = (uint32_t)i;
+            first = i;
+            prev_h = h;
+        }
+        const int64_t pos = i - first;
+        const size_t l = (size_t)(t & lmask);
+        if (pos > 0xFFFF || (seen[l] && jlo[l] != (uint16_t)pos)) return;
+        jlo[l] = (uint16_t)pos;
+        seen[l] = 1;
+    }
+    b->n2_m = m;
+    b->n2_nrows = nrows;
+    b->d_n2_top = (uint32_t *)ctx->alloc(top.size() * sizeof(uint32_t));
+    b->d_n2_rows = (uint32_t *)ctx->alloc(rows.size() * sizeof(uint32_t));
+    b->d_jlo_n16 = (uint16_t *)ctx->alloc(jlo.size() * sizeof(uint16_t));
+    EDM_HIP_CHECK(hipMemcpyAsync(b->d_n2_top, top.data(), top.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    EDM_HIP_CHECK(hipMemcpyAsync(b->d_n2_rows, rows.data(), rows.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    EDM_HIP_CHECK(hipMemcpyAsync(b->d_jlo_n16, jlo.data(), jlo.size() * sizeof(uint16_t),
+                                 hipMemcpyHostToDevice, ctx->stream));
+    unsigned int *d_bad = (unsigned int *)ctx->alloc(sizeof(unsigned int));
+    unsigned int bad = 0;
+    try {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, ctx->stream, d_bad, 1, 0u);
+        const RankNarrow2Global chk{b->d_n2_top, b->d_n2_rows, b->d_jlo_n16, m};
+        hipLaunchKernelGGL(k_verify_rank<RankNarrow2Global>, dim3(nblk(D)), dim3(kRowBlock), 0,
+                           ctx->stream, D, b->d_states, chk, d_bad);
+        EDM_HIP_CHECK(hipGetLastError());
+        EDM_HIP_CHECK(hipMemcpyAsync(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        EDM_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->free(d_bad, sizeof(unsigned int));
+        throw;
+    }
+    ctx->free(d_bad, sizeof(unsigned int));
+    if (bad != 0) free_narrow_tables(b);
 }
 
 void build_narrow_tables(edm_basis *b)
 {
     free_narrow_tables(b);
     const int nhi = b->nbits - kNarrowLo;
+    if (b->rank_mode == 1 && nhi > 24 && b->D >= ((int64_t)1 << 18)) {
+        build_narrow2_tables(b);
+        return;
+    }
     // only where the tables split wider than kNarrowLo, J_hi_n stays within 64 MB and within
     // the basis' own 8 D bytes (a small sector of a wide system keeps its tables)
     if (b->rank_mode != 1 || b->nlo <= kNarrowLo || nhi < 1 || nhi > 24 || b->D <= 0 ||
@@ -2147,7 +2304,17 @@ static bool narrow_view(const edm_basis *b, RankNarrow &r, const uint16_t *&jlo)
     return false;
 }
 
-static size_t narrow_lds_bytes(const CompactTable &ct, const RankNarrow &r)
+// The two-level view (edm_basis::d_n2_top), when it was built.
+static bool narrow2_view(const edm_basis *b, RankNarrow2 &r, const uint16_t *&jlo)
+{
+    if (b->rank_mode != 1 || !b->d_n2_top || !b->d_n2_rows || !b->d_jlo_n16) return false;
+    r = RankNarrow2{b->d_n2_top, b->d_n2_rows, nullptr, b->n2_m, (1u << kNarrowLo) - 1u};
+    jlo = b->d_jlo_n16;
+    return true;
+}
+
+template <class R>
+static size_t narrow_lds_bytes(const CompactTable &ct, const R &r)
 {
     return (((size_t)ct.lds_bytes + 15) & ~(size_t)15) + ((size_t)r.lomask + 1) * sizeof(uint16_t);
 }
@@ -2160,6 +2327,21 @@ static bool narrow_ok(const edm_op *op, const edm_basis *b, const CompactTable &
     const uint16_t *jlo;
     return ct.ok && use_lin(op, b) && narrow_view(b, r, jlo) &&
            narrow_lds_bytes(ct, r) <= kLdsMaxBytes;
+}
+
+// the same with the two-level view (single-vector matvec only)
+static bool narrow2_ok(const edm_op *op, const edm_basis *b, const CompactTable &ct)
+{
+    RankNarrow2 r;
+    const uint16_t *jlo;
+    return ct.ok && use_lin(op, b) && narrow2_view(b, r, jlo) &&
+           narrow_lds_bytes(ct, r) <= kLdsMaxBytes;
+}
+
+// the single-vector LDS rank-table kernel, with either view
+static bool narrow_any_ok(const edm_op *op, const edm_basis *b, const CompactTable &ct)
+{
+    return narrow_ok(op, b, ct) || narrow2_ok(op, b, ct);
 }
 
 // Workgroup order of the LDS rank-table kernel: each XCD takes kNarrowChunk consecutive row
@@ -2183,7 +2365,8 @@ static int narrow_chunk()
 // keeps at least the workgroups per CU of the plain walk (LDS and the CU's 32 wavefronts).
 // 6x6 N=12 sector: 55.3 ms against 56.3 (y the same bit for bit). EDM_NARROW_PIPE=0 turns it
 // off, =2 turns it on wherever it fits the LDS.
-static bool narrow_pipe(const CompactTable &ct, const RankNarrow &r, int block)
+template <class R>
+static bool narrow_pipe(const CompactTable &ct, const R &r, int block)
 {
     const char *v = getenv("EDM_NARROW_PIPE");
     const int mode = v ? atoi(v) : 1;
@@ -2199,13 +2382,10 @@ static bool narrow_pipe(const CompactTable &ct, const RankNarrow &r, int block)
     return std::min(kLdsMaxBytes / pipe, waves) >= std::min(kLdsMaxBytes / plain, waves);
 }
 
-template <int BLOCK, bool PIPE>
+template <int BLOCK, bool PIPE, class R>
 static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
-                                const double2 *x, double2 *y)
+                                const R &r, const uint16_t *jlo, const double2 *x, double2 *y)
 {
-    RankNarrow r;
-    const uint16_t *jlo = nullptr;
-    EDM_REQUIRE(narrow_view(b, r, jlo), EDM_EINVAL, "basis has no LDS rank split");
     const CompactDev cd = PIPE ? image_nodiag(compact_dev(ct)) : compact_dev(ct);
     const size_t lds = (((size_t)cd.lds_bytes + 15) & ~(size_t)15) +
                        ((size_t)r.lomask + 1) * sizeof(uint16_t);
@@ -2216,7 +2396,7 @@ static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactT
         static std::atomic<uint64_t> attr_set{0};
         const uint64_t bit = 1ULL << (ctx->device & 63);
         if (!(attr_set.load() & bit)) {
-            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_narrow<BLOCK, PIPE>,
+            EDM_HIP_CHECK(hipFuncSetAttribute((const void *)k_matvec_narrow<BLOCK, PIPE, R>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize,
                                               (int)(kLdsMaxBytes - slots)));
             attr_set.fetch_or(bit);
@@ -2224,24 +2404,34 @@ static void launch_narrow_block(edm_ctx *ctx, const edm_basis *b, const CompactT
     }
     const unsigned nb = (unsigned)((b->D + BLOCK - 1) / BLOCK);
     const int chunk = narrow_chunk();
-    hipLaunchKernelGGL((k_matvec_narrow<BLOCK, PIPE>), dim3(nb), dim3(BLOCK), lds, ctx->stream,
+    hipLaunchKernelGGL((k_matvec_narrow<BLOCK, PIPE, R>), dim3(nb), dim3(BLOCK), lds, ctx->stream,
                        b->D, b->d_states, compact_dev(ct), r, jlo, x, y, chunk);
+}
+
+template <class R>
+static void launch_narrow_view(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
+                               const R &r, const uint16_t *jlo, const double2 *x, double2 *y,
+                               int block)
+{
+    const bool pipe = narrow_pipe(ct, r, block == 512 ? 512 : 1024);
+    if (block == 512) {
+        if (pipe) launch_narrow_block<512, true>(ctx, b, ct, r, jlo, x, y);
+        else launch_narrow_block<512, false>(ctx, b, ct, r, jlo, x, y);
+    } else {
+        if (pipe) launch_narrow_block<1024, true>(ctx, b, ct, r, jlo, x, y);
+        else launch_narrow_block<1024, false>(ctx, b, ct, r, jlo, x, y);
+    }
 }
 
 static void launch_narrow(edm_ctx *ctx, const edm_basis *b, const CompactTable &ct,
                           const double2 *x, double2 *y, int block)
 {
     RankNarrow r;
+    RankNarrow2 r2;
     const uint16_t *jlo = nullptr;
-    EDM_REQUIRE(narrow_view(b, r, jlo), EDM_EINVAL, "basis has no LDS rank split");
-    const bool pipe = narrow_pipe(ct, r, block == 512 ? 512 : 1024);
-    if (block == 512) {
-        if (pipe) launch_narrow_block<512, true>(ctx, b, ct, x, y);
-        else launch_narrow_block<512, false>(ctx, b, ct, x, y);
-    } else {
-        if (pipe) launch_narrow_block<1024, true>(ctx, b, ct, x, y);
-        else launch_narrow_block<1024, false>(ctx, b, ct, x, y);
-    }
+    if (narrow_view(b, r, jlo)) launch_narrow_view(ctx, b, ct, r, jlo, x, y, block);
+    else if (narrow2_view(b, r2, jlo)) launch_narrow_view(ctx, b, ct, r2, jlo, x, y, block);
+    else EDM_REQUIRE(false, EDM_EINVAL, "basis has no LDS rank split");
 }
 
 template <class Rank>
@@ -2288,7 +2478,7 @@ static void launch_compact(edm_ctx *ctx, const edm_basis *b, const CompactTable 
 static int bucketed_code(const edm_op *op, const edm_basis *b, ApplyMode mode,
                          const CompactTable &ct, int variant)
 {
-    const bool nok = narrow_ok(op, b, ct);
+    const bool nok = narrow_any_ok(op, b, ct);
     if (variant == 15 || variant == 16)
         return nok ? ((variant == 16 ? 512 : 1024) | EDM_TUNE_NARROW) : kLdsBlock;
     int block = default_kernel_code(b->D, nok);
@@ -2526,7 +2716,7 @@ bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, doubl
     // it (the batch engine's lone path; same row walk, so the same y): in the shared 512-row
     // launch its image and table would halve the occupancy of every problem
     if (matvec_kernel_code(op, b) & EDM_TUNE_NARROW &&
-        narrow_ok(op, b, ct) && b->D >= ((int64_t)1 << 18))
+        narrow_any_ok(op, b, ct) && b->D >= ((int64_t)1 << 18))
         return false;
     BatchMvDesc d{};
     d.D = b->D;
@@ -2728,7 +2918,7 @@ int matvec_tune(edm_ctx *ctx, const edm_op *op, const edm_basis *b, double *ms)
         EDM_HIP_CHECK(hipEventCreate(&e0));
         EDM_HIP_CHECK(hipEventCreate(&e1));
         const int blocks[3] = {256, 512, 1024};
-        const bool nok = narrow_ok(op, b, op->cfwd);
+        const bool nok = narrow_any_ok(op, b, op->cfwd);
         double bestms = 1e300;
         for (int k = 0; k < EDM_TUNE_CANDIDATES; ++k) {
             const bool amp_lds = k >= 3 && k < 6;

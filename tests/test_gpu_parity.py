@@ -736,8 +736,10 @@ def test_large_sector_enumeration_and_sampled_matvec(gpu_ctx, oracle_mod):
 def test_literal_batched_sector_48_bits(gpu_ctx, oracle_mod):
     """The batched configuration as written: two conserved components on a 4x6 mesh each (48
     orbitals), N_up = N_down = 4, sector (0,0) (about 4.7 M states). Its basis gets rank
-    tables (24 + 24 bits); H x is checked against the oracle on sampled rows and is bitwise
-    the same with the bucketed binary search (rank tables dropped)."""
+    tables (24 + 24 bits) and the two-level 14-bit split of the LDS rank-table kernel; H x is
+    checked against the oracle on sampled rows and is bitwise the same with the global-table
+    kernel, the plain walk, 512-row workgroups and the bucketed binary search (rank tables
+    dropped)."""
     import torch
     para = W.literal_batched_para(1.3)
     op = me.MBOperator(me.ED_scatterlist_twobody(para), upper_hermitian=True)
@@ -764,8 +766,25 @@ def test_literal_batched_sector_48_bits(gpu_ctx, oracle_mod):
     xd = torch.from_numpy(x).cuda()
     yd = torch.empty_like(xd)
     print("[48-bit] matvec with rank tables", flush=True)
-    dev.ROCmLinearMap(dop, b)(yd, xd)
+    H = dev.ROCmLinearMap(dop, b)
+    # the default: the LDS rank-table kernel on the two-level split (14-bit low table in LDS,
+    # the 34-bit high part as a 24-bit top table and 10-bit rows), bitwise the global-table
+    # kernel and its own plain walk
+    assert H.kernel() == 1024 | dev.TUNE_NARROW
+    H(yd, xd)
     y = yd.cpu().numpy()
+    y1 = torch.empty_like(xd)
+    for env in ({"EDM_NARROW_PIPE": "0"}, {"EDM_MATVEC": "u1", "EDM_MATVEC_LANES": "1"},
+                {"EDM_MATVEC": "u1n512"}):
+        os.environ.update(env)
+        try:
+            if "EDM_MATVEC_LANES" in env:
+                assert H.kernel() == 512
+            H(y1, xd)
+            assert torch.equal(y1, yd), env
+        finally:
+            for k in env:
+                del os.environ[k]
     full = _oracle_full(oracle_mod, op)
     print("[48-bit] oracle rows", flush=True)
     yo = oracle_mod.matvec(basis, full, x, rows=sample)
