@@ -1,6 +1,8 @@
 // edm_api.cpp — extern "C" entry points of libedm.so (declared in include/edm.h).
 #include "edm_internal.h"
 
+#include <new>
+
 #include <complex>
 
 #include <algorithm>
@@ -66,6 +68,8 @@ void verify_or_drop(edm_basis *b)
             build_narrow_tables(b);
         } catch (const EdmError &e) {
             if (e.code != EDM_ENOMEM) throw;
+            free_narrow_tables(b);
+        } catch (const std::bad_alloc &) {   // host tables of the two-level split
             free_narrow_tables(b);
         }
     } else {
