@@ -316,6 +316,21 @@ struct XJob {
 // Rows from which a shared-walk group gathers x from an interleaved copy of its vectors: below,
 // the vectors are small enough for the L2 and the copy would not pay. EDM_BATCH_INTERLEAVE_MIN
 // overrides (0: every group, -1: none).
+// Groups of this many rows or more get their own launch of 1,024-row workgroups, whose LDS
+// can hold the 14-bit rank split (the shared 512-row launch keeps three workgroups per CU, too
+// little LDS for it): 48-orbital sector, 8 ratios batched, 24.0 -> 21.1 s. EDM_BATCH_BIG_GROUP=0
+// keeps them in the shared launch; EDM_BATCH_BIG_GROUP_ROWS=n moves the threshold.
+static int64_t big_group_rows()
+{
+    const char *v = getenv("EDM_BATCH_BIG_GROUP_ROWS");
+    return v ? std::max<int64_t>(0, (int64_t)atoll(v)) : (int64_t)1 << 18;
+}
+static bool batch_big_groups()
+{
+    const char *v = getenv("EDM_BATCH_BIG_GROUP");
+    return !(v && atoi(v) == 0);
+}
+
 static int64_t batch_interleave_min()
 {
     const char *v = getenv("EDM_BATCH_INTERLEAVE_MIN");
@@ -340,6 +355,9 @@ struct edm_batch {
     // shared-walk matvec descriptors and block tables by group size (index 2..max group)
     std::vector<unsigned char> gdesc[8];
     std::vector<int> gblk[8];
+    // the same for groups of big_group_rows() rows and more (own launch, 1,024-row workgroups)
+    std::vector<unsigned char> gdesc_big[8];
+    std::vector<int> gblk_big[8];
     std::vector<size_t> coef_off, vtab_off;
     std::vector<BatchVecDesc> vd;
     int64_t steps = 0;
@@ -692,7 +710,12 @@ static int batch_step_once(edm_batch *bt)
     for (int g = 2; g <= maxg; ++g) {
         bt->gdesc[g].clear();
         bt->gblk[g].clear();
+        bt->gdesc_big[g].clear();
+        bt->gblk_big[g].clear();
     }
+    size_t lds_big[8] = {0};
+    const bool big_groups = batch_big_groups();
+    const int64_t big_rows = big_group_rows();
     std::vector<edm_problem *> solo;
     {
         std::map<std::pair<uint64_t, uint64_t>, std::vector<edm_problem *>> byshape;
@@ -747,15 +770,18 @@ static int batch_step_once(edm_batch *bt)
                 xi = (double2 *)((unsigned char *)bt->d_xi.p + xi_at);
                 xi_at += align_up((size_t)G * v[0]->D * sizeof(double2));
             }
-            std::vector<unsigned char> &dv = bt->gdesc[G];
-            std::vector<int> &bv = bt->gblk[G];
+            const bool big = big_groups && v[0]->D >= big_rows;
+            std::vector<unsigned char> &dv = big ? bt->gdesc_big[G] : bt->gdesc[G];
+            std::vector<int> &bv = big ? bt->gblk_big[G] : bt->gblk[G];
             const size_t at = dv.size();
             dv.resize(at + gsz);
             size_t l = 0;
             if (batch_mv_group_desc(ops, v[0]->b, xs, ys, G, (int64_t)bv.size(), dv.data() + at,
-                                    &l, xi)) {
-                bv.insert(bv.end(), (size_t)batch_mv_blocks(v[0]->D), (int)(at / gsz));
-                lds = std::max(lds, l);
+                                    &l, xi, big)) {
+                bv.insert(bv.end(), (size_t)batch_mv_blocks(v[0]->D, big ? 1024 : 512),
+                          (int)(at / gsz));
+                if (big) lds_big[G] = std::max(lds_big[G], l);
+                else lds = std::max(lds, l);
                 if (xi) {
                     XJob job{G, v[0]->D, {}, xi};
                     for (int g = 0; g < G; ++g) job.x[g] = xs[g];
@@ -775,7 +801,7 @@ static int batch_step_once(edm_batch *bt)
         if (batch_mv_desc(p->op, p->b, p->V[p->j], p->V[p->j + 1], blk0, mvdesc.data() + at,
                           &l)) {
             int pid = (int)(at / mvsz);
-            mvblk.insert(mvblk.end(), (size_t)batch_mv_blocks(p->D), pid);
+            mvblk.insert(mvblk.end(), (size_t)batch_mv_blocks(p->D, 512), pid);
             lds = std::max(lds, l);
         } else {
             mvdesc.resize(at);
@@ -784,8 +810,10 @@ static int batch_step_once(edm_batch *bt)
     }
     size_t mv_off = pl.put(mvdesc.data(), mvdesc.size());
     size_t mvblk_off = pl.put(mvblk.data(), mvblk.size() * sizeof(int));
-    size_t gdesc_off[8] = {0}, gblk_off[8] = {0};
+    size_t gdesc_off[8] = {0}, gblk_off[8] = {0}, gbdesc_off[8] = {0}, gbblk_off[8] = {0};
     for (int g = 2; g <= maxg; ++g) {
+        gbdesc_off[g] = pl.put(bt->gdesc_big[g].data(), bt->gdesc_big[g].size());
+        gbblk_off[g] = pl.put(bt->gblk_big[g].data(), bt->gblk_big[g].size() * sizeof(int));
         gdesc_off[g] = pl.put(bt->gdesc[g].data(), bt->gdesc[g].size());
         gblk_off[g] = pl.put(bt->gblk[g].data(), bt->gblk[g].size() * sizeof(int));
     }
@@ -843,7 +871,11 @@ static int batch_step_once(edm_batch *bt)
         for (int g = 2; g <= maxg; ++g)
             launch_batch_mv_group(ctx, g, dplan_base + gdesc_off[g],
                                   (const int *)(dplan_base + gblk_off[g]),
-                                  (int64_t)bt->gblk[g].size(), lds);
+                                  (int64_t)bt->gblk[g].size(), lds, 512);
+        for (int g = 2; g <= maxg; ++g)
+            launch_batch_mv_group(ctx, g, dplan_base + gbdesc_off[g],
+                                  (const int *)(dplan_base + gbblk_off[g]),
+                                  (int64_t)bt->gblk_big[g].size(), lds_big[g], 1024);
         const BatchVecDesc *dvd = (const BatchVecDesc *)(dplan_base + vd_off);
         bblas_cgs2_norm(ctx, dvd, n, maxvec, (const int *)(dplan_base + t2p_off),
                         (int64_t)tile2prob.size(), (const int *)(dplan_base + b2p_off),

@@ -363,7 +363,8 @@ void herm_eigh(int n, std::vector<std::complex<double>> &A, std::vector<double> 
 // descriptor (batch_mv_desc_size() bytes) for y = H x and returns false when the operator has
 // no LDS-staged table (the caller then runs launch_matvec for it); blocks are 512 rows.
 size_t batch_mv_desc_size();
-int64_t batch_mv_blocks(int64_t D);
+// workgroups of `block` rows (512 in the shared launches, 1,024 for big groups) over D rows
+int64_t batch_mv_blocks(int64_t D, int block);
 bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, double2 *y,
                    int64_t blk0, void *out, size_t *lds_bytes);
 void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, int64_t nblocks,
@@ -375,14 +376,16 @@ int batch_mv_max_group();
 size_t batch_mv_group_desc_size();
 bool batch_mv_same_shape(const edm_op *a, const edm_op *b);
 // xi: the group's x vectors interleaved by launch_batch_interleave (nullptr: read x[g])
+// big: the group has its own launch of 1,024-row workgroups (groups of 2^18 rows and more),
+// whose LDS may hold the 14-bit rank split of either form
 bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const double2 *const *x,
                          double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes,
-                         const double2 *xi);
+                         const double2 *xi, bool big);
 // xi[j * G + g] = x[g][j] for the D rows of G vectors (G = 2..batch_mv_max_group())
 void launch_batch_interleave(edm_ctx *ctx, int G, const double2 *const *x, int64_t D,
                              double2 *xi);
 void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,
-                           int64_t nblocks, size_t lds_bytes);
+                           int64_t nblocks, size_t lds_bytes, int block);
 // multi-problem Lanczos engine (edm_batch.cpp)
 edm_batch *batch_create(edm_ctx *ctx);
 void batch_destroy(edm_batch *bt);

@@ -1478,8 +1478,9 @@ struct BatchMvGroupDesc {
     RankLin lin;
     RankSearch search;
     RankNarrow nar;
+    RankNarrow2 nar2;
     const uint16_t *jlo_n;
-    int use_lin;   // as in BatchMvDesc
+    int use_lin;   // as in BatchMvDesc; 3 = the two-level split (RankNarrow2, jlo_n)
     int64_t blk0;
     const double2 *amp_pm[kMaxGroup];   // position-major amplitudes, [p * nkeys + key]
     const double2 *diag[kMaxGroup];
@@ -1490,12 +1491,15 @@ struct BatchMvGroupDesc {
     const double2 *xi;
 };
 
+// off_code: the codes' offset in the LDS image (the image is staged without its diagonal
+// table, which the walk reads from each problem's own global table)
 template <class Rank, int G, bool Inter>
 __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned char *base,
-                                          const BatchMvGroupDesc &P, const Rank &rank)
+                                          int off_code, const BatchMvGroupDesc &P,
+                                          const Rank &rank)
 {
     const uint16_t *s_keyptr = (const uint16_t *)base;
-    const uint16_t *s_code = (const uint16_t *)(base + P.t.off_code);
+    const uint16_t *s_code = (const uint16_t *)(base + off_code);
     double yr[G], yi[G], dr[G], di[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) yr[g] = yi[g] = dr[g] = di[g] = 0.0;
@@ -1569,30 +1573,39 @@ __device__ __forceinline__ void row_group(int64_t i, uint64_t s, const unsigned 
     }
 }
 
-template <int G>
-__global__ void __launch_bounds__(kLdsBlock)
+template <int G, bool Inter>
+__device__ __forceinline__ void group_rows(int64_t i, const unsigned char *base, int off_code,
+                                           const BatchMvGroupDesc &P, const RankNarrow &nar,
+                                           const RankNarrow2 &nar2)
+{
+    if (P.use_lin == 3) row_group<RankNarrow2, G, Inter>(i, P.states[i], base, off_code, P, nar2);
+    else if (P.use_lin == 2) row_group<RankNarrow, G, Inter>(i, P.states[i], base, off_code, P, nar);
+    else if (P.use_lin) row_group<RankLin, G, Inter>(i, P.states[i], base, off_code, P, P.lin);
+    else row_group<RankSearch, G, Inter>(i, P.states[i], base, off_code, P, P.search);
+}
+
+// BLOCK rows per workgroup: 512 in the step's shared launch, 1,024 for groups of 2^18 rows and
+// more, which have their own launch (their LDS rank tables and 32 waves per CU)
+template <int G, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
 k_batch_mv_group(const BatchMvGroupDesc *__restrict__ descs, const int *__restrict__ blk2grp,
                  int chunk)
 {
     extern __shared__ uint4 lds_u4[];
     const int64_t blk = xcd_chunk_block(chunk);
     const BatchMvGroupDesc &P = descs[blk2grp[blk]];
-    stage_image(P.t, lds_u4);
+    stage_image_nodiag(P.t, lds_u4);
+    const CompactDev tl = image_nodiag(P.t);
     RankNarrow nar = P.nar;
-    if (P.use_lin == 2) nar.sjlo = stage_narrow(P.t, nar, P.jlo_n, lds_u4);
+    RankNarrow2 nar2 = P.nar2;
+    if (P.use_lin == 2) nar.sjlo = stage_narrow(tl, nar, P.jlo_n, lds_u4);
+    if (P.use_lin == 3) nar2.sjlo = stage_narrow(tl, nar2, P.jlo_n, lds_u4);
     __syncthreads();
-    const int64_t i = (blk - P.blk0) * blockDim.x + threadIdx.x;
+    const int64_t i = (blk - P.blk0) * BLOCK + threadIdx.x;
     if (i >= P.D) return;
     const unsigned char *base = (const unsigned char *)lds_u4;
-    if (P.xi) {
-        if (P.use_lin == 2) row_group<RankNarrow, G, true>(i, P.states[i], base, P, nar);
-        else if (P.use_lin) row_group<RankLin, G, true>(i, P.states[i], base, P, P.lin);
-        else row_group<RankSearch, G, true>(i, P.states[i], base, P, P.search);
-    } else {
-        if (P.use_lin == 2) row_group<RankNarrow, G, false>(i, P.states[i], base, P, nar);
-        else if (P.use_lin) row_group<RankLin, G, false>(i, P.states[i], base, P, P.lin);
-        else row_group<RankSearch, G, false>(i, P.states[i], base, P, P.search);
-    }
+    if (P.xi) group_rows<G, true>(i, base, tl.off_code, P, nar, nar2);
+    else group_rows<G, false>(i, base, tl.off_code, P, nar, nar2);
 }
 
 // ---------------------------------------------------------------- eigenvector consumers
@@ -2704,7 +2717,7 @@ static void batch_lds_optin(edm_ctx *ctx, const void *kernel, std::atomic<uint64
 
 size_t batch_mv_desc_size() { return sizeof(BatchMvDesc); }
 
-int64_t batch_mv_blocks(int64_t D) { return (D + kLdsBlock - 1) / kLdsBlock; }
+int64_t batch_mv_blocks(int64_t D, int block) { return (D + block - 1) / block; }
 
 bool batch_mv_desc(const edm_op *op, const edm_basis *b, const double2 *x, double2 *y,
                    int64_t blk0, void *out, size_t *lds_bytes)
@@ -2763,7 +2776,7 @@ bool batch_mv_same_shape(const edm_op *a, const edm_op *b)
 
 bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const double2 *const *x,
                          double2 *const *y, int G, int64_t blk0, void *out, size_t *lds_bytes,
-                         const double2 *xi)
+                         const double2 *xi, bool big)
 {
     EDM_REQUIRE(G >= 2 && G <= kMaxGroup, EDM_EINVAL, "group size out of range");
     if (matvec_variant() != 11 || matvec_passes() != 1) return false;
@@ -2790,11 +2803,21 @@ bool batch_mv_group_desc(const edm_op *const *ops, const edm_basis *b, const dou
         d.y[g] = y[g];
     }
     const CompactTable &ct = ops[0]->cfwd;
-    *lds_bytes = ct.lds_bytes;
+    // the LDS image leaves out the diagonal table (row_group reads each problem's own)
+    const size_t img = ((size_t)image_nodiag(d.t).lds_bytes + 15) & ~(size_t)15;
+    const size_t jlo_bytes = ((size_t)1 << kNarrowLo) * sizeof(uint16_t);
+    // a shared launch keeps three 512-row workgroups per CU; a big group's own launch two
+    // 1,024-row ones
+    const size_t cap = big ? kLdsMaxBytes / 2 : kBatchLdsMax;
+    *lds_bytes = img;
     if (batch_narrow_enabled() && lin && narrow_ok(ops[0], b, ct) &&
-        narrow_view(b, d.nar, d.jlo_n) && narrow_lds_bytes(ct, d.nar) <= kBatchLdsMax) {
+        narrow_view(b, d.nar, d.jlo_n) && img + ((size_t)d.nar.lomask + 1) * 2 <= cap) {
         d.use_lin = 2;
-        *lds_bytes = narrow_lds_bytes(ct, d.nar);
+        *lds_bytes = img + ((size_t)d.nar.lomask + 1) * sizeof(uint16_t);
+    } else if (big && batch_narrow_enabled() && lin && narrow2_ok(ops[0], b, ct) &&
+               narrow2_view(b, d.nar2, d.jlo_n) && img + jlo_bytes <= cap) {
+        d.use_lin = 3;
+        *lds_bytes = img + jlo_bytes;
     }
     memcpy(out, &d, sizeof(d));
     return true;
@@ -2828,30 +2851,30 @@ void launch_batch_interleave(edm_ctx *ctx, int G, const double2 *const *x, int64
     EDM_HIP_CHECK(hipGetLastError());
 }
 
-void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,
+template <int G, int BLOCK>
+static void launch_group_g(edm_ctx *ctx, const BatchMvGroupDesc *dd, const int *d_blk2grp,
                            int64_t nblocks, size_t lds_bytes)
 {
+    static std::atomic<uint64_t> optin{0};
+    batch_lds_optin(ctx, (const void *)k_batch_mv_group<G, BLOCK>, optin, lds_bytes);
+    hipLaunchKernelGGL((k_batch_mv_group<G, BLOCK>), dim3((unsigned)nblocks), dim3(BLOCK),
+                       lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
+}
+
+void launch_batch_mv_group(edm_ctx *ctx, int G, const void *d_descs, const int *d_blk2grp,
+                           int64_t nblocks, size_t lds_bytes, int block)
+{
     if (nblocks == 0) return;
+    EDM_REQUIRE(block == kLdsBlock || block == 1024, EDM_EINVAL, "group workgroup size");
     const auto *dd = (const BatchMvGroupDesc *)d_descs;
-    static std::atomic<uint64_t> optin[kMaxGroup + 1];
-    batch_lds_optin(ctx,
-                    G == 2 ? (const void *)k_batch_mv_group<2>
-                           : G == 3 ? (const void *)k_batch_mv_group<3>
-                                    : (const void *)k_batch_mv_group<4>,
-                    optin[G == 2 ? 2 : G == 3 ? 3 : 4], lds_bytes);
-    switch (G) {
-    case 2:
-        hipLaunchKernelGGL(k_batch_mv_group<2>, dim3((unsigned)nblocks), dim3(kLdsBlock),
-                           lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
-        break;
-    case 3:
-        hipLaunchKernelGGL(k_batch_mv_group<3>, dim3((unsigned)nblocks), dim3(kLdsBlock),
-                           lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
-        break;
-    default:
-        hipLaunchKernelGGL(k_batch_mv_group<4>, dim3((unsigned)nblocks), dim3(kLdsBlock),
-                           lds_bytes, ctx->stream, dd, d_blk2grp, batch_chunk());
-        break;
+    if (block == 1024) {
+        if (G == 2) launch_group_g<2, 1024>(ctx, dd, d_blk2grp, nblocks, lds_bytes);
+        else if (G == 3) launch_group_g<3, 1024>(ctx, dd, d_blk2grp, nblocks, lds_bytes);
+        else launch_group_g<4, 1024>(ctx, dd, d_blk2grp, nblocks, lds_bytes);
+    } else {
+        if (G == 2) launch_group_g<2, kLdsBlock>(ctx, dd, d_blk2grp, nblocks, lds_bytes);
+        else if (G == 3) launch_group_g<3, kLdsBlock>(ctx, dd, d_blk2grp, nblocks, lds_bytes);
+        else launch_group_g<4, kLdsBlock>(ctx, dd, d_blk2grp, nblocks, lds_bytes);
     }
     EDM_HIP_CHECK(hipGetLastError());
 }

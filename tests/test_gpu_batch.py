@@ -113,6 +113,41 @@ def test_batch_shared_walk_groups_bitwise_equal_alone(gpu_ctx, monkeypatch, inte
         assert r.info["matvecs"] == info["matvecs"], label
 
 
+def test_batch_big_groups_bitwise_equal_alone(gpu_ctx, monkeypatch):
+    """Groups of 2^18 rows and more walk their rows in a launch of their own (1,024-row
+    workgroups, the 14-bit LDS rank split in either form): four ratios of the 48-orbital
+    two-component sector (D = 4.7 M, the two-level split), then small problems forced through
+    the same launch; every problem's values, vectors and matvec count are its lone solve's."""
+    para0 = W.literal_batched_para(1.0)
+    sysd = dev.system_struct(para0.k_list, para0.Gk, para0.Nc_mix, para0.Nc_conserve, (4, 4))
+    b = dev.DeviceBasis.enumerate(gpu_ctx, sysd, 0)
+    maps = []
+    for r in (0.6, 0.9, 1.3, 1.7):
+        pr = W.literal_batched_para(r)
+        op = me.MBOperator(me.ED_scatterlist_twobody(pr), upper_hermitian=True)
+        maps.append(dev.ROCmLinearMap(dev.DeviceOperator.from_mboperator(gpu_ctx, op, pr.bits), b))
+    kw = dict(krylovdim=16, tol=1e-6)
+    print(f"[big-group] 48-orbital sector D={b.D}: batch of 4", flush=True)
+    with B.BatchSolver(gpu_ctx) as bs:
+        ids = [bs.add(m, 2, **kw) for m in maps]
+        res = bs.run()
+    for m, pid in zip(maps, ids):
+        vals, _, info = dev.krylov_map_solve(m, 2, want_vectors=False, **kw)
+        np.testing.assert_array_equal(res[pid].values, vals)
+        assert res[pid].info["matvecs"] == info["matvecs"]
+    monkeypatch.setenv("EDM_BATCH_BIG_GROUP_ROWS", "0")
+    probs = _two_component_problems(gpu_ctx, [0.55, 0.7, 0.85, 1.1, 1.25], [(3, 4)], [0, 5])
+    print("[big-group] small problems through the big-group launch", flush=True)
+    with B.BatchSolver(gpu_ctx) as bs:
+        ids = [bs.add(m, W.BATCH_NEV, want_vectors=True) for m, _, _ in probs]
+        res = bs.run()
+    for (m, label, _), pid in zip(probs, ids):
+        vals, vecs, info = _alone(m, W.BATCH_NEV)
+        np.testing.assert_array_equal(res[pid].values, vals)
+        np.testing.assert_array_equal(np.array(res[pid].vectors), vecs)
+        assert res[pid].info["matvecs"] == info["matvecs"], label
+
+
 def test_batch_continuous_admission_and_removal(gpu_ctx):
     """Problems admitted while others are mid-solve, one removed mid-run: every collected
     result equals its solve alone."""
