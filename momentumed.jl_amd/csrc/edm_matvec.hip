@@ -2762,7 +2762,12 @@ void launch_batch_mv(edm_ctx *ctx, const void *d_descs, const int *d_blk2prob, i
     EDM_HIP_CHECK(hipGetLastError());
 }
 
-int batch_mv_max_group() { return kMaxGroup; }
+int batch_mv_max_group()
+{
+    const char *v = getenv("EDM_BATCH_MAX_GROUP");
+    const int g = v ? atoi(v) : kMaxGroup;
+    return g < 1 ? 1 : (g > kMaxGroup ? kMaxGroup : g);
+}
 
 size_t batch_mv_group_desc_size() { return sizeof(BatchMvGroupDesc); }
 
