@@ -187,9 +187,10 @@ int edm_connectivity_totals(edm_ctx *ctx, const edm_op *op, const edm_basis *b, 
  * kernel with 256-, 512- and 1024-row workgroups, the variant that also stages the (distinct)
  * amplitudes in LDS at the same three sizes when they fit, the split-row variant with 2, 4, 8
  * and 16 lanes per row (512-lane workgroups), and, when the basis has a rank split whose low
- * table (at most 2^14 16-bit entries) fits in LDS beside the term image (built at upload for
- * complete sectors of 2^18 states or more: a dense high table up to 38 bits, a two-level one
- * from 39 to 50 bits), the variant that reads the low-half rank table from LDS (and gathers x
+ * table (at most 2^14 16-bit entries) fits in LDS beside the term image (built at upload
+ * beside the rank tables of complete sectors: a dense high table up to 38 bits when it stays
+ * within the basis' own size, a two-level one from 39 to 50 bits for 2^18 states or more),
+ * the variant that reads the low-half rank table from LDS (and gathers x
  * into LDS while its walk goes on) with 512- and 1024-row workgroups; it uses the
  * fastest for every later edm_matvec / edm_eigsolve on the pair. The per-row arithmetic and
  * its order are the same in all twelve, so y does not change. *kernel = chosen code: rows per
