@@ -808,6 +808,94 @@ def test_literal_batched_sector_48_bits(gpu_ctx, oracle_mod):
     assert np.abs(y1 - yo1).max() <= Y_TOL * np.abs(yo1).max()
 
 
+def _random_conserving_op(rng, k, dims, ncomp, n2=240, n1=12):
+    """A general (not Hermitian) operator of random complex two-body terms that conserve
+    momentum (k_a + k_b = k_c + k_d on the mesh) and each component's particle number, plus
+    number operators with complex amplitudes; orbital comp * nk + i has momentum k[:, i]."""
+    nk = k.shape[1]
+    dims = np.asarray(dims)
+    where = {tuple(k[:, i] % dims): i for i in range(nk)}
+    bits = ncomp * nk
+    terms = []
+    while len(terms) < n2:
+        c1, c2 = rng.integers(ncomp, size=2)
+        i1, i2, o1 = rng.integers(nk, size=3)
+        o2 = where[tuple((k[:, i1] + k[:, i2] - k[:, o1]) % dims)]
+        d, c = c1 * nk + i1, c2 * nk + i2
+        a, b_ = c1 * nk + o1, c2 * nk + o2
+        if a == b_ or c == d:
+            continue
+        terms.append(me.Scatter(complex(rng.normal(), rng.normal()), int(a) + 1, int(b_) + 1,
+                                int(c) + 1, int(d) + 1, bits=bits))
+    for a in rng.choice(bits, n1, replace=False):
+        terms.append(me.Scatter(complex(rng.normal(), rng.normal()), int(a) + 1, int(a) + 1,
+                                bits=bits))
+    return me.MBOperator(terms, upper_hermitian=False)
+
+
+@pytest.mark.parametrize("case", ["39 bits", "50 bits"])
+def test_two_level_rank_split_edges(gpu_ctx, oracle_mod, monkeypatch, case):
+    """The two-level split of the LDS rank-table kernel at both ends of its range: a 39-bit
+    basis (3x13 mesh, N = 7: rows of 2 words under the 24-bit top table) and a 50-bit one (two
+    conserved components on 5x5 meshes, N = 6 + 3: rows of 4,096 words). A random momentum-
+    and component-conserving general operator; y of the one-lane global-table kernel is the
+    oracle's to 1e-13 on sampled rows, and the LDS rank-table kernel (1024- and 512-row
+    workgroups; pipelined, plain and forced-pipelined walk), the block kernel and the bucketed
+    binary search give it bit for bit."""
+    import torch
+    if case == "39 bits":
+        k, dims, ncomp, n_each = W.mesh(3, 13), (3, 13), 1, (7,)
+    else:
+        k, dims, ncomp, n_each = W.mesh(5, 5), (5, 5), 2, (6, 3)
+    nbits = ncomp * k.shape[1]
+    op = _random_conserving_op(np.random.default_rng(nbits), k, dims, ncomp)
+    print(f"[split edges] {case}: enumerate", flush=True)
+    b = dev.DeviceBasis.enumerate(gpu_ctx, dev.system_struct(k, dims, 1, ncomp, n_each), 0)
+    assert b.rank_mode == 1 and b.D >= 1 << 18, (b.rank_mode, b.D)
+    d = dev.DeviceOperator.from_mboperator(gpu_ctx, op, nbits)
+    H = dev.ROCmLinearMap(d, b)
+    x = _rand(b.D, 4000 + nbits)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    monkeypatch.setenv("EDM_MATVEC", "u1")
+    monkeypatch.setenv("EDM_MATVEC_LANES", "1")
+    assert H.kernel() & 0xFFFF == 512 and not H.kernel() & dev.TUNE_NARROW
+    H(yd, xd)
+    monkeypatch.delenv("EDM_MATVEC_LANES")
+    y = yd.cpu().numpy()
+    basis = b.download()
+    rows = np.unique(np.concatenate([np.arange(64), np.arange(b.D - 64, b.D),
+                                     np.random.default_rng(5).integers(0, b.D, 4000)]))
+    print(f"[split edges] {case}: D={b.D}, oracle rows", flush=True)
+    yo = oracle_mod.matvec(basis, _oracle_full(oracle_mod, op), x, rows=rows)
+    assert np.count_nonzero(yo) > len(rows) // 2
+    assert np.abs(y[rows] - yo).max() <= Y_TOL * np.abs(yo).max()
+    y1 = torch.empty_like(xd)
+    for v, blk in (("u1n", 1024), ("u1n512", 512)):
+        monkeypatch.setenv("EDM_MATVEC", v)
+        assert H.kernel() == blk | dev.TUNE_NARROW, (case, v, hex(H.kernel()))
+        for pipe in (None, "0", "2"):
+            if pipe is None:
+                monkeypatch.delenv("EDM_NARROW_PIPE", raising=False)
+            else:
+                monkeypatch.setenv("EDM_NARROW_PIPE", pipe)
+            H(y1, xd)
+            assert torch.equal(y1, yd), (case, v, pipe)
+        monkeypatch.delenv("EDM_NARROW_PIPE")
+    monkeypatch.setenv("EDM_MATVEC", "u1")
+    xs = [np.ascontiguousarray(x * (v + 1) + _rand(b.D, 77 + v)) for v in range(2)]
+    yb = _gpu_matvec_block(gpu_ctx, d, b, xs)
+    for v in range(2):
+        np.testing.assert_array_equal(yb[v], _gpu_matvec(gpu_ctx, d, b, xs[v]),
+                                      err_msg=f"{case} block col {v}")
+    monkeypatch.delenv("EDM_MATVEC")
+    b.drop_rank()
+    assert b.rank_mode == 0
+    print(f"[split edges] {case}: bucketed binary search", flush=True)
+    dev.ROCmLinearMap(d, b)(y1, xd)
+    assert torch.equal(y1, yd)
+
+
 # ------------------------------------------------------------------ general eigensolver
 def _random_general_op(rng, bits, n2=60, n1=20):
     terms = []
