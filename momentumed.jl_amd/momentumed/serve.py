@@ -217,6 +217,10 @@ class SolveServer:
         rank = (1 << ((nbits + 1) // 2)) * 6 + (1 << nhi) * 4
         if 14 < nbits <= 38:   # the split at bit 14 the LDS rank-table matvec reads
             rank += (4 << (nbits - 14)) + (2 << 14)
+        elif 38 < nbits <= 50 and D >= 1 << 18:
+            # the same split with J_hi in two levels: a 24-bit top table and at most 128 MB
+            # of rows of 2^(nbits - 38) words, one per distinct top
+            rank += (4 << 24) + (2 << 14) + min(128 << 20, D * (4 << (nbits - 38)))
         ws = problem_bytes(D, req.nev, self.kw["krylovdim"]) if self.mode == "batched" else \
             (2 * min(max(self.kw["krylovdim"], req.nev + 2), D) + 1) * D * 16
         return ws + D * 8 + rank + (4 << 20)
