@@ -400,10 +400,7 @@ static void single_cgs2(edm_ctx *ctx, edm_problem *p, int n, double2 *w, std::ve
     EDM_HIP_CHECK(hipMemcpyAsync(p->d_vptrs, p->V.data(), p->V.size() * sizeof(double2 *),
                                  hipMemcpyHostToDevice, ctx->stream));
     std::vector<double2> h2(n);
-    blas_multidot(ctx, p->d_vptrs, n, w, p->D, p->d_h);
-    blas_multiaxpy(ctx, p->d_vptrs, n, p->d_h, w, p->D);
-    blas_multidot(ctx, p->d_vptrs, n, w, p->D, p->d_h2);
-    blas_multiaxpy(ctx, p->d_vptrs, n, p->d_h2, w, p->D);
+    blas_cgs2(ctx, p->d_vptrs, n, w, p->D, p->d_h, p->d_h2);
     EDM_HIP_CHECK(hipMemcpyAsync(h.data(), p->d_h, n * sizeof(double2), hipMemcpyDeviceToHost,
                                  ctx->stream));
     EDM_HIP_CHECK(hipMemcpyAsync(h2.data(), p->d_h2, n * sizeof(double2), hipMemcpyDeviceToHost,

@@ -23,6 +23,27 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// The complex products of the Gram-Schmidt kernels with their rounding fixed in the source
+// (one product of each component rounded, the other fused into an explicit fma, no
+// contraction left to the compiler, which may fuse either product depending on the code
+// around it), so the single-problem and the batched kernels get the same bits for the same
+// operands.
+// acc -= a * c
+__device__ __forceinline__ void cmul_sub(double2 &acc, double2 a, double2 c)
+{
+#pragma clang fp contract(off)
+    acc.x -= fma(a.x, c.x, -(a.y * c.y));
+    acc.y -= fma(a.y, c.x, a.x * c.y);
+}
+
+// (ar, ai) += conj(a) * w
+__device__ __forceinline__ void cdot_acc(double &ar, double &ai, double2 a, double2 w)
+{
+#pragma clang fp contract(off)
+    ar += fma(a.y, w.y, a.x * w.x);
+    ai += fma(a.x, w.y, -(a.y * w.x));
+}
+
 // block-wide sum of (a, b); result valid in thread 0
 __device__ __forceinline__ double2 block_sum2(double a, double b, double2 *sh)
 {
@@ -88,10 +109,7 @@ __device__ __forceinline__ void multidot_tile(const double2 *const *__restrict__
             ai[g] = 0.0;
 #pragma unroll
             for (int k = 0; k < kRpt; ++k) {
-                if (in[k]) {
-                    ar[g] += a[g][k].x * wv[k].x + a[g][k].y * wv[k].y;
-                    ai[g] += a[g][k].x * wv[k].y - a[g][k].y * wv[k].x;
-                }
+                if (in[k]) cdot_acc(ar[g], ai[g], a[g][k], wv[k]);
             }
             ar[g] = wave_sum(ar[g]);
             ai[g] = wave_sum(ai[g]);
@@ -147,12 +165,7 @@ k_multiaxpy(const double2 *const *__restrict__ vptrs, int nvec, const double2 *_
     int64_t r = (int64_t)blockIdx.x * kBlk + threadIdx.x;
     if (r >= D) return;
     double2 acc = w[r];
-    for (int i = 0; i < nvec; ++i) {
-        double2 a = vptrs[i][r];
-        double2 c = hs[i];
-        acc.x -= a.x * c.x - a.y * c.y;
-        acc.y -= a.x * c.y + a.y * c.x;
-    }
+    for (int i = 0; i < nvec; ++i) cmul_sub(acc, vptrs[i][r], hs[i]);
     w[r] = acc;
 }
 
@@ -318,12 +331,7 @@ k_baxpy(const BatchVecDesc *__restrict__ descs, const int *__restrict__ blk2prob
     int64_t r = (bx - P.blk0) * kBlk + threadIdx.x;
     if (r >= P.D) return;
     double2 acc = P.w[r];
-    for (int i = 0; i < P.nvec; ++i) {
-        double2 a = P.V[i][r];
-        double2 h = hs[i];
-        acc.x -= a.x * h.x - a.y * h.y;
-        acc.y -= a.x * h.y + a.y * h.x;
-    }
+    for (int i = 0; i < P.nvec; ++i) cmul_sub(acc, P.V[i][r], hs[i]);
     P.w[r] = acc;
 }
 
@@ -433,6 +441,15 @@ void blas_multiaxpy(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const
     hipLaunchKernelGGL(k_multiaxpy, dim3(rows_blocks(D)), dim3(kBlk), 0, ctx->stream, d_vptrs,
                        nvec, d_h, w, D);
     EDM_HIP_CHECK(hipGetLastError());
+}
+
+void blas_cgs2(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, double2 *w, int64_t D,
+               double2 *d_h, double2 *d_h2)
+{
+    blas_multidot(ctx, d_vptrs, nvec, w, D, d_h);
+    blas_multiaxpy(ctx, d_vptrs, nvec, d_h, w, D);
+    blas_multidot(ctx, d_vptrs, nvec, w, D, d_h2);
+    blas_multiaxpy(ctx, d_vptrs, nvec, d_h2, w, D);
 }
 
 void blas_norm(edm_ctx *ctx, const double2 *w, int64_t D, double *d_out)

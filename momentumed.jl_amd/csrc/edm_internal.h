@@ -410,6 +410,10 @@ void blas_multidot(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const 
 // w -= sum_i V_i h_i (h on device)
 void blas_multiaxpy(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, const double2 *d_h,
                     double2 *w, int64_t D);
+// both passes of classical Gram-Schmidt with one re-orthogonalisation: h = V^H w,
+// w -= V h, h2 = V^H w, w -= V h2
+void blas_cgs2(edm_ctx *ctx, const double2 *const *d_vptrs, int nvec, double2 *w, int64_t D,
+               double2 *d_h, double2 *d_h2);
 // d_out[0] = ||w||_2
 void blas_norm(edm_ctx *ctx, const double2 *w, int64_t D, double *d_out);
 // v = w / (*d_beta)

@@ -356,10 +356,7 @@ void cgs2(Workspace &ws, int n, double2 *target, std::vector<double2> &h_out)
     h_out.assign(n, make_double2(0.0, 0.0));
     if (n == 0) return;
     std::vector<double2> h2(n);
-    blas_multidot(ctx, ws.d_vptrs, n, target, ws.D, ws.d_h);
-    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h, target, ws.D);
-    blas_multidot(ctx, ws.d_vptrs, n, target, ws.D, ws.d_h2);
-    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h2, target, ws.D);
+    blas_cgs2(ctx, ws.d_vptrs, n, target, ws.D, ws.d_h, ws.d_h2);
     EDM_HIP_CHECK(hipMemcpyAsync(h_out.data(), ws.d_h, n * sizeof(double2), hipMemcpyDeviceToHost,
                                  ctx->stream));
     EDM_HIP_CHECK(hipMemcpyAsync(h2.data(), ws.d_h2, n * sizeof(double2), hipMemcpyDeviceToHost,
@@ -376,10 +373,7 @@ void cgs2(Workspace &ws, int n, double2 *target, std::vector<double2> &h_out)
 void queue_cgs2_norm_scale(Workspace &ws, int n, double2 *w, int slot)
 {
     edm_ctx *ctx = ws.ctx;
-    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.d_h);
-    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h, w, ws.D);
-    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.d_h2);
-    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.d_h2, w, ws.D);
+    blas_cgs2(ctx, ws.d_vptrs, n, w, ws.D, ws.d_h, ws.d_h2);
     blas_norm(ctx, w, ws.D, ws.d_beta);
     blas_scale_inv(ctx, w, w, ws.d_beta, ws.D);
     // h, h2 and beta are adjacent on the device (d_h2 = d_h + m + 1, d_beta follows). Work
@@ -643,10 +637,7 @@ void queue_column(BlockWorkspace &ws, int c, int n)
 {
     edm_ctx *ctx = ws.ctx;
     double2 *w = ws.V[n];
-    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.h(c));
-    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.h(c), w, ws.D);
-    blas_multidot(ctx, ws.d_vptrs, n, w, ws.D, ws.h2(c));
-    blas_multiaxpy(ctx, ws.d_vptrs, n, ws.h2(c), w, ws.D);
+    blas_cgs2(ctx, ws.d_vptrs, n, w, ws.D, ws.h(c), ws.h2(c));
     blas_norm(ctx, w, ws.D, ws.beta(c));
 }
 
